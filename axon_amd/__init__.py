@@ -1,0 +1,293 @@
+"""axon_amd -- Python mirror of the MI355X AXON engine's C ABI (libaxon.so).
+
+The engine itself is C++/HIP (``axon_amd/csrc``); this module only binds the
+C ABI declared in ``include/axon_api.h`` with ctypes, so tests and the bench
+drive exactly the entry points a C or Objective-C++ caller would.  There is no
+fallback: if ``libaxon.so`` is missing or no gfx950 device is usable, calls
+raise ``AxonError``.
+
+The reference's interface for this path is its key=value CLI and the
+``present(p, keep, T, clampSteps, learn, freeze, seed, useBias)`` routine of
+src/axon.mm (mm:660-667, mm:815-888); ``Config.parse`` and ``Net.present``
+keep those argument names and meanings.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaxon.so")
+CLI_PATH = os.path.join(HERE, "bin", "axon")
+
+CONFIG_FIELDS = (
+    "ncol incols nand target k dthr boost mode "
+    "epochs T clampsteps stim seed quiet split unionchar "
+    "biasup biasdown biasshift biasdownevery inbr stimcols diag profile "
+    "topkpar vtopk contrast bandlo bandhi margingate hetero vbs "
+    "ltdrep ltdevery ktest facil deep wave wavet trainclamp max_t"
+).split()
+
+# Configuration C of the paper (README.md:45-47 of the reference).
+CONFIG_C = (
+    "nand=4 target=64 k=24 dthr=1 mode=1 inbr=4 stimcols=64 contrast=1 margingate=0 "
+    "ltdrep=1 ltdevery=2 biasup=73 biasdown=1 T=10 clampsteps=5"
+).split()
+
+
+class AxonError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"axon status {status}: {message}")
+        self.status = status
+
+
+class ConfigStruct(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint32) for f in CONFIG_FIELDS] + [("reserved", ctypes.c_uint32 * 7)]
+
+
+class PresentArgs(ctypes.Structure):
+    _fields_ = [
+        ("pattern", ctypes.c_uint32), ("T", ctypes.c_uint32), ("clamp_steps", ctypes.c_uint32),
+        ("learn", ctypes.c_uint32), ("freeze", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+        ("use_bias", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("keep", ctypes.c_double),
+        ("stim", ctypes.POINTER(ctypes.c_uint32)),
+    ]
+
+
+U32P = ctypes.POINTER(ctypes.c_uint32)
+
+
+class PresentOut(ctypes.Structure):
+    _fields_ = [
+        ("spikes", U32P), ("hist", U32P), ("stats", U32P), ("drive_log", U32P), ("theta", U32P),
+        ("ms_per_step", ctypes.c_double),
+    ]
+
+
+BUFFERS = ("W0 W1 W2 W3 CONN SPK_A SPK_B DRIVE BBITS ACC BIAS MARG REFBALL DRIVELOG MULT "
+           "STIM STATS HIST THETA").split()
+PRESENT_ASYNC = 1
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libaxon.so (built by __graft_entry__.build / axon_amd/Makefile)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AxonError(-1, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.axon_status_str.restype = ctypes.c_char_p
+        L.axon_last_error.restype = ctypes.c_char_p
+        L.axon_abi_version.restype = ctypes.c_uint32
+        L.axon_config_defaults.restype = None
+        L.axon_net_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        L = lib()
+        raise AxonError(status, f"{L.axon_status_str(status).decode()}: {L.axon_last_error().decode()}")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(U32P)
+
+
+@dataclass
+class Config:
+    """All key=value tunables of the reference CLI (mm:687-735)."""
+
+    raw: ConfigStruct
+
+    @classmethod
+    def parse(cls, args: Sequence[str]) -> "Config":
+        """argu() semantics: prefix match + '=', last occurrence wins (mm:660-667)."""
+        argv = ["axon"] + list(args)
+        arr = (ctypes.c_char_p * len(argv))(*[a.encode() for a in argv])
+        c = ConfigStruct()
+        _check(lib().axon_config_parse(len(argv), arr, ctypes.byref(c)))
+        return cls(c)
+
+    @classmethod
+    def defaults(cls) -> "Config":
+        c = ConfigStruct()
+        lib().axon_config_defaults(ctypes.byref(c))
+        return cls(c)
+
+    def validate(self) -> None:
+        _check(lib().axon_config_validate(ctypes.byref(self.raw)))
+
+    def __getattr__(self, name):
+        if name in CONFIG_FIELDS:
+            return getattr(self.raw, name)
+        raise AttributeError(name)
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self.raw, f) for f in CONFIG_FIELDS}
+
+
+@dataclass
+class PresentResult:
+    spikes: np.ndarray
+    hist: np.ndarray
+    stats: np.ndarray
+    drive_log: np.ndarray
+    theta: np.ndarray
+    ms_per_step: float
+
+
+class Net:
+    """One network resident in HBM (planes, connectivity, bias, stimuli)."""
+
+    def __init__(self, config: Config):
+        self.config = config
+        self._h = ctypes.c_void_p()
+        _check(lib().axon_net_create(ctypes.byref(config.raw), ctypes.byref(self._h)))
+        self.ncol = config.ncol
+        self.n = config.ncol * 32
+
+    def close(self) -> None:
+        if self._h:
+            lib().axon_net_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def words(self, which: str) -> int:
+        w = ctypes.c_size_t()
+        _check(lib().axon_buffer_words(self._h, BUFFERS.index(which), ctypes.byref(w)))
+        return w.value
+
+    def read(self, which: str, nwords: Optional[int] = None) -> np.ndarray:
+        n = self.words(which) if nwords is None else nwords
+        out = np.empty(n, dtype=np.uint32)
+        _check(lib().axon_read_buffer(self._h, BUFFERS.index(which), out.ctypes.data_as(ctypes.c_void_p),
+                                      ctypes.c_size_t(n * 4)))
+        return out
+
+    def write(self, which: str, data: np.ndarray) -> None:
+        data = np.ascontiguousarray(data, dtype=np.uint32)
+        _check(lib().axon_write_buffer(self._h, BUFFERS.index(which), data.ctypes.data_as(ctypes.c_void_p),
+                                       ctypes.c_size_t(data.size * 4)))
+
+    def stim_cols(self, p: int) -> np.ndarray:
+        out = np.empty(self.config.stimcols, dtype=np.uint32)
+        _check(lib().axon_stim_cols(self._h, ctypes.c_uint32(p), _ptr(out)))
+        return out
+
+    def present(self, p: int, keep: float, T: int, clamp_steps: int, learn: bool, freeze: bool = True,
+                seed: int = 0, use_bias: int = 2, stim: Optional[np.ndarray] = None,
+                outputs: bool = True) -> Optional[PresentResult]:
+        """present() of mm:815-888.  outputs=False enqueues without waiting."""
+        a = PresentArgs(p, T, clamp_steps, int(bool(learn)), int(bool(freeze)), seed, use_bias,
+                        0 if outputs else PRESENT_ASYNC, float(keep), None)
+        if stim is not None:
+            stim = np.ascontiguousarray(stim, dtype=np.uint32)
+            a.stim = _ptr(stim)
+        o = PresentOut()
+        res = None
+        if outputs:
+            res = PresentResult(np.zeros(self.ncol, np.uint32), np.zeros(max(T, 1) * self.ncol, np.uint32),
+                                np.zeros(self.words("STATS"), np.uint32), np.zeros(self.words("DRIVELOG"), np.uint32),
+                                np.zeros(2, np.uint32), 0.0)
+            o.spikes, o.hist, o.stats = _ptr(res.spikes), _ptr(res.hist), _ptr(res.stats)
+            o.drive_log, o.theta = _ptr(res.drive_log), _ptr(res.theta)
+        _check(lib().axon_present(self._h, ctypes.byref(a), ctypes.byref(o)))
+        if res is not None:
+            res.ms_per_step = o.ms_per_step
+            res.hist = res.hist[: T * self.ncol].reshape(T, self.ncol)
+        return res
+
+    def sync(self) -> None:
+        _check(lib().axon_sync(self._h))
+
+    def set_ref_ball(self, ball: np.ndarray) -> None:
+        _check(lib().axon_set_ref_ball(self._h, _ptr(np.ascontiguousarray(ball, np.uint32))))
+
+    def set_mult(self, mult: np.ndarray) -> None:
+        _check(lib().axon_set_mult(self._h, _ptr(np.ascontiguousarray(mult, np.uint32))))
+
+    def clear_drive_log(self) -> None:
+        _check(lib().axon_clear_drive_log(self._h))
+
+    def set_topk_parallel(self, par: bool) -> None:
+        _check(lib().axon_net_set_topk_parallel(self._h, ctypes.c_uint32(int(par))))
+
+    def wstat(self) -> int:
+        v = ctypes.c_uint32()
+        _check(lib().axon_wstat(self._h, ctypes.byref(v)))
+        return v.value
+
+    def within(self, ball: np.ndarray, post_mask: Optional[np.ndarray] = None) -> np.ndarray:
+        out = np.zeros(16, np.uint32)
+        ball = np.ascontiguousarray(ball, np.uint32)
+        pm = None if post_mask is None else np.ascontiguousarray(post_mask, np.uint32)
+        _check(lib().axon_within(self._h, _ptr(ball), _ptr(pm), _ptr(out)))
+        return out
+
+    def indeg(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().axon_indeg(self._h, _ptr(out)))
+        return out
+
+    def untouched(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().axon_untouched(self._h, _ptr(out)))
+        return out
+
+    def wdiag(self, memb: np.ndarray, asm_spk: np.ndarray, nstim: int, sel: int) -> np.ndarray:
+        out = np.zeros(48, np.uint32)
+        _check(lib().axon_wdiag(self._h, _ptr(np.ascontiguousarray(memb, np.uint32)),
+                                _ptr(np.ascontiguousarray(asm_spk, np.uint32)), ctypes.c_uint32(nstim),
+                                ctypes.c_uint32(sel), _ptr(out)))
+        return out
+
+
+def bs_test(op: int, lo: int, hi: int, planes: Sequence[np.ndarray], mask: np.ndarray) -> list:
+    """k_bstest (mm:424-434): apply inc (0), dec (1) or clip[lo,hi] (2) on the GPU."""
+    p = [np.ascontiguousarray(x, np.uint32).copy() for x in planes]
+    m = np.ascontiguousarray(mask, np.uint32)
+    _check(lib().axon_bs_test(op, lo, hi, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(p[3]), _ptr(m),
+                              ctypes.c_size_t(m.size)))
+    return p
+
+
+def kwta_test(values: np.ndarray, k: int):
+    """kwta_ballot_margin (mm:134-144) on rows of 32 values."""
+    v = np.ascontiguousarray(values, np.uint32).reshape(-1, 32)
+    n = v.shape[0]
+    ball, vk, vk1 = (np.zeros(n, np.uint32) for _ in range(3))
+    _check(lib().axon_kwta_test(_ptr(v), ctypes.c_uint32(n), ctypes.c_uint32(k), _ptr(ball), _ptr(vk), _ptr(vk1)))
+    return ball, vk, vk1
+
+
+def topk_test(drives: np.ndarray, incols: int, target: int, par: bool) -> np.ndarray:
+    """k_topk_par / k_topk_serial (mm:259-352) on rows of column drives."""
+    d = np.ascontiguousarray(drives, np.uint32)
+    if d.ndim == 1:
+        d = d[None, :]
+    out = np.zeros(2 * d.shape[0], np.uint32)
+    _check(lib().axon_topk_test(_ptr(d), ctypes.c_uint32(d.shape[0]), ctypes.c_uint32(d.shape[1]),
+                                ctypes.c_uint32(incols), ctypes.c_uint32(target), ctypes.c_uint32(int(par)), _ptr(out)))
+    return out.reshape(-1, 2)

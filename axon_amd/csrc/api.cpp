@@ -1,0 +1,634 @@
+// C-ABI host layer of the AXON engine (libaxon.so): configuration, device
+// memory, the presentation sequencer and the readouts.  See axon_api.h.
+//
+// A presentation (present(), mm:815-888) is T steps of drive -> top-K ->
+// emit(+bias+plasticity).  The per-step parameters depend only on (T, clamp
+// steps, learn, k, facil, useBias), so the whole presentation is captured once
+// per such key in a hipGraph and replayed; the stimulus is the only per-call
+// input and is staged into device memory before the replay.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/axon_api.h"
+#include "engine.h"
+#include "host_rng.h"
+
+using axon::NetDev;
+using axon::Params;
+
+namespace {
+
+thread_local std::string g_err;
+
+axon_status fail(axon_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+axon_status fail(axon_status s, const char* fmt, ...) {
+    char b[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(b, sizeof b, fmt, ap);
+    va_end(ap);
+    g_err = b;
+    return s;
+}
+
+#define HIPCHECK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(AXON_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_));     \
+    } while (0)
+
+constexpr uint32_t NSTAGE = 8;  // pinned stimulus staging slots
+
+// One device, checked once: the code objects in this library are gfx950 only.
+axon_status select_device(int* dev) {
+    static int cached = -2;
+    static std::string why;
+    if (cached == -2) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            cached = -1;
+            why = "no HIP device visible";
+        } else {
+            int d = 0;
+            if (const char* e = getenv("AXON_DEVICE")) d = atoi(e);
+            hipDeviceProp_t p;
+            if (d >= n || hipGetDeviceProperties(&p, d) != hipSuccess) {
+                cached = -1;
+                why = "cannot query HIP device";
+            } else if (strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+                cached = -1;
+                why = std::string("device is ") + p.gcnArchName + ", this build targets gfx950 (MI355X)";
+            } else {
+                cached = d;
+            }
+        }
+    }
+    if (cached < 0) return fail(AXON_ERR_DEVICE, "%s", why.c_str());
+    if (hipSetDevice(cached) != hipSuccess) return fail(AXON_ERR_DEVICE, "hipSetDevice failed");
+    *dev = cached;
+    return AXON_OK;
+}
+
+uint32_t argu(int argc, const char* const* argv, const char* key, uint32_t dflt) {  // mm:660-667
+    const size_t kl = strlen(key);
+    uint32_t v = dflt;
+    for (int i = 1; i < argc; ++i)
+        if (!strncmp(argv[i], key, kl) && argv[i][kl] == '=') v = (uint32_t)atoi(argv[i] + kl + 1);
+    return v;
+}
+
+struct GraphKey {
+    uint32_t T, clamp, learn, k, facil, useBias;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(T, clamp, learn, k, facil, useBias) < std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias);
+    }
+};
+
+}  // namespace
+
+struct axon_net {
+    axon_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    NetDev d{};
+    uint32_t NCOL = 0, N = 0, MAXT = 0, NSTATS = 0, NDLOG = 0;
+    size_t PLANE_U4 = 0;
+    std::vector<std::vector<uint32_t>> stimCols;
+    Params P{};
+    uint32_t K = 0, kTest = 0, facil = 0, contrast = 0, ltdEvery = 1;
+    uint32_t* h_stim[NSTAGE] = {};
+    hipEvent_t stim_ev[NSTAGE] = {};
+    bool stim_ev_live[NSTAGE] = {};
+    uint32_t stage = 0;
+    uint32_t* d_small = nullptr;  // 64 words of readout counters
+    uint32_t* d_ball = nullptr;   // NCOL words
+    uint32_t* d_mask = nullptr;   // N words
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    bool use_graphs = true;
+    std::vector<void*> allocs;
+};
+
+// ================================================================== config
+extern "C" uint32_t axon_abi_version(void) { return AXON_ABI_VERSION; }
+
+extern "C" const char* axon_status_str(axon_status s) {
+    switch (s) {
+        case AXON_OK: return "ok";
+        case AXON_ERR_INVALID_CONFIG: return "invalid config";
+        case AXON_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case AXON_ERR_BUDGET_EXCEEDED: return "HBM budget exceeded";
+        case AXON_ERR_DEVICE: return "device error";
+        case AXON_ERR_INVALID_ARG: return "invalid argument";
+        case AXON_ERR_CAPACITY: return "capacity exceeded";
+        case AXON_ERR_NOT_READY: return "not ready";
+    }
+    return "unknown status";
+}
+
+extern "C" const char* axon_last_error(void) { return g_err.c_str(); }
+
+extern "C" void axon_config_defaults(axon_config* c) {  // mm:688-728 (built-in defaults, not Config C)
+    memset(c, 0, sizeof(*c));
+    c->nand = 4; c->target = 256; c->k = 6; c->dthr = 2; c->boost = 4; c->mode = 1; c->epochs = 30;
+    c->T = 10; c->clampsteps = 5; c->stim = 12; c->quiet = 0; c->seed = 0; c->split = 0; c->unionchar = 0;
+    c->biasup = 0; c->biasdown = 1; c->biasshift = 4; c->biasdownevery = 1; c->inbr = 8; c->stimcols = 128;
+    c->incols = 512; c->ncol = 4096; c->diag = 0; c->profile = 0; c->topkpar = 1; c->vtopk = 0;
+    c->contrast = 0; c->bandlo = 5; c->bandhi = 11; c->margingate = 1; c->hetero = 0; c->vbs = 0;
+    c->ltdrep = 1; c->ltdevery = 1; c->ktest = 0; c->facil = 0; c->deep = 0; c->wave = 0; c->wavet = 35;
+    c->trainclamp = c->contrast ? c->clampsteps : c->T;
+    c->max_t = 64;
+}
+
+extern "C" axon_status axon_config_parse(int argc, const char* const* argv, axon_config* c) {
+    if (!c || (argc > 0 && !argv)) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    axon_config_defaults(c);
+    c->nand = argu(argc, argv, "nand", 4);
+    c->target = argu(argc, argv, "target", 256);
+    c->k = argu(argc, argv, "k", 6);
+    c->dthr = argu(argc, argv, "dthr", 2);
+    c->boost = argu(argc, argv, "boost", 4);
+    c->mode = argu(argc, argv, "mode", 1);
+    c->epochs = argu(argc, argv, "epochs", 30);
+    c->T = argu(argc, argv, "T", 10);
+    c->clampsteps = argu(argc, argv, "clampsteps", 5);
+    c->stim = argu(argc, argv, "stim", 12);
+    c->quiet = argu(argc, argv, "quiet", 0);
+    c->seed = argu(argc, argv, "seed", 0);
+    c->split = argu(argc, argv, "split", 0);
+    c->unionchar = argu(argc, argv, "unionchar", 0);
+    c->biasup = argu(argc, argv, "biasup", 0);
+    c->biasdown = argu(argc, argv, "biasdown", 1);
+    c->biasshift = argu(argc, argv, "biasshift", 4);
+    c->biasdownevery = argu(argc, argv, "biasdownevery", 1);
+    c->inbr = argu(argc, argv, "inbr", 8);
+    c->stimcols = argu(argc, argv, "stimcols", 128);
+    c->incols = argu(argc, argv, "incols", 512);
+    c->ncol = argu(argc, argv, "ncol", 4096);
+    c->diag = argu(argc, argv, "diag", 0);
+    c->profile = argu(argc, argv, "profile", 0);
+    c->topkpar = argu(argc, argv, "topkpar", 1);
+    c->vtopk = argu(argc, argv, "vtopk", 0);
+    c->contrast = argu(argc, argv, "contrast", 0);
+    c->bandlo = argu(argc, argv, "bandlo", 5);
+    c->bandhi = argu(argc, argv, "bandhi", 11);
+    c->margingate = argu(argc, argv, "margingate", 1);
+    c->hetero = argu(argc, argv, "hetero", 0);
+    c->vbs = argu(argc, argv, "vbs", 0);
+    c->ltdrep = argu(argc, argv, "ltdrep", 1);
+    c->ltdevery = argu(argc, argv, "ltdevery", 1);
+    c->ktest = argu(argc, argv, "ktest", 0);
+    c->facil = argu(argc, argv, "facil", 0);
+    c->deep = argu(argc, argv, "deep", 0);
+    c->wave = argu(argc, argv, "wave", 0);
+    c->wavet = argu(argc, argv, "wavet", 35);
+    c->trainclamp = argu(argc, argv, "trainclamp", c->contrast ? c->clampsteps : c->T);  // mm:731
+    if (c->biasup == 1u && c->ncol > c->incols) {                                        // mm:732-735
+        const double pr = (double)c->target * c->k / (double)(c->ncol - c->incols) / axon::NPC;
+        c->biasup = (uint32_t)std::max(2.0, (1.0 - pr) / pr * c->biasdown);
+    }
+    c->max_t = argu(argc, argv, "maxt", 64);
+    return AXON_OK;
+}
+
+// Rejects the configurations on which the reference has undefined behaviour
+// (e.g. incols >= ncol divides by zero in k_init_conn, mm:116) instead of
+// running them.
+extern "C" axon_status axon_config_validate(const axon_config* c) {
+    if (!c) return fail(AXON_ERR_INVALID_ARG, "null config");
+    auto bad = [](const char* m) { return fail(AXON_ERR_INVALID_CONFIG, "%s", m); };
+    if (c->ncol == 0 || c->ncol % axon::COLS_PER_GRP) return bad("ncol must be a positive multiple of 8");
+    if (c->ncol > (1u << 20)) return bad("ncol must be <= 1048576");
+    if (c->incols == 0 || c->incols % axon::COLS_PER_GRP) return bad("incols must be a positive multiple of 8");
+    if (c->incols >= c->ncol) return bad("incols must be < ncol");
+    if (c->stimcols > c->incols) return bad("stimcols must be <= incols");
+    if (c->inbr > axon::NSLOT) return bad("inbr must be <= 32");
+    if (c->biasshift >= 32) return bad("biasshift must be < 32");
+    if (c->contrast && c->ltdevery == 0) return bad("ltdevery must be >= 1");
+    if (c->max_t == 0 || c->max_t > 4096) return bad("max_t must be in 1..4096");
+    if (c->stim == 0) return bad("stim must be >= 1");
+    return AXON_OK;
+}
+
+// ================================================================== network
+static axon_status dev_alloc(axon_net* n, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(AXON_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    n->allocs.push_back(*p);
+    return AXON_OK;
+}
+
+extern "C" void axon_net_destroy(axon_net* n) {
+    if (!n) return;
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    for (auto& kv : n->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (uint32_t i = 0; i < NSTAGE; ++i) {
+        if (n->stim_ev[i]) (void)hipEventDestroy(n->stim_ev[i]);
+        if (n->h_stim[i]) (void)hipHostFree(n->h_stim[i]);
+    }
+    for (void* p : n->allocs) (void)hipFree(p);
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+}
+
+extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
+    if (!c || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    axon_status st = axon_config_validate(c);
+    if (st != AXON_OK) return st;
+    int dev = 0;
+    if ((st = select_device(&dev)) != AXON_OK) return st;
+    axon_net* n = new axon_net();
+    n->cfg = *c;
+    n->device = dev;
+    n->NCOL = c->ncol;
+    n->N = c->ncol * axon::NPC;
+    n->MAXT = c->max_t;
+    n->NSTATS = std::max<uint32_t>(1024u, 64u + 8u * n->MAXT);
+    n->NDLOG = std::max<uint32_t>(256u, 4u * n->MAXT);
+    n->PLANE_U4 = (size_t)n->NCOL * axon::NSLOT * axon::TILE_U4;
+    if (const char* e = getenv("AXON_NO_GRAPH")) n->use_graphs = atoi(e) == 0;
+
+    auto cleanup = [&](axon_status s) { axon_net_destroy(n); return s; };
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(AXON_ERR_DEVICE, "hipStreamCreate failed"));
+    NetDev& d = n->d;
+    void* p = nullptr;
+    uint4** planes[4] = {&d.W0, &d.W1, &d.W2, &d.W3};
+    for (int i = 0; i < 4; ++i) {
+        if ((st = dev_alloc(n, &p, n->PLANE_U4 * 16)) != AXON_OK) return cleanup(st);
+        *planes[i] = (uint4*)p;
+    }
+    struct { uint32_t** ptr; size_t words; } bufs[] = {
+        {&d.conn, (size_t)n->NCOL * axon::NSLOT}, {&d.bias, n->N}, {&d.spk[0], n->NCOL}, {&d.spk[1], n->NCOL},
+        {&d.acc, n->N}, {&d.bbits, n->N}, {&d.drive, n->NCOL}, {&d.theta, 16}, {&d.marg, n->NCOL},
+        {&d.stim, n->NCOL}, {&d.stats, n->NSTATS}, {&d.hist, (size_t)n->MAXT * n->NCOL}, {&d.driveLog, n->NDLOG},
+        {&d.refBall, n->NCOL}, {&d.mult, n->N}, {&n->d_small, 64}, {&n->d_ball, n->NCOL}, {&n->d_mask, n->N},
+    };
+    for (auto& b : bufs) {
+        if ((st = dev_alloc(n, &p, b.words * 4)) != AXON_OK) return cleanup(st);
+        *b.ptr = (uint32_t*)p;
+        // Metal zero-fills new buffers; bias, refBall, mult and driveLog rely on it (mm:758-763)
+        if (hipMemsetAsync(p, 0, b.words * 4, n->stream) != hipSuccess)
+            return cleanup(fail(AXON_ERR_DEVICE, "hipMemsetAsync failed"));
+    }
+    d.ncol = n->NCOL; d.incols = c->incols; d.inbr = c->inbr; d.topkpar = c->topkpar;
+    d.nstats = n->NSTATS; d.ndlog = n->NDLOG; d.maxt = n->MAXT;
+    for (uint32_t i = 0; i < NSTAGE; ++i) {
+        if (hipHostMalloc((void**)&n->h_stim[i], (size_t)n->NCOL * 4, hipHostMallocDefault) != hipSuccess)
+            return cleanup(fail(AXON_ERR_OUT_OF_MEMORY, "hipHostMalloc failed"));
+        if (hipEventCreateWithFlags(&n->stim_ev[i], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(AXON_ERR_DEVICE, "hipEventCreate failed"));
+    }
+    if (axon::launch_init(d, c->nand, c->seed, n->stream) != hipSuccess)
+        return cleanup(fail(AXON_ERR_DEVICE, "init kernels failed to launch"));
+    if (hipStreamSynchronize(n->stream) != hipSuccess)
+        return cleanup(fail(AXON_ERR_DEVICE, "init kernels failed"));
+
+    // stimulus columns, mm:793-799
+    std::mt19937 rng(7u + c->seed * 104729u);
+    n->stimCols.resize(c->stim);
+    std::vector<uint32_t> all(c->incols);
+    for (uint32_t q = 0; q < c->stim; ++q) {
+        for (uint32_t i = 0; i < c->incols; ++i) all[i] = i;
+        axon::libcxx_shuffle(all.data(), c->incols, rng);
+        n->stimCols[q].assign(all.begin(), all.begin() + c->stimcols);
+    }
+    // Params fixed for the network, mm:810-812
+    Params& P = n->P;
+    P = Params{};
+    P.k = c->k; P.dthr = c->dthr; P.boost = c->boost; P.mode = c->mode; P.inCols = c->incols;
+    P.targetCols = c->target; P.biasUp = c->biasup; P.biasDown = c->biasdown; P.biasShift = c->biasshift;
+    P.bandLo = c->bandlo; P.bandHi = c->contrast ? c->bandhi : 0u; P.marginGate = c->margingate;
+    P.hetero = c->hetero; P.ltdRep = c->ltdrep; P.biasDownEvery = c->biasdownevery; P.histOn = 1u;
+    n->K = c->k; n->kTest = c->ktest; n->facil = c->facil; n->contrast = c->contrast;
+    n->ltdEvery = c->ltdevery ? c->ltdevery : 1u;
+    *out = n;
+    return AXON_OK;
+}
+
+// ================================================================== present
+static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn, hipError_t* err) {
+    hipError_t e = axon::launch_clear(n->d, n->stream);
+    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {  // mm:837-884
+        P.step = t;
+        P.learn = learn ? 1u : 0u;
+        P.freeze = learn ? 0u : 1u;
+        P.clamp = (t < clampSteps) ? 1u : 0u;
+        P.depress = (learn && (t % 2u == 1u)) ? 1u : 0u;
+        P.phase = (!learn || !n->contrast) ? 0u
+                  : (t < clampSteps)      ? 1u
+                  : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
+        e = axon::launch_drive(n->d, P, t, n->stream);
+        if (e == hipSuccess) e = axon::launch_topk(n->d, P, n->stream);
+        if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, n->stream);
+    }
+    *err = e;
+}
+
+extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axon_present_out* o) {
+    if (!n || !a) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (a->T > n->MAXT)
+        return fail(AXON_ERR_CAPACITY, "T=%u exceeds the history capacity max_t=%u", a->T, n->MAXT);
+    if (!a->stim && a->pattern >= n->stimCols.size())
+        return fail(AXON_ERR_INVALID_ARG, "pattern %u out of range (stim=%zu)", a->pattern, n->stimCols.size());
+    if (a->use_bias > 2u) return fail(AXON_ERR_INVALID_ARG, "use_bias must be 0, 1 or 2");
+    const bool async = (a->flags & AXON_PRESENT_ASYNC) != 0u;
+    if (async && o && (o->spikes || o->hist || o->stats || o->drive_log || o->theta))
+        return fail(AXON_ERR_INVALID_ARG, "AXON_PRESENT_ASYNC takes no outputs");
+    HIPCHECK(hipSetDevice(n->device));
+    const bool learn = a->learn != 0u;
+    Params P = n->P;  // mm:818-822
+    P.k = learn ? n->K : (n->kTest != 0u ? n->kTest : n->K);
+    P.facil = learn ? 0u : n->facil;
+    P.useBias = (a->use_bias == 2u) ? (learn ? 1u : 0u) : a->use_bias;
+
+    // stimulus: loadStim (mm:800-804) into a pinned staging slot
+    const uint32_t slot = n->stage;
+    n->stage = (n->stage + 1u) % NSTAGE;
+    if (n->stim_ev_live[slot]) HIPCHECK(hipEventSynchronize(n->stim_ev[slot]));
+    uint32_t* hs = n->h_stim[slot];
+    if (a->stim) {
+        memcpy(hs, a->stim, (size_t)n->NCOL * 4);
+    } else {
+        memset(hs, 0, (size_t)n->NCOL * 4);
+        std::mt19937 r(a->seed);
+        for (uint32_t c : n->stimCols[a->pattern])
+            if (axon::canonical01(r) <= a->keep) hs[c] = 0xFFFFFFFFu;
+    }
+    HIPCHECK(hipMemcpyAsync(n->d.stim, hs, (size_t)n->NCOL * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipEventRecord(n->stim_ev[slot], n->stream));
+    n->stim_ev_live[slot] = true;
+
+    const auto t0 = std::chrono::steady_clock::now();
+    if (n->use_graphs && a->T > 0) {
+        const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias};
+        auto it = n->graphs.find(key);
+        if (it == n->graphs.end()) {
+            hipGraph_t g = nullptr;
+            HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            enqueue_steps(n, P, a->T, a->clamp_steps, learn, &e);
+            hipError_t e2 = hipStreamEndCapture(n->stream, &g);
+            if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "capture launch: %s", hipGetErrorString(e));
+            if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+            hipGraphExec_t ex = nullptr;
+            hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e3 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+            it = n->graphs.emplace(key, ex).first;
+        }
+        HIPCHECK(hipGraphLaunch(it->second, n->stream));
+    } else {
+        hipError_t e = hipSuccess;
+        enqueue_steps(n, P, a->T, a->clamp_steps, learn, &e);
+        if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
+    }
+    if (async) {
+        if (o) o->ms_per_step = 0.0;
+        return AXON_OK;
+    }
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (o) {
+        o->ms_per_step = a->T ? ms / a->T : 0.0;
+        const size_t NC = n->NCOL;
+        if (o->spikes) HIPCHECK(hipMemcpy(o->spikes, n->d.spk[a->T & 1u], NC * 4, hipMemcpyDeviceToHost));
+        if (o->hist && a->T) HIPCHECK(hipMemcpy(o->hist, n->d.hist, (size_t)a->T * NC * 4, hipMemcpyDeviceToHost));
+        if (o->stats) HIPCHECK(hipMemcpy(o->stats, n->d.stats, (size_t)n->NSTATS * 4, hipMemcpyDeviceToHost));
+        if (o->drive_log) HIPCHECK(hipMemcpy(o->drive_log, n->d.driveLog, (size_t)n->NDLOG * 4, hipMemcpyDeviceToHost));
+        if (o->theta) HIPCHECK(hipMemcpy(o->theta, n->d.theta, 8, hipMemcpyDeviceToHost));
+    }
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_sync(axon_net* n) {
+    if (!n) return fail(AXON_ERR_INVALID_ARG, "null net");
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    return AXON_OK;
+}
+
+// ================================================================== buffers
+static axon_status buffer_ptr(const axon_net* n, axon_buffer w, void** p, size_t* words) {
+    const NetDev& d = n->d;
+    const size_t pw = n->PLANE_U4 * 4;
+    switch (w) {
+        case AXON_BUF_W0: *p = d.W0; *words = pw; break;
+        case AXON_BUF_W1: *p = d.W1; *words = pw; break;
+        case AXON_BUF_W2: *p = d.W2; *words = pw; break;
+        case AXON_BUF_W3: *p = d.W3; *words = pw; break;
+        case AXON_BUF_CONN: *p = d.conn; *words = (size_t)n->NCOL * axon::NSLOT; break;
+        case AXON_BUF_SPK_A: *p = d.spk[0]; *words = n->NCOL; break;
+        case AXON_BUF_SPK_B: *p = d.spk[1]; *words = n->NCOL; break;
+        case AXON_BUF_DRIVE: *p = d.drive; *words = n->NCOL; break;
+        case AXON_BUF_BBITS: *p = d.bbits; *words = n->N; break;
+        case AXON_BUF_ACC: *p = d.acc; *words = n->N; break;
+        case AXON_BUF_BIAS: *p = d.bias; *words = n->N; break;
+        case AXON_BUF_MARG: *p = d.marg; *words = n->NCOL; break;
+        case AXON_BUF_REFBALL: *p = d.refBall; *words = n->NCOL; break;
+        case AXON_BUF_DRIVELOG: *p = d.driveLog; *words = n->NDLOG; break;
+        case AXON_BUF_MULT: *p = d.mult; *words = n->N; break;
+        case AXON_BUF_STIM: *p = d.stim; *words = n->NCOL; break;
+        case AXON_BUF_STATS: *p = d.stats; *words = n->NSTATS; break;
+        case AXON_BUF_HIST: *p = d.hist; *words = (size_t)n->MAXT * n->NCOL; break;
+        case AXON_BUF_THETA: *p = d.theta; *words = 16; break;
+        default: return fail(AXON_ERR_INVALID_ARG, "unknown buffer %d", (int)w);
+    }
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_buffer_words(const axon_net* n, axon_buffer w, size_t* words) {
+    if (!n || !words) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    void* p;
+    return buffer_ptr(n, w, &p, words);
+}
+
+extern "C" axon_status axon_read_buffer(axon_net* n, axon_buffer w, void* dst, size_t bytes) {
+    if (!n || !dst) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    void* p; size_t words;
+    axon_status st = buffer_ptr(n, w, &p, &words);
+    if (st != AXON_OK) return st;
+    if (bytes > words * 4) return fail(AXON_ERR_INVALID_ARG, "read of %zu bytes exceeds buffer (%zu)", bytes, words * 4);
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    HIPCHECK(hipMemcpy(dst, p, bytes, hipMemcpyDeviceToHost));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_write_buffer(axon_net* n, axon_buffer w, const void* src, size_t bytes) {
+    if (!n || !src) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    void* p; size_t words;
+    axon_status st = buffer_ptr(n, w, &p, &words);
+    if (st != AXON_OK) return st;
+    if (bytes > words * 4) return fail(AXON_ERR_INVALID_ARG, "write of %zu bytes exceeds buffer (%zu)", bytes, words * 4);
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    HIPCHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_set_ref_ball(axon_net* n, const uint32_t* ball) {
+    return axon_write_buffer(n, AXON_BUF_REFBALL, ball, (size_t)(n ? n->NCOL : 0) * 4);
+}
+extern "C" axon_status axon_set_mult(axon_net* n, const uint32_t* mult) {
+    return axon_write_buffer(n, AXON_BUF_MULT, mult, (size_t)(n ? n->N : 0) * 4);
+}
+extern "C" axon_status axon_clear_drive_log(axon_net* n) {
+    if (!n) return fail(AXON_ERR_INVALID_ARG, "null net");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipMemsetAsync(n->d.driveLog, 0, (size_t)n->NDLOG * 4, n->stream));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_net_set_topk_parallel(axon_net* n, uint32_t par) {
+    if (!n) return fail(AXON_ERR_INVALID_ARG, "null net");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    n->d.topkpar = par ? 1u : 0u;
+    for (auto& kv : n->graphs) (void)hipGraphExecDestroy(kv.second);  // kernel args captured the old mode
+    n->graphs.clear();
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_stim_cols(const axon_net* n, uint32_t p, uint32_t* out) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (p >= n->stimCols.size()) return fail(AXON_ERR_INVALID_ARG, "pattern %u out of range", p);
+    std::copy(n->stimCols[p].begin(), n->stimCols[p].end(), out);
+    return AXON_OK;
+}
+
+// ================================================================== readouts
+extern "C" axon_status axon_wstat(axon_net* n, uint32_t* out) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipMemsetAsync(n->d_small, 0, 4, n->stream));
+    HIPCHECK(axon::launch_wstat(n->d, n->d_small, n->stream));
+    HIPCHECK(hipMemcpyAsync(out, n->d_small, 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_within(axon_net* n, const uint32_t* ball, const uint32_t* post_mask, uint32_t out[16]) {
+    if (!n || !ball || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipMemcpyAsync(n->d_ball, ball, (size_t)n->NCOL * 4, hipMemcpyHostToDevice, n->stream));
+    if (post_mask) HIPCHECK(hipMemcpyAsync(n->d_mask, post_mask, (size_t)n->N * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemsetAsync(n->d_small, 0, 64, n->stream));
+    HIPCHECK(axon::launch_within(n->d, n->d_ball, n->d_mask, post_mask ? 1u : 0u, n->d_small, n->stream));
+    uint32_t tmp[16];
+    HIPCHECK(hipMemcpyAsync(tmp, n->d_small, 64, hipMemcpyDeviceToHost, n->stream));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    memcpy(out, tmp, sizeof tmp);  // the reference zeroes bWCnt before each call (mm:1701)
+    return AXON_OK;
+}
+
+static axon_status per_neuron(axon_net* n, uint32_t* out, hipError_t (*fn)(const NetDev&, uint32_t*, hipStream_t)) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(fn(n->d, n->d_mask, n->stream));
+    HIPCHECK(hipMemcpyAsync(out, n->d_mask, (size_t)n->N * 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    return AXON_OK;
+}
+extern "C" axon_status axon_indeg(axon_net* n, uint32_t* out) { return per_neuron(n, out, axon::launch_indeg); }
+extern "C" axon_status axon_untouched(axon_net* n, uint32_t* out) { return per_neuron(n, out, axon::launch_untouched); }
+
+extern "C" axon_status axon_wdiag(axon_net* n, const uint32_t* memb, const uint32_t* asm_spk, uint32_t nstim,
+                                  uint32_t sel, uint32_t hist[48]) {
+    if (!n || !memb || !asm_spk || !hist) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (nstim > 32) return fail(AXON_ERR_INVALID_ARG, "nstim must be <= 32 (membership is a u32 mask)");
+    HIPCHECK(hipSetDevice(n->device));
+    uint32_t* d_asm = nullptr;
+    const size_t asmBytes = std::max<size_t>((size_t)nstim * n->NCOL * 4, 4);
+    HIPCHECK(hipMalloc(&d_asm, asmBytes));
+    hipError_t e = hipMemcpyAsync(n->d_mask, memb, (size_t)n->N * 4, hipMemcpyHostToDevice, n->stream);
+    if (e == hipSuccess && nstim) e = hipMemcpyAsync(d_asm, asm_spk, (size_t)nstim * n->NCOL * 4, hipMemcpyHostToDevice, n->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(n->d_small, 0, 48 * 4, n->stream);
+    if (e == hipSuccess) e = axon::launch_wdiag(n->d, n->d_mask, d_asm, nstim, sel, n->d_small, n->stream);
+    uint32_t tmp[48];
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp, n->d_small, 48 * 4, hipMemcpyDeviceToHost, n->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    (void)hipFree(d_asm);
+    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "wdiag: %s", hipGetErrorString(e));
+    memcpy(hist, tmp, sizeof tmp);  // zeroed before each call (mm:1127)
+    return AXON_OK;
+}
+
+// ================================================================== kernel harnesses
+namespace {
+struct DevTmp {
+    std::vector<void*> ps;
+    ~DevTmp() { for (void* p : ps) (void)hipFree(p); }
+    hipError_t alloc(uint32_t** p, size_t bytes) {
+        hipError_t e = hipMalloc((void**)p, std::max<size_t>(bytes, 4));
+        if (e == hipSuccess) ps.push_back(*p);
+        return e;
+    }
+};
+}  // namespace
+
+extern "C" axon_status axon_bs_test(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1,
+                                    uint32_t* a2, uint32_t* a3, const uint32_t* m, size_t nw) {
+    if (!a0 || !a1 || !a2 || !a3 || !m) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    int dev;
+    axon_status st = select_device(&dev);
+    if (st != AXON_OK) return st;
+    DevTmp t;
+    uint32_t* d[5];
+    for (int i = 0; i < 5; ++i) HIPCHECK(t.alloc(&d[i], nw * 4));
+    uint32_t* h[4] = {a0, a1, a2, a3};
+    for (int i = 0; i < 4; ++i) HIPCHECK(hipMemcpy(d[i], h[i], nw * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d[4], m, nw * 4, hipMemcpyHostToDevice));
+    HIPCHECK(axon::launch_bstest(op, lo, hi, d[0], d[1], d[2], d[3], d[4], nw, nullptr));
+    HIPCHECK(hipDeviceSynchronize());
+    for (int i = 0; i < 4; ++i) HIPCHECK(hipMemcpy(h[i], d[i], nw * 4, hipMemcpyDeviceToHost));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_kwta_test(const uint32_t* v, uint32_t ncols, uint32_t k, uint32_t* ball, uint32_t* vk,
+                                      uint32_t* vk1) {
+    if (!v || !ball || !vk || !vk1) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    int dev;
+    axon_status st = select_device(&dev);
+    if (st != AXON_OK) return st;
+    DevTmp t;
+    uint32_t *dv, *db, *dk, *dk1;
+    HIPCHECK(t.alloc(&dv, (size_t)ncols * 32 * 4));
+    HIPCHECK(t.alloc(&db, (size_t)ncols * 4));
+    HIPCHECK(t.alloc(&dk, (size_t)ncols * 4));
+    HIPCHECK(t.alloc(&dk1, (size_t)ncols * 4));
+    HIPCHECK(hipMemcpy(dv, v, (size_t)ncols * 32 * 4, hipMemcpyHostToDevice));
+    HIPCHECK(axon::launch_kwta_test(dv, ncols, k, db, dk, dk1, nullptr));
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(ball, db, (size_t)ncols * 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(vk, dk, (size_t)ncols * 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(vk1, dk1, (size_t)ncols * 4, hipMemcpyDeviceToHost));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol, uint32_t incols,
+                                      uint32_t target, uint32_t par, uint32_t* theta) {
+    if (!drive || !theta || ninst == 0) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    int dev;
+    axon_status st = select_device(&dev);
+    if (st != AXON_OK) return st;
+    DevTmp t;
+    uint32_t *dd, *dt;
+    HIPCHECK(t.alloc(&dd, (size_t)ninst * ncol * 4));
+    HIPCHECK(t.alloc(&dt, (size_t)ninst * 2 * 4));
+    HIPCHECK(hipMemcpy(dd, drive, (size_t)ninst * ncol * 4, hipMemcpyHostToDevice));
+    HIPCHECK(axon::launch_topk_test(dd, ninst, ncol, incols, target, par, dt, nullptr));
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(theta, dt, (size_t)ninst * 2 * 4, hipMemcpyDeviceToHost));
+    return AXON_OK;
+}

@@ -1,0 +1,70 @@
+// Internal interface between the C-ABI host layer (api.cpp) and the HIP
+// kernels (kernels.hip).  Not installed; the public surface is axon_api.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace axon {
+
+constexpr uint32_t NPC = 32;       // neurons per column, mm:32
+constexpr uint32_t NSLOT = 32;     // dendritic branches per neuron, mm:34
+constexpr uint32_t PRESYN = 256;   // presynaptic neurons per branch, mm:36
+constexpr uint32_t FANIN = 8192;   // mm:37
+constexpr uint32_t COLS_PER_GRP = 8;
+constexpr uint32_t TILE_U4 = 64;   // uint4 per (column, branch) tile per plane: [j][lane]
+
+// Per-step kernel parameters.  Same 26 words as the reference Params
+// (mm:56-83); the fields no kernel reads (learn, freeze, theta) are kept so
+// the struct stays a faithful mirror.
+struct Params {
+    uint32_t k, theta, dthr, boost, mode, inCols, step, learn, depress, targetCols, clamp,
+        freeze, biasUp, biasDown, biasShift, histOn, useBias, phase, bandLo, bandHi,
+        marginGate, hetero, ltdRep, biasDownEvery, facil, pad;
+};
+
+// Device pointers of one network instance.
+struct NetDev {
+    uint4* W0; uint4* W1; uint4* W2; uint4* W3;
+    uint32_t* conn;
+    uint32_t* bias;
+    uint32_t* spk[2];
+    uint32_t* acc;
+    uint32_t* bbits;
+    uint32_t* drive;
+    uint32_t* theta;
+    uint32_t* marg;
+    uint32_t* stim;
+    uint32_t* stats;
+    uint32_t* hist;
+    uint32_t* driveLog;
+    uint32_t* refBall;
+    uint32_t* mult;
+    uint32_t ncol, incols, inbr, topkpar;
+    uint32_t nstats, ndlog, maxt, pad;
+};
+
+// ---- launchers (kernels.hip) ; all asynchronous on `s` ----
+hipError_t launch_init(const NetDev& d, uint32_t nand, uint32_t seed, hipStream_t s);
+hipError_t launch_clear(const NetDev& d, hipStream_t s);
+hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s);
+hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s);
+hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, hipStream_t s);
+
+hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s);
+hipError_t launch_within(const NetDev& d, const uint32_t* ball, const uint32_t* postMask, uint32_t useMask,
+                         uint32_t* out, hipStream_t s);
+hipError_t launch_indeg(const NetDev& d, uint32_t* out, hipStream_t s);
+hipError_t launch_untouched(const NetDev& d, uint32_t* out, hipStream_t s);
+hipError_t launch_wdiag(const NetDev& d, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
+                        uint32_t sel, uint32_t* hist, hipStream_t s);
+hipError_t launch_fill(uint32_t* p, uint32_t value, size_t nwords, hipStream_t s);
+
+hipError_t launch_bstest(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1, uint32_t* a2,
+                         uint32_t* a3, const uint32_t* m, size_t nwords, hipStream_t s);
+hipError_t launch_kwta_test(const uint32_t* v, uint32_t ncols, uint32_t k, uint32_t* ball, uint32_t* vk,
+                            uint32_t* vk1, hipStream_t s);
+hipError_t launch_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol, uint32_t incols,
+                            uint32_t target, uint32_t par, uint32_t* theta, hipStream_t s);
+
+}  // namespace axon

@@ -1,0 +1,729 @@
+// HIP/CDNA4 (gfx950) kernels of the AXON engine.
+//
+// Geometry (shared with the reference so the synapse store is bit-identical,
+// mm:30-43): a column is 32 neurons, each neuron has 32 dendritic branches of
+// 256 presynaptic bits.  A (column, branch) tile of one plane is 64 uint4 =
+// 1 KiB laid out [j][neuron] (mm:170), so one wave64 reads a whole tile with a
+// single global_load_dwordx4 per lane: lane L = j*32 + neuron, j selecting the
+// low or high 128 presynaptic bits.  The two halves of the wave are combined
+// with one cross-half exchange per branch.
+//
+// All arithmetic is u32 integer with the reference's wrap-around semantics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "engine.h"
+
+namespace axon {
+
+namespace {
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {  // mm:54
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint32_t ballot32(bool p) { return (uint32_t)__ballot(p); }
+__device__ __forceinline__ uint32_t popc(uint32_t x) { return (uint32_t)__popc(x); }
+
+// Sum over all 64 lanes (u32 wrap-around), result in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    x += __shfl_xor(x, 32);
+    x += __shfl_xor(x, 16);
+    x += __shfl_xor(x, 8);
+    x += __shfl_xor(x, 4);
+    x += __shfl_xor(x, 2);
+    x += __shfl_xor(x, 1);
+    return x;
+}
+// Sum over the 32 lanes of each half.
+__device__ __forceinline__ uint32_t half_sum(uint32_t x) {
+    x += __shfl_xor(x, 16);
+    x += __shfl_xor(x, 8);
+    x += __shfl_xor(x, 4);
+    x += __shfl_xor(x, 2);
+    x += __shfl_xor(x, 1);
+    return x;
+}
+__device__ __forceinline__ uint32_t half_or(uint32_t x) {
+    x |= __shfl_xor(x, 16);
+    x |= __shfl_xor(x, 8);
+    x |= __shfl_xor(x, 4);
+    x |= __shfl_xor(x, 2);
+    x |= __shfl_xor(x, 1);
+    return x;
+}
+
+// ---- exact k-of-32 winners-take-all with margin (kwta_ballot_margin, mm:134-144) ----
+// Lanes 0..31 hold the column's 32 values (lanes 32..63 duplicate them).
+// The reference ranks by (value desc, lane asc): rank_l = #{j: v_j > v_l or
+// (v_j == v_l and j < l)}; winners are rank < k; vk / vk1 are the values at
+// rank k-1 / k (0 when that rank does not exist).  Here the m-th largest value
+// is found by a bitwise radix select over ballots, which gives the same sets:
+//   winners = {v > vk} plus the (k - #{v > vk}) lowest lanes with v == vk.
+__device__ __forceinline__ uint32_t kth_largest(uint32_t v, uint32_t m, int hb) {
+    uint32_t x = 0;
+    for (int b = hb; b >= 0; --b) {
+        uint32_t t = x | (1u << b);
+        if (popc(ballot32(v >= t)) >= m) x = t;
+    }
+    return x;
+}
+struct Kwta { uint32_t ball, vk, vk1; };
+__device__ __forceinline__ Kwta kwta32(uint32_t v, uint32_t k, uint32_t n) {
+    const uint32_t ov = half_or(v);
+    const int hb = ov ? 31 - __clz(ov) : -1;
+    Kwta r;
+    r.vk = (k >= 1u && k <= 32u) ? kth_largest(v, k, hb) : 0u;
+    r.vk1 = (k <= 31u) ? kth_largest(v, k + 1u, hb) : 0u;
+    if (k == 0u) r.ball = 0u;
+    else if (k >= 32u) r.ball = 0xFFFFFFFFu;
+    else {
+        const uint32_t gt = ballot32(v > r.vk);
+        const uint32_t eq = ballot32(v == r.vk);
+        const uint32_t need = k - popc(gt);
+        const bool win = (v > r.vk) || (v == r.vk && popc(eq & ((1u << n) - 1u)) < need);
+        r.ball = ballot32(win);
+    }
+    return r;
+}
+
+// ---- bit-serial 4-plane counters on one 32-synapse word (mm:371-420) ----
+__device__ __forceinline__ void bs_inc(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3, uint32_t m) {
+    uint32_t c = m, t;
+    t = a0 ^ c; c = a0 & c; a0 = t;
+    t = a1 ^ c; c = a1 & c; a1 = t;
+    t = a2 ^ c; c = a2 & c; a2 = t;
+    t = a3 ^ c; c = a3 & c; a3 = t;
+    a0 |= c; a1 |= c; a2 |= c; a3 |= c;
+}
+__device__ __forceinline__ void bs_dec(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3, uint32_t m) {
+    uint32_t b = m, t;
+    t = a0 ^ b; b = (~a0) & b; a0 = t;
+    t = a1 ^ b; b = (~a1) & b; a1 = t;
+    t = a2 ^ b; b = (~a2) & b; a2 = t;
+    t = a3 ^ b; b = (~a3) & b; a3 = t;
+    const uint32_t keep = ~b;
+    a0 &= keep; a1 &= keep; a2 &= keep; a3 &= keep;
+}
+__device__ __forceinline__ uint32_t bs_lt(uint32_t a3, uint32_t a2, uint32_t a1, uint32_t a0, uint32_t K) {
+    uint32_t lt = 0, eq = 0xFFFFFFFFu, kb;
+    kb = (K >> 3) & 1u; if (kb) lt |= eq & ~a3; eq &= kb ? a3 : ~a3;
+    kb = (K >> 2) & 1u; if (kb) lt |= eq & ~a2; eq &= kb ? a2 : ~a2;
+    kb = (K >> 1) & 1u; if (kb) lt |= eq & ~a1; eq &= kb ? a1 : ~a1;
+    kb = (K >> 0) & 1u; if (kb) lt |= eq & ~a0;
+    return lt;
+}
+__device__ __forceinline__ uint32_t bs_gt(uint32_t a3, uint32_t a2, uint32_t a1, uint32_t a0, uint32_t K) {
+    uint32_t gt = 0, eq = 0xFFFFFFFFu, kb;
+    kb = (K >> 3) & 1u; if (!kb) gt |= eq & a3; eq &= kb ? a3 : ~a3;
+    kb = (K >> 2) & 1u; if (!kb) gt |= eq & a2; eq &= kb ? a2 : ~a2;
+    kb = (K >> 1) & 1u; if (!kb) gt |= eq & a1; eq &= kb ? a1 : ~a1;
+    kb = (K >> 0) & 1u; if (!kb) gt |= eq & a0;
+    return gt;
+}
+__device__ __forceinline__ void bs_set(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3, uint32_t m,
+                                       uint32_t C) {
+    const uint32_t keep = ~m;
+    a0 = (a0 & keep) | (((C >> 0) & 1u) ? m : 0u);
+    a1 = (a1 & keep) | (((C >> 1) & 1u) ? m : 0u);
+    a2 = (a2 & keep) | (((C >> 2) & 1u) ? m : 0u);
+    a3 = (a3 & keep) | (((C >> 3) & 1u) ? m : 0u);
+}
+__device__ __forceinline__ void bs_clip(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3, uint32_t lo,
+                                        uint32_t hi) {
+    const uint32_t l = bs_lt(a3, a2, a1, a0, lo);
+    const uint32_t g = bs_gt(a3, a2, a1, a0, hi);
+    bs_set(a0, a1, a2, a3, l, lo);
+    bs_set(a0, a1, a2, a3, g, hi);
+}
+
+// One 32-synapse word of k_learn's per-tile update (mm:471-487).
+__device__ __forceinline__ void learn_word(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3, uint32_t sp,
+                                           bool want, const Params& P, uint32_t& nLTP, uint32_t& nLTD) {
+    const uint32_t co = want ? sp : 0u;
+    const uint32_t nco = popc(co);
+    if (P.phase == 1u) {
+        bs_inc(a0, a1, a2, a3, co);
+        nLTP += nco;
+    } else if (P.phase == 2u) {
+        for (uint32_t rr = 0; rr < P.ltdRep; ++rr) bs_dec(a0, a1, a2, a3, co);
+        nLTD += nco * P.ltdRep;
+    } else {
+        bs_inc(a0, a1, a2, a3, co);
+        bs_dec(a0, a1, a2, a3, (want && P.depress != 0u) ? ~sp : 0u);
+    }
+    if (P.hetero != 0u && P.phase == 1u) bs_dec(a0, a1, a2, a3, want ? ~sp : 0u);
+    if (P.bandHi != 0u) bs_clip(a0, a1, a2, a3, P.bandLo, P.bandHi);
+}
+
+// ============================================================== init
+// k_init_w, mm:86-104
+__global__ __launch_bounds__(256) void k_init_w(uint4* __restrict__ W0, uint4* __restrict__ W1,
+                                                uint4* __restrict__ W2, uint4* __restrict__ W3, uint32_t nAnd,
+                                                uint32_t seed, uint32_t n) {
+    const uint32_t gg = blockIdx.x * 256u + threadIdx.x;
+    if (gg >= n) return;
+    const uint32_t g = gg ^ (seed * 0x9E3779B9u);
+    uint4 m = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    for (uint32_t i = 0; i < nAnd; ++i) {
+        m.x &= hash32(g * 32u + i * 4u + 0u);
+        m.y &= hash32(g * 32u + i * 4u + 1u);
+        m.z &= hash32(g * 32u + i * 4u + 2u);
+        m.w &= hash32(g * 32u + i * 4u + 3u);
+    }
+    const uint4 c = make_uint4(hash32(g * 8u + 11u), hash32(g * 8u + 13u), hash32(g * 8u + 17u),
+                               hash32(g * 8u + 19u));
+    W3[gg] = m;
+    W2[gg] = make_uint4(~m.x | c.x, ~m.y | c.y, ~m.z | c.z, ~m.w | c.w);
+    W1[gg] = c;
+    W0[gg] = make_uint4(~c.x, ~c.y, ~c.z, ~c.w);
+}
+
+// k_init_conn, mm:109-117
+__global__ __launch_bounds__(256) void k_init_conn(uint32_t* __restrict__ conn, uint32_t nGrp, uint32_t nInGrp,
+                                                   uint32_t inBr, uint32_t seed, uint32_t n) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t s = g % NSLOT;
+    const uint32_t h = hash32(g * 2246822519u + 7u + seed * 2654435761u);
+    conn[g] = (s < inBr) ? (h % nInGrp) : (nInGrp + h % (nGrp - nInGrp));
+}
+
+__global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) p[i] = v;
+}
+
+// Start of a presentation: both spike buffers and the stats words to zero
+// (k_clear mm:497, memset of bStat mm:824).  A kernel, not a memset, so the
+// whole presentation can be captured in one hipGraph.
+__global__ __launch_bounds__(256) void k_clear(uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint32_t ncol,
+                                               uint32_t* __restrict__ st, uint32_t nst) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < ncol) { a[i] = 0u; b[i] = 0u; }
+    if (i < nst) st[i] = 0u;
+}
+
+// ============================================================== propagation
+// k_drive, mm:150-190.  One wave per column; lane = (j, neuron).
+// Branches whose presynaptic group carries no spike are skipped wave-uniformly
+// (mm:169); the remaining tiles are fetched four at a time so several 1 KiB
+// loads are in flight per wave.  Spike words of a group are wave-uniform and
+// come through scalar loads.
+__global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
+    const uint32_t lane = lane_id();
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (col >= d.ncol) return;
+    const bool hi = lane >= 32u;
+    const uint32_t n = lane & 31u;
+    const uint32_t base = col * NSLOT;
+
+    const uint32_t gl = d.conn[base + n];
+    const uint4 sp = reinterpret_cast<const uint4*>(spkIn)[2u * gl + (hi ? 1u : 0u)];
+    const uint32_t anyh = sp.x | sp.y | sp.z | sp.w;
+    uint32_t mask = ballot32((anyh | __shfl_xor(anyh, 32)) != 0u);
+
+    const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
+    uint32_t acc = 0u, bb = 0u;
+    while (mask) {
+        uint32_t sidx[4];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (mask) { sidx[k] = __builtin_ctz(mask); mask &= mask - 1u; ++cnt; }
+            else sidx[k] = sidx[0];
+        }
+        uint4 w[4];
+        uint4 sv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            w[k] = Wc[(size_t)sidx[k] * TILE_U4];
+            const uint32_t g = __builtin_amdgcn_readlane(gl, sidx[k]);
+            const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
+            const uint4 lo4 = s4[0], hi4 = s4[1];
+            sv[k] = hi ? hi4 : lo4;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if ((uint32_t)k >= cnt) break;
+            uint32_t c = popc(w[k].x & sv[k].x) + popc(w[k].y & sv[k].y) + popc(w[k].z & sv[k].z) +
+                         popc(w[k].w & sv[k].w);
+            c += __shfl_xor(c, 32);
+            const bool nmda = c >= P.dthr;
+            if (nmda) bb |= 1u << sidx[k];
+            if (P.mode == 0u) acc += c;
+            else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
+            else acc += nmda ? 1u : 0u;
+        }
+    }
+    if (P.facil != 0u && ((spkIn[col] >> n) & 1u) != 0u) acc += P.facil;
+    const uint32_t gid = base + n;
+    const uint32_t bi = (P.useBias != 0u) ? (d.bias[gid] >> P.biasShift) : 0u;
+    acc = (acc > bi) ? (acc - bi) : 0u;
+    if (!hi) { d.acc[gid] = acc; d.bbits[gid] = bb; }
+    const uint32_t drv = half_sum(acc);
+    if (lane == 0u) d.drive[col] = drv;
+}
+
+// ============================================================== column top-K
+// theta = target-th largest drive over recurrent columns by three byte-wise
+// histogram passes (k_topk_par mm:301-352, k_topk_serial mm:259-292).  One
+// workgroup per instance.  Histogram bins are accumulated with one LDS atomic
+// per distinct bin per wave; the 256-bin suffix scan and the crossing bin are
+// done by wave 0 in registers.
+__device__ void topk_block(const uint32_t* __restrict__ drive, uint32_t* __restrict__ theta, uint32_t ncol,
+                           uint32_t inCols, uint32_t target, uint32_t par) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_need, sh_prefix;
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    if (tid == 0) { sh_need = target; sh_prefix = 0u; }
+    for (int by = 2; by >= 0; --by) {
+        if (tid < 256u) hist[tid] = 0u;
+        __syncthreads();
+        const uint32_t need = sh_need, prefix = sh_prefix;
+        const uint32_t shHi = (uint32_t)(by + 1) * 8u, shLo = (uint32_t)by * 8u;
+        for (uint32_t c0 = inCols; c0 < ncol; c0 += blockDim.x) {
+            const uint32_t c = c0 + tid;
+            uint32_t bin = 0;
+            bool act = false;
+            if (c < ncol) {
+                const uint32_t dv = drive[c];
+                act = (dv >> shHi) == (prefix >> shHi);
+                bin = (dv >> shLo) & 255u;
+            }
+            uint64_t m = __ballot(act);
+            while (m) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                const uint32_t lb = __builtin_amdgcn_readlane(bin, leader);
+                const uint64_t same = __ballot(act && bin == lb);
+                if (lane == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(same));
+                m &= ~same;
+            }
+        }
+        __syncthreads();
+        if (tid < 64u) {
+            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                           h3 = hist[4 * lane + 3];
+            const uint32_t loc = h0 + h1 + h2 + h3;
+            uint32_t S = loc;  // inclusive suffix sum over lanes
+#pragma unroll
+            for (uint32_t off = 1; off < 64u; off <<= 1) {
+                const uint32_t y = __shfl_down(S, off);
+                if (lane + off < 64u) S += y;
+            }
+            const uint32_t above4 = S - loc;
+            const uint32_t sc3 = h3 + above4, sc2 = h2 + sc3, sc1 = h1 + sc2, sc0 = S;
+            // largest bin i with scan[i] >= need
+            int best = -1;
+            if (sc0 >= need) best = 4 * (int)lane;
+            if (sc1 >= need) best = 4 * (int)lane + 1;
+            if (sc2 >= need) best = 4 * (int)lane + 2;
+            if (sc3 >= need) best = 4 * (int)lane + 3;
+#pragma unroll
+            for (uint32_t off = 32; off >= 1u; off >>= 1) best = max(best, __shfl_xor(best, off));
+            auto scan_at = [&](uint32_t i) -> uint32_t {  // i < 256, wave-uniform
+                const uint32_t q = i & 3u, src = i >> 2;
+                const uint32_t v = q == 0 ? sc0 : q == 1 ? sc1 : q == 2 ? sc2 : sc3;
+                return __shfl(v, (int)src);
+            };
+            uint32_t b, above;
+            if (par) {
+                if (best >= 0 && need > 0u) {
+                    b = (uint32_t)best;
+                    above = b + 1u < 256u ? scan_at(b + 1u) : 0u;
+                } else {
+                    b = 0u;
+                    above = scan_at(1u);
+                }
+            } else {
+                if (best >= 0) {
+                    b = (uint32_t)best;
+                    above = b + 1u < 256u ? scan_at(b + 1u) : 0u;
+                } else {
+                    b = 0u;
+                    above = 0u;
+                }
+            }
+            if (tid == 0) { sh_need = need - above; sh_prefix = prefix | (b << shLo); }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) { theta[0] = max(sh_prefix, 1u); theta[1] = sh_need; }
+}
+
+__global__ __launch_bounds__(256) void k_topk(NetDev d, Params P) {
+    topk_block(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+}
+
+__global__ __launch_bounds__(256) void k_topk_test(const uint32_t* __restrict__ drive, uint32_t* theta,
+                                                   uint32_t ncol, uint32_t inCols, uint32_t target, uint32_t par) {
+    topk_block(drive + (size_t)blockIdx.x * ncol, theta + 2 * blockIdx.x, ncol, inCols, target, par);
+}
+
+// ============================================================== readout + plasticity
+// k_emit (mm:192-236), k_bias (mm:240-253) and k_learn (mm:436-495) fused per
+// column: everything after the global theta is local to the column.
+//   LEARN = false: 4 columns per workgroup, one wave each (readout steps).
+//   LEARN = true : 1 column per workgroup; wave 0 emits and updates bias, and
+//                  each of the 4 waves applies plasticity to 8 of the 32
+//                  branches, so a learning column's 32 tile read-modify-writes
+//                  are spread over 4 waves.
+template <bool LEARN>
+__global__ __launch_bounds__(256) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                              uint32_t* __restrict__ spkOut) {
+    __shared__ uint32_t sLTP[4][32], sLTD[4][32];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t n = lane & 31u;
+    const bool hi = lane >= 32u;
+    const uint32_t col = LEARN ? blockIdx.x : blockIdx.x * 4u + wave;
+    if (!LEARN && col >= d.ncol) return;
+    const bool writer = LEARN ? (wave == 0u) : true;
+    const uint32_t gid = col * NPC + n;
+    const uint32_t th = max(d.theta[0], 1u);
+    const bool clampCol = P.clamp != 0u && col < P.inCols;
+    const bool gatePass = d.drive[col] >= th;
+    if (!writer && !(clampCol || gatePass)) return;  // column silent: nothing to learn
+
+    const uint32_t v = d.acc[gid];
+    const Kwta kw = kwta32(v, P.k, n);
+    uint32_t b = gatePass ? kw.ball : 0u;
+    if (clampCol) b = d.stim[col];
+    const bool post = ((b >> n) & 1u) != 0u;
+
+    if (writer) {
+        const bool rec = col >= P.inCols;
+        const uint32_t x = (!hi && post && rec) ? popc(d.bbits[gid]) : 0u;
+        const bool mem = !hi && rec && ((d.refBall[col] >> n) & 1u) != 0u;
+        const bool shared = d.mult[gid] > 1u;
+        const uint32_t sumS = wave_sum(mem && shared ? v : 0u);
+        const uint32_t sumE = wave_sum(mem && !shared ? v : 0u);
+        const uint32_t cntS = popc(ballot32(mem && shared));
+        const uint32_t cntE = popc(ballot32(mem && !shared));
+        const uint32_t xs = wave_sum(x);
+        if (lane == 0u) {
+            const uint32_t t = P.step;
+            if (xs) atomicAdd(&d.stats[2], xs);
+            if (cntS) { atomicAdd(&d.driveLog[t * 4u + 0u], sumS); atomicAdd(&d.driveLog[t * 4u + 1u], cntS); }
+            if (cntE) { atomicAdd(&d.driveLog[t * 4u + 2u], sumE); atomicAdd(&d.driveLog[t * 4u + 3u], cntE); }
+            spkOut[col] = b;
+            d.marg[col] = (kw.vk > kw.vk1) ? 1u : 0u;
+            if (P.histOn != 0u) d.hist[(size_t)t * d.ncol + col] = b;
+            if (b != 0u && rec) {
+                const uint32_t sb = 64u + t * 8u;
+                atomicAdd(&d.stats[0], 1u);
+                atomicAdd(&d.stats[1], popc(b));
+                atomicAdd(&d.stats[sb + 0u], kw.vk - kw.vk1);
+                atomicAdd(&d.stats[sb + 1u], kw.vk);
+                atomicAdd(&d.stats[sb + 2u], 1u);
+                if (kw.vk == kw.vk1) atomicAdd(&d.stats[sb + 3u], 1u);
+                if (kw.vk == 0u) atomicAdd(&d.stats[sb + 4u], 1u);
+            }
+        }
+        if (LEARN && !hi && rec) {  // k_bias
+            const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
+            const int nb = (int)d.bias[gid] + (post ? (int)P.biasUp : -dec);
+            d.bias[gid] = (uint32_t)max(0, nb);
+        }
+    }
+    if (!LEARN) return;
+    // ---- k_learn for branches 8*wave .. 8*wave+7 ----
+    if (P.phase == 3u || b == 0u) return;
+    if (P.marginGate != 0u && !(kw.vk > kw.vk1)) return;
+    const uint32_t bb = d.bbits[gid];
+    uint32_t voted = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8u; ++i) {
+        const uint32_t s = wave * 8u + i;
+        if (ballot32(post && ((bb >> s) & 1u) != 0u)) voted |= 1u << i;
+    }
+    uint32_t nLTP = 0, nLTD = 0;
+    const size_t tileBase = (size_t)col * NSLOT * TILE_U4 + lane;
+    while (voted) {
+        uint32_t si[2];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (voted) { si[k] = wave * 8u + __builtin_ctz(voted); voted &= voted - 1u; ++cnt; }
+            else si[k] = si[0];
+        }
+        uint4 a0[2], a1[2], a2[2], a3[2], sp[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
+            a0[k] = d.W0[idx]; a1[k] = d.W1[idx]; a2[k] = d.W2[idx]; a3[k] = d.W3[idx];
+            const uint32_t g = __builtin_amdgcn_readfirstlane(d.conn[col * NSLOT + si[k]]);
+            const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
+            const uint4 lo4 = s4[0], hi4 = s4[1];
+            sp[k] = hi ? hi4 : lo4;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if ((uint32_t)k >= cnt) break;
+            const bool want = post && ((bb >> si[k]) & 1u) != 0u;
+            learn_word(a0[k].x, a1[k].x, a2[k].x, a3[k].x, sp[k].x, want, P, nLTP, nLTD);
+            learn_word(a0[k].y, a1[k].y, a2[k].y, a3[k].y, sp[k].y, want, P, nLTP, nLTD);
+            learn_word(a0[k].z, a1[k].z, a2[k].z, a3[k].z, sp[k].z, want, P, nLTP, nLTD);
+            learn_word(a0[k].w, a1[k].w, a2[k].w, a3[k].w, sp[k].w, want, P, nLTP, nLTD);
+            const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
+            d.W0[idx] = a0[k]; d.W1[idx] = a1[k]; d.W2[idx] = a2[k]; d.W3[idx] = a3[k];
+        }
+    }
+    // per-neuron totals over both halves and all four waves (mm:491-494 counts per neuron)
+    nLTP += __shfl_xor(nLTP, 32);
+    nLTD += __shfl_xor(nLTD, 32);
+    if (!hi) { sLTP[wave][n] = nLTP; sLTD[wave][n] = nLTD; }
+    __syncthreads();
+    if (wave == 0u) {
+        const uint32_t tp = hi ? 0u : sLTP[0][n] + sLTP[1][n] + sLTP[2][n] + sLTP[3][n];
+        const uint32_t td = hi ? 0u : sLTD[0][n] + sLTD[1][n] + sLTD[2][n] + sLTD[3][n];
+        const uint32_t sp_ = wave_sum(tp), sd_ = wave_sum(td);
+        const uint32_t cp = popc(ballot32(tp != 0u)), cd = popc(ballot32(td != 0u));
+        if (lane == 0u) {
+            if (sp_) atomicAdd(&d.stats[8], sp_);
+            if (sd_) atomicAdd(&d.stats[9], sd_);
+            if (cp) atomicAdd(&d.stats[10], cp);
+            if (cd) atomicAdd(&d.stats[11], cd);
+        }
+    }
+}
+
+// ============================================================== readouts
+// k_wstat, mm:641-646: functional synapse count (popcount of W3)
+__global__ __launch_bounds__(256) void k_wstat(const uint4* __restrict__ W3, size_t n, uint32_t* out) {
+    uint32_t s = 0;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) {
+        const uint4 w = W3[i];
+        s += popc(w.x) + popc(w.y) + popc(w.z) + popc(w.w);
+    }
+    s = wave_sum(s);
+    if (lane_id() == 0u) atomicAdd(out, s);
+}
+
+// k_within, mm:524-562.  One thread per postsynaptic neuron (lane = neuron of a
+// 32-lane half); the ten counters are reduced per wave before the atomics.
+__global__ __launch_bounds__(256) void k_within(NetDev d, const uint32_t* __restrict__ ball,
+                                                const uint32_t* __restrict__ postMask, uint32_t useMask,
+                                                uint32_t* out) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t col = gid >> 5, n = gid & 31u;
+    bool act = gid < d.ncol * NPC && col >= d.incols;
+    if (act) act = useMask ? (postMask[gid] != 0u) : (((ball[col] >> n) & 1u) != 0u);
+    uint32_t fIn = 0, pIn = 0, fOut = 0, pOut = 0;
+    bool shared = false;
+    if (act) {
+        const uint32_t base = col * NSLOT;
+        for (uint32_t s = d.inbr; s < NSLOT; ++s) {
+            const uint32_t g = d.conn[base + s];
+            for (uint32_t j = 0; j < 2u; ++j) {
+                const uint4 ap = reinterpret_cast<const uint4*>(ball)[2u * g + j];
+                const uint4 w = d.W3[(size_t)(base + s) * TILE_U4 + j * 32u + n];
+                fIn += popc(w.x & ap.x) + popc(w.y & ap.y) + popc(w.z & ap.z) + popc(w.w & ap.w);
+                pIn += popc(ap.x) + popc(ap.y) + popc(ap.z) + popc(ap.w);
+                fOut += popc(w.x & ~ap.x) + popc(w.y & ~ap.y) + popc(w.z & ~ap.z) + popc(w.w & ~ap.w);
+                pOut += popc(~ap.x) + popc(~ap.y) + popc(~ap.z) + popc(~ap.w);
+            }
+        }
+        shared = d.mult[gid] > 1u;
+    }
+    const uint32_t v[10] = {fIn, pIn, fOut, pOut, shared ? fIn : 0u, shared ? pIn : 0u, shared ? 0u : fIn,
+                            shared ? 0u : pIn, (act && shared) ? 1u : 0u, (act && !shared) ? 1u : 0u};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint32_t s = wave_sum(v[i]);
+        if (lane_id() == 0u && s) atomicAdd(&out[i], s);
+    }
+}
+
+// k_indeg, mm:566-581
+__global__ __launch_bounds__(256) void k_indeg(NetDev d, uint32_t* __restrict__ out) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    if (gid >= d.ncol * NPC) return;
+    const uint32_t col = gid >> 5, n = gid & 31u;
+    uint32_t dd = 0;
+    for (uint32_t s = d.inbr; s < NSLOT; ++s)
+        for (uint32_t j = 0; j < 2u; ++j) {
+            const uint4 w = d.W3[(size_t)(col * NSLOT + s) * TILE_U4 + j * 32u + n];
+            dd += popc(w.x) + popc(w.y) + popc(w.z) + popc(w.w);
+        }
+    out[gid] = dd;
+}
+
+// k_untouched, mm:501-519: synapses still at the initial values 5 or 6
+__global__ __launch_bounds__(256) void k_untouched(NetDev d, uint32_t* __restrict__ out) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    if (gid >= d.ncol * NPC) return;
+    const uint32_t col = gid >> 5, n = gid & 31u;
+    uint32_t u = 0;
+    for (uint32_t s = d.inbr; s < NSLOT; ++s)
+        for (uint32_t j = 0; j < 2u; ++j) {
+            const size_t idx = (size_t)(col * NSLOT + s) * TILE_U4 + j * 32u + n;
+            const uint4 a0 = d.W0[idx], a1 = d.W1[idx], a2 = d.W2[idx], a3 = d.W3[idx];
+            auto f = [](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
+                return popc(((~x3) & x2 & (~x1) & x0) | ((~x3) & x2 & x1 & (~x0)));
+            };
+            u += f(a0.x, a1.x, a2.x, a3.x) + f(a0.y, a1.y, a2.y, a3.y) + f(a0.z, a1.z, a2.z, a3.z) +
+                 f(a0.w, a1.w, a2.w, a3.w);
+        }
+    out[gid] = u;
+}
+
+// k_wdiag, mm:591-638: 4-bit counter histogram split by pre/post assembly relation
+__global__ __launch_bounds__(256) void k_wdiag(NetDev d, const uint32_t* __restrict__ memb,
+                                               const uint32_t* __restrict__ asmSpk, uint32_t nstim, uint32_t sel,
+                                               uint32_t* __restrict__ hist) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    if (gid >= d.ncol * NPC) return;
+    const uint32_t mypost = memb[gid];
+    if (mypost == 0u) return;
+    const uint32_t col = gid >> 5, n = gid & 31u, base = col * NSLOT;
+    uint32_t acc[48];
+    for (int i = 0; i < 48; ++i) acc[i] = 0u;
+    for (uint32_t s = 0; s < NSLOT; ++s) {
+        const bool isAff = s < d.inbr;
+        if (sel == 0u && isAff) continue;
+        if (sel == 1u && !isAff) continue;
+        const uint32_t g = d.conn[base + s];
+        for (uint32_t j = 0; j < 2u; ++j) {
+            const size_t idx = (size_t)(base + s) * TILE_U4 + j * 32u + n;
+            const uint4 A0 = d.W0[idx], A1 = d.W1[idx], A2 = d.W2[idx], A3 = d.W3[idx];
+            const uint32_t a0[4] = {A0.x, A0.y, A0.z, A0.w}, a1[4] = {A1.x, A1.y, A1.z, A1.w},
+                           a2[4] = {A2.x, A2.y, A2.z, A2.w}, a3[4] = {A3.x, A3.y, A3.z, A3.w};
+            for (int q = 0; q < 4; ++q) {
+                uint32_t within = 0, anyp = 0;
+                for (uint32_t st = 0; st < nstim; ++st) {
+                    const uint32_t ap = asmSpk[(size_t)st * d.ncol + 8u * g + 4u * j + q];
+                    anyp |= ap;
+                    if (((mypost >> st) & 1u) != 0u) within |= ap;
+                }
+                const uint32_t between = anyp & ~within;
+                for (uint32_t v = 0; v < 16u; ++v) {
+                    const uint32_t eq = ((v & 1u) ? a0[q] : ~a0[q]) & ((v & 2u) ? a1[q] : ~a1[q]) &
+                                        ((v & 4u) ? a2[q] : ~a2[q]) & ((v & 8u) ? a3[q] : ~a3[q]);
+                    acc[v] += popc(eq & within);
+                    acc[16 + v] += popc(eq & between);
+                    acc[32 + v] += popc(eq);
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 48; ++i)
+        if (acc[i]) atomicAdd(&hist[i], acc[i]);
+}
+
+// k_bstest, mm:424-434
+__global__ __launch_bounds__(256) void k_bstest(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1,
+                                                uint32_t* a2, uint32_t* a3, const uint32_t* __restrict__ m,
+                                                size_t n) {
+    const size_t i = blockIdx.x * 256ull + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x0 = a0[i], x1 = a1[i], x2 = a2[i], x3 = a3[i];
+    if (op == 0u) bs_inc(x0, x1, x2, x3, m[i]);
+    else if (op == 1u) bs_dec(x0, x1, x2, x3, m[i]);
+    else bs_clip(x0, x1, x2, x3, lo, hi);
+    a0[i] = x0; a1[i] = x1; a2[i] = x2; a3[i] = x3;
+}
+
+__global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ v, uint32_t ncols, uint32_t k,
+                                                   uint32_t* ball, uint32_t* vk, uint32_t* vk1) {
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (col >= ncols) return;
+    const uint32_t n = lane_id() & 31u;
+    const Kwta r = kwta32(v[(size_t)col * 32u + n], k, n);
+    if (lane_id() == 0u) { ball[col] = r.ball; vk[col] = r.vk; vk1[col] = r.vk1; }
+}
+
+inline uint32_t blocks_for(size_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+}  // namespace
+
+// ============================================================== launchers
+hipError_t launch_init(const NetDev& d, uint32_t nand, uint32_t seed, hipStream_t s) {
+    const uint32_t planeU4 = d.ncol * NSLOT * TILE_U4;
+    k_init_w<<<blocks_for(planeU4, 256), 256, 0, s>>>(d.W0, d.W1, d.W2, d.W3, nand, seed, planeU4);
+    const uint32_t nconn = d.ncol * NSLOT;
+    k_init_conn<<<blocks_for(nconn, 256), 256, 0, s>>>(d.conn, d.ncol / COLS_PER_GRP, d.incols / COLS_PER_GRP,
+                                                        d.inbr, seed, nconn);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<size_t>(blocks_for(n, 256), 4096);
+    k_fill<<<blocks, 256, 0, s>>>(p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(const NetDev& d, hipStream_t s) {
+    const uint32_t n = std::max(d.ncol, d.nstats);
+    k_clear<<<blocks_for(n, 256), 256, 0, s>>>(d.spk[0], d.spk[1], d.ncol, d.stats, d.nstats);
+    return hipGetLastError();
+}
+
+hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s) {
+    const uint32_t* in = d.spk[t & 1u];
+    k_drive<<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, P, in);
+    return hipGetLastError();
+}
+
+hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s) {
+    k_topk<<<1, 256, 0, s>>>(d, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, hipStream_t s) {
+    const uint32_t* in = d.spk[t & 1u];
+    uint32_t* out = d.spk[(t & 1u) ^ 1u];
+    if (learn) k_emit<true><<<d.ncol, 256, 0, s>>>(d, P, in, out);
+    else k_emit<false><<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, P, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s) {
+    const size_t n = (size_t)d.ncol * NSLOT * TILE_U4;
+    k_wstat<<<2048, 256, 0, s>>>(d.W3, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_within(const NetDev& d, const uint32_t* ball, const uint32_t* postMask, uint32_t useMask,
+                         uint32_t* out, hipStream_t s) {
+    k_within<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, ball, postMask, useMask, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_indeg(const NetDev& d, uint32_t* out, hipStream_t s) {
+    k_indeg<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_untouched(const NetDev& d, uint32_t* out, hipStream_t s) {
+    k_untouched<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wdiag(const NetDev& d, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
+                        uint32_t sel, uint32_t* hist, hipStream_t s) {
+    k_wdiag<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, memb, asmSpk, nstim, sel, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_bstest(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1, uint32_t* a2,
+                         uint32_t* a3, const uint32_t* m, size_t n, hipStream_t s) {
+    k_bstest<<<blocks_for(n, 256), 256, 0, s>>>(op, lo, hi, a0, a1, a2, a3, m, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_kwta_test(const uint32_t* v, uint32_t ncols, uint32_t k, uint32_t* ball, uint32_t* vk,
+                            uint32_t* vk1, hipStream_t s) {
+    k_kwta_test<<<blocks_for(ncols, 4), 256, 0, s>>>(v, ncols, k, ball, vk, vk1);
+    return hipGetLastError();
+}
+
+hipError_t launch_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol, uint32_t incols, uint32_t target,
+                            uint32_t par, uint32_t* theta, hipStream_t s) {
+    k_topk_test<<<ninst, 256, 0, s>>>(drive, theta, ncol, incols, target, par);
+    return hipGetLastError();
+}
+
+}  // namespace axon

@@ -1,0 +1,147 @@
+/*
+ * axon_api.h -- C ABI of the MI355X AXON engine (libaxon.so).
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, opaque handles, no
+ * C++ or torch types.  Every entry point returns an axon_status and never
+ * aborts; axon_last_error() gives the message for the calling thread.
+ *
+ * Each declaration names the reference interface it replaces.  The reference
+ * is a single Objective-C++/Metal program (src/axon.mm): its "interface" for
+ * the hot path is the key=value CLI, the present() lambda and the kernel
+ * buffer-slot tables.  Citations are src/axon.mm line numbers (mm:N).
+ *
+ * Ownership: the engine owns all device memory.  Output pointers are caller
+ * host buffers sized from the config (NCOL = ncol, N = ncol * 32); NULL means
+ * "do not copy this output".
+ */
+#ifndef AXON_API_H
+#define AXON_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXON_ABI_VERSION 1u
+
+typedef enum axon_status {
+    AXON_OK = 0,
+    AXON_ERR_INVALID_CONFIG = 1, /* a config the reference would run into undefined behaviour */
+    AXON_ERR_OUT_OF_MEMORY = 2,
+    AXON_ERR_BUDGET_EXCEEDED = 3, /* serving admission: request larger than the HBM budget */
+    AXON_ERR_DEVICE = 4,          /* HIP runtime error or no usable gfx950 device */
+    AXON_ERR_INVALID_ARG = 5,
+    AXON_ERR_CAPACITY = 6, /* e.g. T beyond the history capacity max_t */
+    AXON_ERR_NOT_READY = 7 /* serving: no result available yet */
+} axon_status;
+
+/* Every key=value tunable of the reference CLI, mm:687-735 (defaults mm:688-728,
+ * derived rules mm:731-735).  Field order matches the oracle's ora_config. */
+typedef struct axon_config {
+    uint32_t ncol, incols, nand, target, k, dthr, boost, mode;
+    uint32_t epochs, T, clampsteps, stim, seed, quiet, split, unionchar;
+    uint32_t biasup, biasdown, biasshift, biasdownevery, inbr, stimcols, diag, profile;
+    uint32_t topkpar, vtopk, contrast, bandlo, bandhi, margingate, hetero, vbs;
+    uint32_t ltdrep, ltdevery, ktest, facil, deep, wave, wavet, trainclamp;
+    uint32_t max_t; /* history capacity in steps; the reference fixes MAXT = 64 (mm:766) */
+    uint32_t reserved[7];
+} axon_config;
+
+typedef struct axon_net axon_net;
+
+uint32_t axon_abi_version(void);
+const char* axon_status_str(axon_status s);
+const char* axon_last_error(void);
+
+/* ---- configuration: replaces argu() (mm:660-667) and the tunables block (mm:687-735) ---- */
+void axon_config_defaults(axon_config* c);
+/* key=value, strncmp prefix + '=', last occurrence wins, atoi -> u32, then the
+ * derived rules of mm:731-735 (trainclamp default, biasup=1 auto). */
+axon_status axon_config_parse(int argc, const char* const* argv, axon_config* c);
+axon_status axon_config_validate(const axon_config* c);
+
+/* ---- network: replaces the buffer block mm:749-768, k_init_w/k_init_conn
+ * dispatch mm:770-782 and the stimulus draw mm:793-799 ---- */
+axon_status axon_net_create(const axon_config* c, axon_net** out);
+void axon_net_destroy(axon_net* n);
+
+/* ---- one presentation: replaces present() mm:815-888 ----
+ * use_bias: 0, 1 or 2 (= auto: on while learning, off at readout, mm:822).
+ * stim: optional NCOL-word clamp pattern replacing loadStim(pattern, keep, seed)
+ * (mm:800-804); NULL uses the reference draw. */
+typedef struct axon_present_args {
+    uint32_t pattern;
+    uint32_t T;
+    uint32_t clamp_steps;
+    uint32_t learn;
+    uint32_t freeze; /* accepted for signature parity; no kernel reads it (mm:840) */
+    uint32_t seed;
+    uint32_t use_bias;
+    uint32_t flags; /* AXON_PRESENT_ASYNC: enqueue only, outputs must be NULL */
+    double keep;
+    const uint32_t* stim;
+} axon_present_args;
+
+#define AXON_PRESENT_ASYNC 1u
+
+typedef struct axon_present_out {
+    uint32_t* spikes;    /* NCOL: final ballots, (T&1)?B:A (mm:891) */
+    uint32_t* hist;      /* T*NCOL: per-step ballots (k_emit hist, mm:224) */
+    uint32_t* stats;     /* stats words (>= 1024, mm:214-233, 491-494) */
+    uint32_t* drive_log; /* driveLog words (>= 256, mm:216-220) */
+    uint32_t* theta;     /* 2 words (mm:291/351) */
+    double ms_per_step;  /* host wall time per step, as present() returns */
+} axon_present_out;
+
+axon_status axon_present(axon_net* n, const axon_present_args* a, axon_present_out* out);
+axon_status axon_sync(axon_net* n);
+
+/* host writes into Shared buffers of the reference (bRefBall mm:1700, bMult
+ * mm:1604/1656, bDriveLog memset mm:1713) */
+axon_status axon_set_ref_ball(axon_net* n, const uint32_t* ball /* NCOL */);
+axon_status axon_set_mult(axon_net* n, const uint32_t* mult /* N */);
+axon_status axon_clear_drive_log(axon_net* n);
+
+/* raw engine buffers, for snapshots and parity checks */
+typedef enum axon_buffer {
+    AXON_BUF_W0 = 0, AXON_BUF_W1, AXON_BUF_W2, AXON_BUF_W3, AXON_BUF_CONN, AXON_BUF_SPK_A,
+    AXON_BUF_SPK_B, AXON_BUF_DRIVE, AXON_BUF_BBITS, AXON_BUF_ACC, AXON_BUF_BIAS, AXON_BUF_MARG,
+    AXON_BUF_REFBALL, AXON_BUF_DRIVELOG, AXON_BUF_MULT, AXON_BUF_STIM, AXON_BUF_STATS,
+    AXON_BUF_HIST, AXON_BUF_THETA, AXON_BUF_COUNT
+} axon_buffer;
+axon_status axon_buffer_words(const axon_net* n, axon_buffer which, size_t* nwords);
+axon_status axon_read_buffer(axon_net* n, axon_buffer which, void* dst, size_t bytes);
+axon_status axon_write_buffer(axon_net* n, axon_buffer which, const void* src, size_t bytes);
+/* switches the column top-K between the parallel (1) and serial (0) selection
+ * semantics, the pTopK switch of mm:806-809 used by the vtopk check mm:966-992 */
+axon_status axon_net_set_topk_parallel(axon_net* n, uint32_t par);
+/* stimCols[p] of mm:798 (stimcols entries) */
+axon_status axon_stim_cols(const axon_net* n, uint32_t p, uint32_t* out);
+
+/* ---- readout kernels ---- */
+axon_status axon_wstat(axon_net* n, uint32_t* functional);  /* k_wstat mm:641-646, wstat() mm:784-791 */
+/* k_within mm:524-562: post_mask NULL means useMask = 0 (the deep call, mm:1702-1709) */
+axon_status axon_within(axon_net* n, const uint32_t* ball, const uint32_t* post_mask, uint32_t out[16]);
+axon_status axon_indeg(axon_net* n, uint32_t* out /* N */);     /* k_indeg mm:566-581 */
+axon_status axon_untouched(axon_net* n, uint32_t* out /* N */); /* k_untouched mm:501-519 */
+/* k_wdiag mm:591-638: memb N words, asm_spk nstim*NCOL words, hist 48 words */
+axon_status axon_wdiag(axon_net* n, const uint32_t* memb, const uint32_t* asm_spk, uint32_t nstim,
+                       uint32_t sel, uint32_t hist[48]);
+
+/* ---- kernel test harnesses (replace the vbs/vtopk self-checks mm:903-994) ---- */
+/* k_bstest mm:424-434: op 0 inc, 1 dec, 2 clip[lo,hi]; nwords u32 per plane, in place */
+axon_status axon_bs_test(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1, uint32_t* a2,
+                         uint32_t* a3, const uint32_t* m, size_t nwords);
+/* kwta_ballot_margin mm:134-144 on ncols groups of 32 values */
+axon_status axon_kwta_test(const uint32_t* v, uint32_t ncols, uint32_t k, uint32_t* ball, uint32_t* vk,
+                           uint32_t* vk1);
+/* k_topk_par / k_topk_serial mm:259-352 on ninst drive vectors of ncol words */
+axon_status axon_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol, uint32_t incols,
+                           uint32_t target, uint32_t par, uint32_t* theta /* 2*ninst */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -611,6 +611,7 @@ extern "C" void ora_stim_cols(ora_net* n, uint32_t p, uint32_t* out) {
 extern "C" void ora_within(ora_net* n, const uint32_t* ball, const uint32_t* postMask, uint32_t out[16]) {
     const uint32_t inBr = n->cfg.inbr, inCols = n->IN_COLS;
     const uint32_t* W3 = n->W[3].data();
+    memset(out, 0, 16 * sizeof(uint32_t));  // bWCnt is zeroed before each call (mm:1701)
     for (uint32_t gid = 0; gid < n->N; ++gid) {
         uint32_t col = gid >> 5, lane = gid & 31u;
         if (col < inCols) continue;
@@ -677,6 +678,7 @@ extern "C" void ora_untouched(ora_net* n, uint32_t* out) {
 extern "C" void ora_wdiag(ora_net* n, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
                           uint32_t sel, uint32_t hist[48]) {
     const uint32_t inBr = n->cfg.inbr, ncol4 = n->NCOL / 4;
+    memset(hist, 0, 48 * sizeof(uint32_t));  // zeroed before each call (mm:1127)
     for (uint32_t gid = 0; gid < n->N; ++gid) {
         uint32_t mypost = memb[gid];
         if (mypost == 0u) continue;
