@@ -219,6 +219,22 @@ class Net:
             res.hist = res.hist[: T * self.ncol].reshape(T, self.ncol)
         return res
 
+    def work_counters(self, reset: bool = False):
+        """(tiles read by propagation, tiles rewritten by plasticity); 8192 synapses per tile."""
+        out = (ctypes.c_uint64 * 2)()
+        _check(lib().axon_work_counters(self._h, out, ctypes.c_uint32(int(reset))))
+        return int(out[0]), int(out[1])
+
+    def set_profiling(self, enable: bool) -> None:
+        _check(lib().axon_net_set_profiling(self._h, ctypes.c_uint32(int(enable))))
+
+    def kernel_times(self, reset: bool = False):
+        """Summed device ms and launch counts of (propagation, top-K, emit+plasticity)."""
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_uint64 * 3)()
+        _check(lib().axon_kernel_times(self._h, ms, n, ctypes.c_uint32(int(reset))))
+        return list(ms), [int(x) for x in n]
+
     def sync(self) -> None:
         _check(lib().axon_sync(self._h))
 
@@ -262,6 +278,71 @@ class Net:
                                 _ptr(np.ascontiguousarray(asm_spk, np.uint32)), ctypes.c_uint32(nstim),
                                 ctypes.c_uint32(sel), _ptr(out)))
         return out
+
+
+class Probes:
+    """A probe batch: `slots` readout presentations against one resident network."""
+
+    def __init__(self, net: Net, slots: int, max_t: int, tile: int = 32):
+        self.net = net
+        self._h = ctypes.c_void_p()
+        _check(lib().axon_probes_create(net._h, ctypes.c_uint32(slots), ctypes.c_uint32(max_t),
+                                        ctypes.c_uint32(tile), ctypes.byref(self._h)))
+        s, m, t = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().axon_probes_info(self._h, ctypes.byref(s), ctypes.byref(m), ctypes.byref(t)))
+        self.slots, self.max_t, self.tile = s.value, m.value, t.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().axon_probes_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, first: int, stims: np.ndarray, T, clamp_steps: int, use_bias: int = 0) -> None:
+        stims = np.ascontiguousarray(stims, np.uint32).reshape(-1, self.net.ncol)
+        count = stims.shape[0]
+        Ts = np.ascontiguousarray(np.broadcast_to(np.asarray(T, np.uint32), (count,)), np.uint32)
+        _check(lib().axon_probes_load(self._h, ctypes.c_uint32(first), ctypes.c_uint32(count), _ptr(stims),
+                                      _ptr(Ts), ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias)))
+
+    def step(self, steps: int) -> None:
+        _check(lib().axon_probes_step(self._h, ctypes.c_uint32(steps)))
+
+    def sync(self) -> None:
+        _check(lib().axon_probes_sync(self._h))
+
+    def read(self, first: int, count: int, hist: bool = True):
+        spk = np.zeros((count, self.net.ncol), np.uint32)
+        h = np.zeros((count, self.max_t, self.net.ncol), np.uint32) if hist else None
+        done = np.zeros(count, np.uint32)
+        _check(lib().axon_probes_read(self._h, ctypes.c_uint32(first), ctypes.c_uint32(count), _ptr(spk),
+                                      _ptr(h) if hist else None, _ptr(done)))
+        return spk, h, done
+
+    def run(self, stims: np.ndarray, T: int, clamp_steps: int, use_bias: int = 0, hist: bool = True):
+        stims = np.ascontiguousarray(stims, np.uint32).reshape(-1, self.net.ncol)
+        count = stims.shape[0]
+        h = np.zeros((count, T, self.net.ncol), np.uint32) if hist else None
+        spk = np.zeros((count, self.net.ncol), np.uint32)
+        ms = ctypes.c_double()
+        _check(lib().axon_probes_run(self._h, ctypes.c_uint32(count), _ptr(stims), ctypes.c_uint32(T),
+                                     ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias),
+                                     _ptr(h) if hist else None, _ptr(spk), ctypes.byref(ms)))
+        return spk, h, ms.value
+
+
+def probe_cue(net: Net, pattern: int, nreplace: int, seed: int) -> np.ndarray:
+    """Recall cue: pattern's stimulus columns with nreplace of them swapped for
+    other input columns; pattern = -1 draws a random cue."""
+    out = np.zeros(net.ncol, np.uint32)
+    _check(lib().axon_probe_cue(net._h, ctypes.c_uint32(pattern & 0xFFFFFFFF), ctypes.c_uint32(nreplace),
+                                ctypes.c_uint32(seed), _ptr(out)))
+    return out
 
 
 def bs_test(op: int, lo: int, hi: int, planes: Sequence[np.ndarray], mask: np.ndarray) -> list:

@@ -116,6 +116,11 @@ struct axon_net {
     uint32_t* d_mask = nullptr;   // N words
     std::map<GraphKey, hipGraphExec_t> graphs;
     bool use_graphs = true;
+    // per-kernel event timing (axon_net_set_profiling)
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_ev;  // 2 per kernel per step
+    double prof_ms[3] = {0, 0, 0};
+    uint64_t prof_n[3] = {0, 0, 0};
     std::vector<void*> allocs;
 };
 
@@ -236,6 +241,7 @@ extern "C" void axon_net_destroy(axon_net* n) {
         if (n->stim_ev[i]) (void)hipEventDestroy(n->stim_ev[i]);
         if (n->h_stim[i]) (void)hipHostFree(n->h_stim[i]);
     }
+    for (hipEvent_t e : n->prof_ev) (void)hipEventDestroy(e);
     for (void* p : n->allocs) (void)hipFree(p);
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
@@ -274,6 +280,7 @@ extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
         {&d.acc, n->N}, {&d.bbits, n->N}, {&d.drive, n->NCOL}, {&d.theta, 16}, {&d.marg, n->NCOL},
         {&d.stim, n->NCOL}, {&d.stats, n->NSTATS}, {&d.hist, (size_t)n->MAXT * n->NCOL}, {&d.driveLog, n->NDLOG},
         {&d.refBall, n->NCOL}, {&d.mult, n->N}, {&n->d_small, 64}, {&n->d_ball, n->NCOL}, {&n->d_mask, n->N},
+        {&d.cntProp, n->NCOL}, {&d.cntLearn, (size_t)n->NCOL * 4},
     };
     for (auto& b : bufs) {
         if ((st = dev_alloc(n, &p, b.words * 4)) != AXON_OK) return cleanup(st);
@@ -336,6 +343,50 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
     *err = e;
 }
 
+// Same launches as enqueue_steps, each bracketed by events so the three
+// kernels' device durations can be summed after the presentation.
+static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn) {
+    const size_t need = (size_t)6 * std::max<uint32_t>(T, 1);
+    while (n->prof_ev.size() < need) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        n->prof_ev.push_back(e);
+    }
+    hipError_t e = axon::launch_clear(n->d, n->stream);
+    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
+        P.step = t;
+        P.learn = learn ? 1u : 0u;
+        P.freeze = learn ? 0u : 1u;
+        P.clamp = (t < clampSteps) ? 1u : 0u;
+        P.depress = (learn && (t % 2u == 1u)) ? 1u : 0u;
+        P.phase = (!learn || !n->contrast) ? 0u
+                  : (t < clampSteps)      ? 1u
+                  : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
+        hipEvent_t* ev = &n->prof_ev[(size_t)6 * t];
+        if ((e = hipEventRecord(ev[0], n->stream)) != hipSuccess) break;
+        if ((e = axon::launch_drive(n->d, P, t, n->stream)) != hipSuccess) break;
+        if ((e = hipEventRecord(ev[1], n->stream)) != hipSuccess) break;
+        if ((e = hipEventRecord(ev[2], n->stream)) != hipSuccess) break;
+        if ((e = axon::launch_topk(n->d, P, n->stream)) != hipSuccess) break;
+        if ((e = hipEventRecord(ev[3], n->stream)) != hipSuccess) break;
+        if ((e = hipEventRecord(ev[4], n->stream)) != hipSuccess) break;
+        if ((e = axon::launch_emit(n->d, P, t, learn, n->stream)) != hipSuccess) break;
+        e = hipEventRecord(ev[5], n->stream);
+    }
+    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
+    for (uint32_t t = 0; t < T; ++t)
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.f;
+            if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)6 * t + 2 * k], n->prof_ev[(size_t)6 * t + 2 * k + 1])) != hipSuccess)
+                return e;
+            n->prof_ms[k] += ms;
+            n->prof_n[k] += 1;
+        }
+    return hipSuccess;
+}
+
 extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axon_present_out* o) {
     if (!n || !a) return fail(AXON_ERR_INVALID_ARG, "null argument");
     if (a->T > n->MAXT)
@@ -371,7 +422,11 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
     n->stim_ev_live[slot] = true;
 
     const auto t0 = std::chrono::steady_clock::now();
-    if (n->use_graphs && a->T > 0) {
+    if (n->profiling) {
+        if (async) return fail(AXON_ERR_INVALID_ARG, "profiling presentations are synchronous");
+        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn);
+        if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "profiled launch: %s", hipGetErrorString(e));
+    } else if (n->use_graphs && a->T > 0) {
         const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias};
         auto it = n->graphs.find(key);
         if (it == n->graphs.end()) {
@@ -497,6 +552,39 @@ extern "C" axon_status axon_net_set_topk_parallel(axon_net* n, uint32_t par) {
     n->d.topkpar = par ? 1u : 0u;
     for (auto& kv : n->graphs) (void)hipGraphExecDestroy(kv.second);  // kernel args captured the old mode
     n->graphs.clear();
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    std::vector<uint32_t> p(n->NCOL), l((size_t)n->NCOL * 4);
+    HIPCHECK(hipMemcpy(p.data(), n->d.cntProp, p.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(l.data(), n->d.cntLearn, l.size() * 4, hipMemcpyDeviceToHost));
+    out[0] = 0; out[1] = 0;
+    for (uint32_t x : p) out[0] += x;
+    for (uint32_t x : l) out[1] += x;
+    if (reset) {
+        HIPCHECK(hipMemset(n->d.cntProp, 0, p.size() * 4));
+        HIPCHECK(hipMemset(n->d.cntLearn, 0, l.size() * 4));
+    }
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_net_set_profiling(axon_net* n, uint32_t enable) {
+    if (!n) return fail(AXON_ERR_INVALID_ARG, "null net");
+    n->profiling = enable != 0u;
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_kernel_times(axon_net* n, double ms[3], uint64_t launches[3], uint32_t reset) {
+    if (!n || !ms || !launches) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    for (int k = 0; k < 3; ++k) {
+        ms[k] = n->prof_ms[k];
+        launches[k] = n->prof_n[k];
+        if (reset) { n->prof_ms[k] = 0; n->prof_n[k] = 0; }
+    }
     return AXON_OK;
 }
 
@@ -630,5 +718,218 @@ extern "C" axon_status axon_topk_test(const uint32_t* drive, uint32_t ninst, uin
     HIPCHECK(axon::launch_topk_test(dd, ninst, ncol, incols, target, par, dt, nullptr));
     HIPCHECK(hipDeviceSynchronize());
     HIPCHECK(hipMemcpy(theta, dt, (size_t)ninst * 2 * 4, hipMemcpyDeviceToHost));
+    return AXON_OK;
+}
+
+// ================================================================== probe batches
+struct axon_probes {
+    axon_net* net = nullptr;
+    axon::ProbeDev d{};
+    uint32_t tile = 32;
+    std::vector<void*> allocs;
+    std::map<uint32_t, hipGraphExec_t> graphs;  // keyed by steps per launch
+};
+
+extern "C" void axon_probes_destroy(axon_probes* b) {
+    if (!b) return;
+    if (b->net && b->net->stream) (void)hipStreamSynchronize(b->net->stream);
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (void* p : b->allocs) (void)hipFree(p);
+    delete b;
+}
+
+extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t max_t, uint32_t tile,
+                                          axon_probes** out) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (tile != 8 && tile != 16 && tile != 32 && tile != 64)
+        return fail(AXON_ERR_INVALID_ARG, "probe tile must be 8, 16, 32 or 64 (got %u)", tile);
+    if (slots == 0 || max_t == 0 || max_t > 4096) return fail(AXON_ERR_INVALID_ARG, "slots >= 1, max_t in 1..4096");
+    HIPCHECK(hipSetDevice(n->device));
+    axon_probes* b = new axon_probes();
+    b->net = n;
+    b->tile = tile;
+    const uint32_t ns = (slots + tile - 1) / tile * tile;
+    const size_t NC = n->NCOL, NN = n->N;
+    axon::ProbeDev& d = b->d;
+    d.W3 = n->d.W3; d.conn = n->d.conn; d.bias = n->d.bias;
+    d.nslots = ns; d.ncol = n->NCOL; d.incols = n->cfg.incols; d.maxt = max_t;
+    d.gw = (n->NCOL / axon::COLS_PER_GRP + 31) / 32;
+    d.k = n->kTest ? n->kTest : n->K;  // readout k (mm:818)
+    d.facil = n->facil;
+    d.dthr = n->P.dthr; d.boost = n->P.boost; d.mode = n->P.mode; d.biasShift = n->P.biasShift;
+    d.target = n->P.targetCols; d.topkpar = n->d.topkpar;
+    struct { uint32_t** p; size_t words; } bufs[] = {
+        {&d.spk, ns * NC}, {&d.gact, (size_t)ns * d.gw}, {&d.acc, ns * NN}, {&d.drive, ns * NC},
+        {&d.stim, ns * NC}, {&d.hist, (size_t)ns * max_t * NC}, {&d.theta, (size_t)ns * 2},
+        {&d.st_t, ns}, {&d.st_T, ns}, {&d.st_clamp, ns}, {&d.st_bias, ns}};
+    for (auto& x : bufs) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, x.words * 4);
+        if (e != hipSuccess) {
+            axon_probes_destroy(b);
+            return fail(AXON_ERR_OUT_OF_MEMORY, "probe batch: hipMalloc(%zu): %s", x.words * 4, hipGetErrorString(e));
+        }
+        b->allocs.push_back(p);
+        *x.p = (uint32_t*)p;
+        if (hipMemsetAsync(p, 0, x.words * 4, n->stream) != hipSuccess) {
+            axon_probes_destroy(b);
+            return fail(AXON_ERR_DEVICE, "hipMemsetAsync failed");
+        }
+    }
+    if (hipStreamSynchronize(n->stream) != hipSuccess) {
+        axon_probes_destroy(b);
+        return fail(AXON_ERR_DEVICE, "probe batch init failed");
+    }
+    *out = b;
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_info(const axon_probes* b, uint32_t* slots, uint32_t* max_t, uint32_t* tile) {
+    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
+    if (slots) *slots = b->d.nslots;
+    if (max_t) *max_t = b->d.maxt;
+    if (tile) *tile = b->tile;
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_load(axon_probes* b, uint32_t first, uint32_t count, const uint32_t* stims,
+                                        const uint32_t* T, uint32_t clamp_steps, uint32_t use_bias) {
+    if (!b || !stims || !T) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (first + count > b->d.nslots || first + count < first)
+        return fail(AXON_ERR_INVALID_ARG, "slots [%u,%u) beyond %u", first, first + count, b->d.nslots);
+    if (use_bias > 1) return fail(AXON_ERR_INVALID_ARG, "use_bias must be 0 or 1 for probes");
+    for (uint32_t i = 0; i < count; ++i)
+        if (T[i] > b->d.maxt) return fail(AXON_ERR_CAPACITY, "T=%u exceeds the batch's max_t=%u", T[i], b->d.maxt);
+    if (!count) return AXON_OK;
+    axon_net* n = b->net;
+    HIPCHECK(hipSetDevice(n->device));
+    const size_t NC = b->d.ncol;
+    std::vector<uint32_t> cl(count, clamp_steps), ub(count, use_bias);
+    HIPCHECK(hipMemcpyAsync(b->d.stim + first * NC, stims, count * NC * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemcpyAsync(b->d.st_T + first, T, count * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemcpyAsync(b->d.st_clamp + first, cl.data(), count * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(hipMemcpyAsync(b->d.st_bias + first, ub.data(), count * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHECK(axon::launch_probe_reset(b->d, first, count, n->stream));
+    // the host vectors above must outlive the copies
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_step(axon_probes* b, uint32_t steps) {
+    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
+    if (!steps) return AXON_OK;
+    axon_net* n = b->net;
+    HIPCHECK(hipSetDevice(n->device));
+    if (!n->use_graphs) {
+        for (uint32_t i = 0; i < steps; ++i) HIPCHECK(axon::launch_probe_step(b->d, b->tile, n->stream));
+        return AXON_OK;
+    }
+    auto it = b->graphs.find(steps);
+    if (it == b->graphs.end()) {
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (uint32_t i = 0; i < steps && e == hipSuccess; ++i) e = axon::launch_probe_step(b->d, b->tile, n->stream);
+        hipError_t e2 = hipStreamEndCapture(n->stream, &g);
+        if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "probe capture: %s", hipGetErrorString(e));
+        if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+        it = b->graphs.emplace(steps, ex).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second, n->stream));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_sync(axon_probes* b) {
+    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
+    HIPCHECK(hipStreamSynchronize(b->net->stream));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t count, uint32_t* spikes,
+                                        uint32_t* hist, uint32_t* steps_done) {
+    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
+    if (first + count > b->d.nslots || first + count < first) return fail(AXON_ERR_INVALID_ARG, "slot range");
+    axon_net* n = b->net;
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    const size_t NC = b->d.ncol;
+    if (spikes) HIPCHECK(hipMemcpy(spikes, b->d.spk + first * NC, count * NC * 4, hipMemcpyDeviceToHost));
+    if (hist)
+        HIPCHECK(hipMemcpy(hist, b->d.hist + (size_t)first * b->d.maxt * NC, (size_t)count * b->d.maxt * NC * 4,
+                           hipMemcpyDeviceToHost));
+    if (steps_done) HIPCHECK(hipMemcpy(steps_done, b->d.st_t + first, count * 4, hipMemcpyDeviceToHost));
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
+                                       uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes,
+                                       double* ms) {
+    if (!b || !stims) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (count > b->d.nslots) return fail(AXON_ERR_CAPACITY, "%u probes exceed %u slots", count, b->d.nslots);
+    std::vector<uint32_t> Ts(b->d.nslots, 0);
+    std::fill(Ts.begin(), Ts.begin() + count, T);
+    std::vector<uint32_t> zero((size_t)(b->d.nslots - count) * b->d.ncol, 0);
+    axon_status st = axon_probes_load(b, 0, count, stims, Ts.data(), clamp_steps, use_bias);
+    if (st == AXON_OK && count < b->d.nslots)  // idle slots: T = 0
+        st = axon_probes_load(b, count, b->d.nslots - count, zero.data(), Ts.data() + count, clamp_steps, use_bias);
+    if (st != AXON_OK) return st;
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((st = axon_probes_step(b, T)) != AXON_OK) return st;
+    if ((st = axon_probes_sync(b)) != AXON_OK) return st;
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (hist) {  // rows 0..T-1 of each probe, packed
+        axon_net* n = b->net;
+        const size_t NC = b->d.ncol;
+        for (uint32_t i = 0; i < count; ++i)
+            HIPCHECK(hipMemcpy(hist + (size_t)i * T * NC, b->d.hist + (size_t)i * b->d.maxt * NC, (size_t)T * NC * 4,
+                               hipMemcpyDeviceToHost));
+        (void)n;
+    }
+    if (spikes) return axon_probes_read(b, 0, count, spikes, nullptr, nullptr);
+    return AXON_OK;
+}
+
+// Recall cue with replacement (an extension of loadStim, mm:800-804): start
+// from pattern p's stimulus columns, replace `nreplace` of them by input
+// columns outside the pattern.  Both choices are partial Fisher-Yates draws
+// with the libc++ uniform index over mt19937(seed).  pattern = 0xFFFFFFFF
+// draws `stimcols` input columns uniformly instead.
+extern "C" axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint32_t nreplace, uint32_t seed,
+                                      uint32_t* stim) {
+    if (!n || !stim) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    const uint32_t IN = n->cfg.incols, SC = n->cfg.stimcols;
+    std::mt19937 rng(seed);
+    memset(stim, 0, (size_t)n->NCOL * 4);
+    auto pick = [&](std::vector<uint32_t>& v, uint32_t r) {  // first r entries become a uniform r-subset
+        for (uint32_t i = 0; i < r && i + 1 < v.size(); ++i) {
+            const uint64_t j = axon::libcxx_uniform_index(rng, (uint64_t)(v.size() - i));
+            std::swap(v[i], v[i + j]);
+        }
+    };
+    if (pattern == 0xFFFFFFFFu) {
+        std::vector<uint32_t> all(IN);
+        for (uint32_t i = 0; i < IN; ++i) all[i] = i;
+        pick(all, SC);
+        for (uint32_t i = 0; i < SC; ++i) stim[all[i]] = 0xFFFFFFFFu;
+        return AXON_OK;
+    }
+    if (pattern >= n->stimCols.size()) return fail(AXON_ERR_INVALID_ARG, "pattern %u out of range", pattern);
+    std::vector<uint32_t> mem = n->stimCols[pattern];
+    if (nreplace > mem.size() || nreplace > IN - mem.size())
+        return fail(AXON_ERR_INVALID_ARG, "cannot replace %u of %zu columns", nreplace, mem.size());
+    std::vector<uint8_t> isMem(IN, 0);
+    for (uint32_t c : mem) isMem[c] = 1;
+    std::vector<uint32_t> non;
+    for (uint32_t c = 0; c < IN; ++c)
+        if (!isMem[c]) non.push_back(c);
+    pick(mem, nreplace);
+    pick(non, nreplace);
+    for (uint32_t i = nreplace; i < mem.size(); ++i) stim[mem[i]] = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < nreplace; ++i) stim[non[i]] = 0xFFFFFFFFu;
     return AXON_OK;
 }

@@ -40,11 +40,39 @@ struct NetDev {
     uint32_t* driveLog;
     uint32_t* refBall;
     uint32_t* mult;
+    // Work counters (no reference counterpart): per column, the number of
+    // (column, branch) tiles propagation read, and per (column, wave) the
+    // number of tiles plasticity rewrote; accumulated across steps.
+    uint32_t* cntProp;   // ncol
+    uint32_t* cntLearn;  // ncol * 4
     uint32_t ncol, incols, inbr, topkpar;
     uint32_t nstats, ndlog, maxt, pad;
 };
 
+// Device pointers of a probe batch: `nslots` independent readout
+// presentations against one network (its W3, conn and bias).
+struct ProbeDev {
+    const uint4* W3;
+    const uint32_t* conn;
+    const uint32_t* bias;
+    uint32_t* spk;       // [slot][ncol]   spikes of the last completed step
+    uint32_t* gact;      // [slot][gw]     1 bit per presynaptic group with any spike
+    uint32_t* acc;       // [slot][ncol*32]
+    uint32_t* drive;     // [slot][ncol]
+    uint32_t* stim;      // [slot][ncol]
+    uint32_t* hist;      // [slot][maxt][ncol]
+    uint32_t* theta;     // [slot][2]
+    uint32_t* st_t;      // [slot] steps done
+    uint32_t* st_T;      // [slot] steps to run
+    uint32_t* st_clamp;  // [slot] clamped steps
+    uint32_t* st_bias;   // [slot] use the intrinsic bias (0/1)
+    uint32_t nslots, ncol, incols, maxt, gw;
+    uint32_t k, facil, dthr, boost, mode, biasShift, target, topkpar;
+};
+
 // ---- launchers (kernels.hip) ; all asynchronous on `s` ----
+hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, hipStream_t s);
+hipError_t launch_probe_reset(const ProbeDev& d, uint32_t first, uint32_t count, hipStream_t s);
 hipError_t launch_init(const NetDev& d, uint32_t nand, uint32_t seed, hipStream_t s);
 hipError_t launch_clear(const NetDev& d, hipStream_t s);
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s);

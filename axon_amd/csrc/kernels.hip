@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
     const uint4 sp = reinterpret_cast<const uint4*>(spkIn)[2u * gl + (hi ? 1u : 0u)];
     const uint32_t anyh = sp.x | sp.y | sp.z | sp.w;
     uint32_t mask = ballot32((anyh | __shfl_xor(anyh, 32)) != 0u);
+    if (lane == 0u) d.cntProp[col] += popc(mask);
 
     const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
     uint32_t acc = 0u, bb = 0u;
@@ -273,8 +274,10 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
 // workgroup per instance.  Histogram bins are accumulated with one LDS atomic
 // per distinct bin per wave; the 256-bin suffix scan and the crossing bin are
 // done by wave 0 in registers.
-__device__ void topk_block(const uint32_t* __restrict__ drive, uint32_t* __restrict__ theta, uint32_t ncol,
-                           uint32_t inCols, uint32_t target, uint32_t par) {
+// Returns max(theta, 1) to every thread of the block; thread 0 also stores
+// (theta, need) to `theta` when it is not null.
+__device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __restrict__ theta, uint32_t ncol,
+                               uint32_t inCols, uint32_t target, uint32_t par) {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t sh_need, sh_prefix;
     const uint32_t tid = threadIdx.x, lane = lane_id();
@@ -350,7 +353,9 @@ __device__ void topk_block(const uint32_t* __restrict__ drive, uint32_t* __restr
         }
         __syncthreads();
     }
-    if (tid == 0) { theta[0] = max(sh_prefix, 1u); theta[1] = sh_need; }
+    const uint32_t th = max(sh_prefix, 1u);
+    if (tid == 0 && theta) { theta[0] = th; theta[1] = sh_need; }
+    return th;
 }
 
 __global__ __launch_bounds__(256) void k_topk(NetDev d, Params P) {
@@ -438,6 +443,7 @@ __global__ __launch_bounds__(256) void k_emit(NetDev d, Params P, const uint32_t
         const uint32_t s = wave * 8u + i;
         if (ballot32(post && ((bb >> s) & 1u) != 0u)) voted |= 1u << i;
     }
+    if (lane == 0u) d.cntLearn[col * 4u + wave] += popc(voted);
     uint32_t nLTP = 0, nLTD = 0;
     const size_t tileBase = (size_t)col * NSLOT * TILE_U4 + lane;
     while (voted) {
@@ -634,6 +640,155 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
     if (lane_id() == 0u) { ball[col] = r.ball; vk[col] = r.vk; vk1[col] = r.vk1; }
 }
 
+// ============================================================== probe batches
+// Many recall probes against one resident network (readout presentations:
+// only W3, conn and optionally bias are read, nothing persistent is written).
+// Each probe slot keeps its own spikes, drives, accumulators and history.
+//
+// k_probe_drive<PT>: one wave per (column, probe tile of PT slots).  For each
+// branch, the W3 tile is loaded once into registers and reused by every probe
+// of the tile whose presynaptic group fired; per-probe accumulators live in
+// registers (PT of them), and the probe loop is unrolled so a probe whose
+// group is silent costs one scalar branch.  Group activity comes from a bit
+// per presynaptic group (gact) that k_probe_select writes with the spikes.
+//
+// Blocks are numbered so that the blocks one XCD receives (linear id L,
+// L+8, L+16, ...) walk all probe tiles of the same four columns back to back,
+// so the tile's W3 bytes are reused from that XCD's L2.
+template <int PT>
+__global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles) {
+    const uint32_t L = blockIdx.x;
+    const uint32_t q = L >> 3;
+    const uint32_t tile = q % ntiles;
+    const uint32_t cg = (q / ntiles) * 8u + (L & 7u);
+    if (cg >= ncolgroups) return;
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const bool hi = lane >= 32u;
+    const uint32_t col = cg * 4u + (threadIdx.x >> 6);
+    if (col >= d.ncol) return;
+    const uint32_t base = col * NSLOT;
+    const uint32_t slot0 = tile * PT;
+    const uint32_t gl = d.conn[base + n];
+
+    uint32_t bm[PT];
+    uint32_t anyb = 0, live = 0;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const uint32_t sl = slot0 + p;
+        uint32_t m = 0;
+        if (sl < d.nslots && d.st_t[sl] < d.st_T[sl]) {
+            live |= 1u << (p & 31);
+            const uint32_t w = d.gact[(size_t)sl * d.gw + (gl >> 5)];
+            m = ballot32(((w >> (gl & 31u)) & 1u) != 0u);
+        }
+        bm[p] = m;
+        anyb |= m;
+    }
+    if (PT <= 32 && live == 0u) return;
+
+    uint32_t acc[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) acc[p] = 0u;
+    const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
+    uint32_t mask = anyb;
+    uint32_t s = mask ? __builtin_ctz(mask) : 0u;
+    uint4 w = mask ? Wc[(size_t)s * TILE_U4] : make_uint4(0u, 0u, 0u, 0u);
+    while (mask) {
+        mask &= mask - 1u;
+        const uint32_t sn = mask ? __builtin_ctz(mask) : s;
+        const uint4 wn = Wc[(size_t)sn * TILE_U4];  // next tile in flight while this one is used
+        const uint32_t g = __builtin_amdgcn_readlane(gl, s);
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+            if ((bm[p] >> s) & 1u) {
+                const uint4* s4 = reinterpret_cast<const uint4*>(d.spk + (size_t)(slot0 + p) * d.ncol + 8u * g);
+                const uint4 lo4 = s4[0], hi4 = s4[1];
+                const uint4 sv = hi ? hi4 : lo4;
+                uint32_t c = popc(w.x & sv.x) + popc(w.y & sv.y) + popc(w.z & sv.z) + popc(w.w & sv.w);
+                c += __shfl_xor(c, 32);
+                const bool nmda = c >= d.dthr;
+                if (d.mode == 0u) acc[p] += c;
+                else if (d.mode == 1u) acc[p] += nmda ? (c + d.boost) : c;
+                else acc[p] += nmda ? 1u : 0u;
+            }
+        }
+        s = sn;
+        w = wn;
+    }
+    const uint32_t gid = base + n;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const uint32_t sl = slot0 + p;
+        if (sl >= d.nslots || d.st_t[sl] >= d.st_T[sl]) continue;
+        uint32_t a = acc[p];
+        if (d.facil != 0u && ((d.spk[(size_t)sl * d.ncol + col] >> n) & 1u) != 0u) a += d.facil;
+        if (d.st_bias[sl] != 0u) {
+            const uint32_t bi = d.bias[gid] >> d.biasShift;
+            a = (a > bi) ? (a - bi) : 0u;
+        }
+        if (!hi) d.acc[(size_t)sl * d.ncol * NPC + gid] = a;
+        const uint32_t drv = half_sum(a);
+        if (lane == 0u) d.drive[(size_t)sl * d.ncol + col] = drv;
+    }
+}
+
+// k_probe_select: one workgroup per slot.  Column top-K theta, then the gated
+// k-of-32 for columns that pass (the others are silent), the input clamp, the
+// spike and history writes and the slot's group-activity bits.  The margin,
+// stats and drive-log side outputs of k_emit are not produced for probes.
+__global__ __launch_bounds__(256) void k_probe_select(ProbeDev d) {
+    __shared__ uint32_t sg[4096];
+    const uint32_t sl = blockIdx.x;
+    if (sl >= d.nslots) return;
+    const uint32_t t = d.st_t[sl];
+    if (t >= d.st_T[sl]) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
+    const size_t so = (size_t)sl * d.ncol;
+    const uint32_t th = topk_block(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
+    for (uint32_t i = tid; i < d.gw; i += 256u) sg[i] = 0u;
+    __syncthreads();
+    const bool clampPhase = t < d.st_clamp[sl];
+    const uint32_t* __restrict__ acc = d.acc + so * NPC;
+    uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
+    for (uint32_t c0 = wave * 64u; c0 < d.ncol; c0 += 256u) {
+        const uint32_t c = c0 + lane;
+        const bool inb = c < d.ncol;
+        const uint32_t dv = inb ? d.drive[so + c] : 0u;
+        const bool clampCol = inb && clampPhase && c < d.incols;
+        uint32_t b = clampCol ? d.stim[so + c] : 0u;
+        uint64_t m = __ballot(inb && !clampCol && dv >= th);
+        while (m) {
+            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            m &= m - 1ull;
+            const uint32_t v = acc[(size_t)(c0 + i) * NPC + (lane & 31u)];
+            const Kwta kw = kwta32(v, d.k, lane & 31u);
+            if (lane == i) b = kw.ball;
+        }
+        if (inb) { d.spk[so + c] = b; hist[c] = b; }
+        const uint64_t nz = __ballot(inb && b != 0u);
+        if (lane == 0u && nz) {
+            uint32_t bits = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if ((nz >> (8 * j)) & 0xFFull) bits |= 1u << j;
+            const uint32_t g0 = c0 >> 3;
+            atomicOr(&sg[g0 >> 5], bits << (g0 & 31u));
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < d.gw; i += 256u) d.gact[(size_t)sl * d.gw + i] = sg[i];
+    if (tid == 0u) d.st_t[sl] = t + 1u;
+}
+
+// Start of a probe: zero spikes and group bits, step counter to 0.
+__global__ __launch_bounds__(256) void k_probe_reset(ProbeDev d, uint32_t first, uint32_t count) {
+    const uint32_t sl = first + blockIdx.x;
+    if (blockIdx.x >= count || sl >= d.nslots) return;
+    for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) d.spk[(size_t)sl * d.ncol + i] = 0u;
+    for (uint32_t i = threadIdx.x; i < d.gw; i += 256u) d.gact[(size_t)sl * d.gw + i] = 0u;
+    if (threadIdx.x == 0u) d.st_t[sl] = 0u;
+}
+
 inline uint32_t blocks_for(size_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 }  // namespace
@@ -723,6 +878,33 @@ hipError_t launch_kwta_test(const uint32_t* v, uint32_t ncols, uint32_t k, uint3
 hipError_t launch_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol, uint32_t incols, uint32_t target,
                             uint32_t par, uint32_t* theta, hipStream_t s) {
     k_topk_test<<<ninst, 256, 0, s>>>(drive, theta, ncol, incols, target, par);
+    return hipGetLastError();
+}
+
+}  // namespace axon
+
+namespace axon {
+
+hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, hipStream_t s) {
+    const uint32_t ncg = (d.ncol + 3u) / 4u;
+    const uint32_t ntiles = (d.nslots + tile - 1u) / tile;
+    const uint32_t blocks = ((ncg + 7u) / 8u) * 8u * ntiles;
+    switch (tile) {
+        case 8: k_probe_drive<8><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 16: k_probe_drive<16><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 32: k_probe_drive<32><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 64: k_probe_drive<64><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_probe_select<<<d.nslots, 256, 0, s>>>(d);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_reset(const ProbeDev& d, uint32_t first, uint32_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    k_probe_reset<<<count, 256, 0, s>>>(d, first, count);
     return hipGetLastError();
 }
 

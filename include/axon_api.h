@@ -120,6 +120,19 @@ axon_status axon_net_set_topk_parallel(axon_net* n, uint32_t par);
 /* stimCols[p] of mm:798 (stimcols entries) */
 axon_status axon_stim_cols(const axon_net* n, uint32_t p, uint32_t* out);
 
+/* ---- measurement (no reference counterpart; the reference timed whole
+ * presentations with steady_clock, mm:834/886, and per-kernel Metal
+ * timestamps under profile=1, mm:1162-1340) ---- */
+/* out[0]: (column, branch) tiles read by propagation, out[1]: tiles rewritten
+ * by plasticity, accumulated since creation or the last reset.  Each tile is
+ * 8192 synapses (32 neurons x 256 presynaptic bits). */
+axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset);
+/* enable = 1: presentations run eagerly with HIP events around every kernel;
+ * axon_kernel_times returns summed device ms and launch counts for
+ * [0] propagation, [1] column top-K, [2] emit (+bias +plasticity). */
+axon_status axon_net_set_profiling(axon_net* n, uint32_t enable);
+axon_status axon_kernel_times(axon_net* n, double ms[3], uint64_t launches[3], uint32_t reset);
+
 /* ---- readout kernels ---- */
 axon_status axon_wstat(axon_net* n, uint32_t* functional);  /* k_wstat mm:641-646, wstat() mm:784-791 */
 /* k_within mm:524-562: post_mask NULL means useMask = 0 (the deep call, mm:1702-1709) */
@@ -129,6 +142,36 @@ axon_status axon_untouched(axon_net* n, uint32_t* out /* N */); /* k_untouched m
 /* k_wdiag mm:591-638: memb N words, asm_spk nstim*NCOL words, hist 48 words */
 axon_status axon_wdiag(axon_net* n, const uint32_t* memb, const uint32_t* asm_spk, uint32_t nstim,
                        uint32_t sel, uint32_t hist[48]);
+
+/* ---- probe batches: many readout presentations against one network ----
+ * Batching extension with no reference counterpart; each probe is exactly the
+ * reference's readout present(q, keep, T, clampSteps, learn=false, ...)
+ * (mm:815-888 with learn = 0) on its own clamp pattern, and its spike
+ * ballots are bit-identical to that single presentation.  Probes are packed in
+ * tiles of `tile` (8/16/32/64) slots; one W3 tile read serves a whole probe
+ * tile.  Slots advance together one timestep per step; a slot stops at its own
+ * T, so probes of different lengths share launches.  The margin, stats and
+ * drive-log side outputs of a presentation are not produced for probes. */
+typedef struct axon_probes axon_probes;
+axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t max_t, uint32_t tile, axon_probes** out);
+void axon_probes_destroy(axon_probes* b);
+axon_status axon_probes_info(const axon_probes* b, uint32_t* slots, uint32_t* max_t, uint32_t* tile);
+/* stims: count*NCOL clamp patterns; T: count step counts (<= max_t); use_bias 0/1 */
+axon_status axon_probes_load(axon_probes* b, uint32_t first, uint32_t count, const uint32_t* stims,
+                             const uint32_t* T, uint32_t clamp_steps, uint32_t use_bias);
+axon_status axon_probes_step(axon_probes* b, uint32_t steps); /* asynchronous */
+axon_status axon_probes_sync(axon_probes* b);
+/* spikes: count*NCOL final ballots; hist: count*max_t*NCOL; steps_done: count */
+axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t count, uint32_t* spikes, uint32_t* hist,
+                             uint32_t* steps_done);
+/* load `count` probes with the same T into slots 0.., run them to completion;
+ * hist is count*T*NCOL (packed), ms the device wall time of the steps */
+axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
+                            uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes, double* ms);
+/* recall cue: pattern's stimulus columns with `nreplace` of them replaced by
+ * other input columns (mt19937(seed) draws); pattern 0xFFFFFFFF = a random cue
+ * of stimcols input columns.  Extends loadStim (mm:800-804). */
+axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint32_t nreplace, uint32_t seed, uint32_t* stim);
 
 /* ---- kernel test harnesses (replace the vbs/vtopk self-checks mm:903-994) ---- */
 /* k_bstest mm:424-434: op 0 inc, 1 dec, 2 clip[lo,hi]; nwords u32 per plane, in place */

@@ -86,6 +86,7 @@ def lib() -> ctypes.CDLL:
         L.ora_libcxx_shuffle.argtypes = [U32P, ctypes.c_uint32, ctypes.c_uint32]
         L.ora_uniform01.restype = ctypes.c_double
         L.ora_uniform01.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.ora_work_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
         L.ora_run_protocol.argtypes = [ctypes.POINTER(OraConfig), ctypes.c_int, ctypes.POINTER(OraPinned),
                                        ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
@@ -148,6 +149,11 @@ class OracleNet:
         out = np.zeros(self.cfg.stimcols, np.uint32)
         lib().ora_stim_cols(self.h, p, _p(out))
         return out
+
+    def work_counters(self, reset: bool = False):
+        out = (ctypes.c_uint64 * 2)()
+        lib().ora_work_counters(self.h, out, int(reset))
+        return int(out[0]), int(out[1])
 
     def within(self, ball, post_mask=None) -> np.ndarray:
         out = np.zeros(16, np.uint32)

@@ -200,6 +200,7 @@ struct ora_net {
     std::vector<uint32_t> W[4];
     std::vector<uint32_t> conn, spkA, spkB, drv, bbits, acc, bias, marg, refBall, driveLog,
         mult, stim, stat, hist, theta;
+    std::vector<uint64_t> cntProp, cntLearn;  // per column: tiles read / tiles rewritten
     std::vector<std::vector<uint32_t>> stimCols;
     Params P;
     uint32_t K, kTest, facil, contrast, ltdEvery, topkPar;
@@ -312,6 +313,7 @@ extern "C" ora_net* ora_net_create(const ora_config* c, int nthreads) {
     n->marg.assign(n->NCOL, 0u); n->refBall.assign(n->NCOL, 0u); n->driveLog.assign(n->NDLOG, 0u);
     n->mult.assign(n->N, 0u); n->stim.assign(n->NCOL, 0u); n->stat.assign(n->NSTATS, 0u);
     n->hist.assign((size_t)n->MAXT * n->NCOL, 0u); n->theta.assign(16, 0u);
+    n->cntProp.assign(n->NCOL, 0); n->cntLearn.assign(n->NCOL, 0);
 
     const uint32_t seed = c->seed, nAnd = c->nand;
     uint32_t* W0 = n->W[0].data(); uint32_t* W1 = n->W[1].data();
@@ -376,6 +378,7 @@ static void k_drive(ora_net* n, const uint32_t* spkIn) {
                 const uint32_t* sp = spkIn + (size_t)g * 8;
                 uint32_t any = sp[0] | sp[1] | sp[2] | sp[3] | sp[4] | sp[5] | sp[6] | sp[7];
                 if (any == 0u) continue;
+                n->cntProp[col]++;
                 for (uint32_t lane = 0; lane < 32; ++lane) {
                     const size_t wb = ((size_t)(base + s) * SLOT_U4 * 32u + lane) * 4;
                     const uint32_t* w0 = W3 + wb;           // uint4 at wb
@@ -485,6 +488,7 @@ static void k_learn(ora_net* n, const uint32_t* spkIn, const uint32_t* spkOut) {
                     if (want) vote |= 1u << l;
                 }
                 if (vote == 0u) continue;
+                n->cntLearn[col]++;
                 const uint32_t g = n->conn[base + s];
                 for (uint32_t l = 0; l < 32; ++l) {
                     const bool want = ((vote >> l) & 1u) != 0u;
@@ -754,4 +758,14 @@ extern "C" double ora_uniform01(uint32_t seed, uint32_t idx) {
     double u = 0;
     for (uint32_t i = 0; i <= idx; ++i) u = uniform01(g);
     return u;
+}
+
+extern "C" void ora_work_counters(ora_net* n, uint64_t out[2], int reset) {
+    out[0] = 0; out[1] = 0;
+    for (uint64_t x : n->cntProp) out[0] += x;
+    for (uint64_t x : n->cntLearn) out[1] += x;
+    if (reset) {
+        std::fill(n->cntProp.begin(), n->cntProp.end(), 0);
+        std::fill(n->cntLearn.begin(), n->cntLearn.end(), 0);
+    }
 }

@@ -70,6 +70,10 @@ uint32_t* ora_buffer(ora_net* n, int which, size_t* nwords);
 const uint32_t* ora_final_spikes(ora_net* n, uint32_t T); /* (T&1)?B:A, mm:891 */
 void ora_stim_cols(ora_net* n, uint32_t p, uint32_t* out); /* stimcols entries */
 
+/* Work counters (no reference counterpart): tiles read by propagation and
+ * tiles rewritten by plasticity since creation / the last reset. */
+void ora_work_counters(ora_net* n, uint64_t out[2], int reset);
+
 /* Readout kernels (mm:501-646).  post_mask == NULL means useMask = 0. */
 void ora_within(ora_net* n, const uint32_t* ball, const uint32_t* post_mask, uint32_t out[16]);
 uint32_t ora_wstat(ora_net* n);

@@ -36,7 +36,7 @@ def cap10_fields(out):
     s, o, m = row[2], row[3], row[4]
     j = [k for k, l in enumerate(lines) if "WAVE:" in l][0]
     free = []
-    for l in lines[j:]:
+    for l in lines[j + 1:]:  # sed -n '/WAVE:/,/^=====/p': the range end is looked for after the start line
         if l.startswith("====="):
             break
         p = l.split()

@@ -63,6 +63,7 @@ def test_protocol_parity(variant):
     ora.present(0, 1.0, T, tc, True, 31337)
     compare_present(g, ora, T, f"{variant} extra")
     compare_state(net, ora, f"{variant} extra")
+    assert net.work_counters(reset=True) == ora.work_counters(reset=True), "work counters"
     print(f"parity {variant}: readouts", flush=True)
     refs = []
     for p in range(nstim):

@@ -1,0 +1,127 @@
+"""Probe batches (many recall presentations against one network in shared
+launches) must give, for every probe, exactly the spike history of that probe
+presented alone: the single-network engine and the CPU oracle are the checks.
+Covers every probe tile size, probes of different lengths sharing launches,
+slots refilled while other probes are mid-flight, bias on/off and the readout
+knobs ktest/facil."""
+import numpy as np
+import pytest
+
+import axon_amd
+import oracle
+from conftest import C0, CONFIG_C
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ["ncol=512", "incols=128"]
+
+
+def trained(args, epochs):
+    net = axon_amd.Net(axon_amd.Config.parse(args))
+    ora = oracle.OracleNet(oracle.parse(args), threads=4)
+    cfg = net.config
+    for ep in range(epochs):
+        for p in range(cfg.stim):
+            net.present(p, 1.0, cfg.T, cfg.trainclamp, True, False, 1000 + p, outputs=False)
+            ora.present(p, 1.0, cfg.T, cfg.trainclamp, True, 1000 + p)
+    net.sync()
+    return net, ora
+
+
+def cues(net, n):
+    cfg = net.config
+    out = []
+    for i in range(n):
+        if i % 5 == 4:
+            out.append(axon_amd.probe_cue(net, -1, 0, 900 + i))
+        else:
+            r = [0, 6, 16, 32][i % 4]
+            out.append(axon_amd.probe_cue(net, i % cfg.stim, r, 900 + i))
+    return np.stack(out)
+
+
+def test_cue_generator():
+    net = axon_amd.Net(axon_amd.Config.parse(C0))
+    for p in range(12):
+        members = set(net.stim_cols(p).tolist())
+        for r in (0, 6, 16, 32, 64):
+            c = axon_amd.probe_cue(net, p, r, 77 + r)
+            on = set(np.flatnonzero(c).tolist())
+            assert all(c[x] == 0xFFFFFFFF for x in on)
+            assert len(on) == 64 and len(on & members) == 64 - r and max(on) < 128
+        a = axon_amd.probe_cue(net, p, 16, 5)
+        assert np.array_equal(a, axon_amd.probe_cue(net, p, 16, 5))  # deterministic in the seed
+    rnd = axon_amd.probe_cue(net, -1, 0, 3)
+    assert np.count_nonzero(rnd) == 64 and np.flatnonzero(rnd).max() < 128
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32, 64])
+def test_probe_batch_matches_single_presentations(tile):
+    net, ora = trained(C0, 3)
+    stims = cues(net, 40)
+    T, clamp = 35, 5
+    pb = axon_amd.Probes(net, slots=40, max_t=64, tile=tile)
+    print(f"probe tile {tile}: {pb.slots} slots", flush=True)
+    spk, hist, ms = pb.run(stims, T, clamp, use_bias=0)
+    for i in range(stims.shape[0]):
+        r = net.present(0, 1.0, T, clamp, False, True, 0, 0, stim=stims[i])
+        assert np.array_equal(hist[i], r.hist), f"tile {tile} probe {i}: history"
+        assert np.array_equal(spk[i], r.spikes), f"tile {tile} probe {i}: final"
+        if i % 7 == 0:
+            ora.present(0, 1.0, T, clamp, False, 0, 0, stim=stims[i])
+            assert np.array_equal(hist[i].ravel(), ora.buf("HIST")[: T * 512]), f"probe {i}: oracle"
+
+
+def test_probes_of_different_lengths_and_refilled_slots():
+    net, ora = trained(C0, 2)
+    stims = cues(net, 48)
+    rng = np.random.default_rng(11)
+    Ts = rng.integers(1, 64, 48).astype(np.uint32)
+    pb = axon_amd.Probes(net, slots=16, max_t=64, tile=8)
+    results = {}
+    pending = list(range(48))
+    inflight = {}
+    # first fill
+    for s in range(16):
+        i = pending.pop(0)
+        pb.load(s, stims[i][None], [Ts[i]], 5, use_bias=1)
+        inflight[s] = i
+    while inflight:
+        pb.step(7)
+        spk, hist, done = pb.read(0, 16)
+        for s, i in list(inflight.items()):
+            if done[s] >= Ts[i]:
+                results[i] = (spk[s].copy(), hist[s, : Ts[i]].copy())
+                del inflight[s]
+                if pending:  # refill this slot while the others are mid-flight
+                    j = pending.pop(0)
+                    pb.load(s, stims[j][None], [Ts[j]], 5, use_bias=1)
+                    inflight[s] = j
+    assert len(results) == 48
+    for i, (spk, hist) in results.items():
+        r = net.present(0, 1.0, int(Ts[i]), 5, False, True, 0, 1, stim=stims[i])
+        assert np.array_equal(hist, r.hist), f"probe {i} T={Ts[i]}"
+        assert np.array_equal(spk, r.spikes)
+
+
+def test_probe_readout_knobs():
+    args = SMALL + CONFIG_C + ["ktest=10", "facil=40", "stim=6", "seed=2"]
+    net, ora = trained(args, 2)
+    stims = np.stack([axon_amd.probe_cue(net, p % 6, 8 * (p % 3), p) for p in range(24)])
+    pb = axon_amd.Probes(net, slots=24, max_t=40, tile=16)
+    spk, hist, _ = pb.run(stims, 30, 5, use_bias=0)
+    for i in range(24):
+        ora.present(0, 1.0, 30, 5, False, 0, 0, stim=stims[i])
+        assert np.array_equal(hist[i].ravel(), ora.buf("HIST")[: 30 * 512]), f"probe {i}"
+
+
+def test_probe_errors():
+    net = axon_amd.Net(axon_amd.Config.parse(C0))
+    with pytest.raises(axon_amd.AxonError):
+        axon_amd.Probes(net, slots=8, max_t=16, tile=12)
+    pb = axon_amd.Probes(net, slots=8, max_t=16, tile=8)
+    with pytest.raises(axon_amd.AxonError) as e:
+        pb.load(0, np.zeros((1, 512), np.uint32), [17], 5)
+    assert e.value.status == 6
+    with pytest.raises(axon_amd.AxonError):
+        pb.load(7, np.zeros((2, 512), np.uint32), [5, 5], 5)
