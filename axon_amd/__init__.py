@@ -336,6 +336,145 @@ class Probes:
         return spk, h, ms.value
 
 
+class MultiNet:
+    """Independent networks trained in shared launches (per-network work queues)."""
+
+    def __init__(self, nets: Sequence[Net]):
+        self.nets = list(nets)
+        arr = (ctypes.c_void_p * len(self.nets))(*[n._h.value for n in self.nets])
+        self._h = ctypes.c_void_p()
+        _check(lib().axon_multi_create(arr, ctypes.c_uint32(len(self.nets)), ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().axon_multi_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train(self, epochs: Sequence[int], extra: bool = True) -> float:
+        ep = np.ascontiguousarray(epochs, np.uint32)
+        ms = ctypes.c_double()
+        _check(lib().axon_multi_train(self._h, _ptr(ep), ctypes.c_uint32(int(extra)), ctypes.byref(ms)))
+        return ms.value
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots_per_net", ctypes.c_uint32), ("max_t", ctypes.c_uint32), ("tile", ctypes.c_uint32),
+                ("steps_per_tick", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Server:
+    """Continuous-batching recall server over resident networks."""
+
+    def __init__(self, nets: Sequence[Net], slots_per_net: int = 64, max_t: int = 200, tile: int = 32,
+                 steps_per_tick: int = 5, hbm_budget: int = 0):
+        self.nets = list(nets)
+        self.ncol = self.nets[0].ncol
+        arr = (ctypes.c_void_p * len(self.nets))(*[n._h.value for n in self.nets])
+        cfg = ServerConfig(slots_per_net, max_t, tile, steps_per_tick, hbm_budget)
+        self._h = ctypes.c_void_p()
+        L = lib()
+        L.axon_server_clock.restype = ctypes.c_double
+        L.axon_server_clock.argtypes = [ctypes.c_void_p]
+        _check(L.axon_server_create(arr, ctypes.c_uint32(len(self.nets)), ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().axon_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self) -> float:
+        return lib().axon_server_clock(self._h)
+
+    def submit(self, net: int, stim: np.ndarray, T: int, clamp_steps: int = 5, use_bias: int = 0,
+               arrival: float = -1.0) -> int:
+        stim = np.ascontiguousarray(stim, np.uint32)
+        rid = ctypes.c_uint64()
+        _check(lib().axon_server_submit(self._h, ctypes.c_uint32(net), _ptr(stim), ctypes.c_uint32(T),
+                                        ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias),
+                                        ctypes.c_double(arrival), ctypes.byref(rid)))
+        return rid.value
+
+    def tick(self) -> int:
+        n = ctypes.c_uint32()
+        _check(lib().axon_server_tick(self._h, ctypes.byref(n)))
+        return n.value
+
+    def completed(self) -> list:
+        """Ids of the queries the last tick finished."""
+        n = ctypes.c_uint32()
+        _check(lib().axon_server_completed(self._h, None, 0, ctypes.byref(n)))
+        ids = np.zeros(max(n.value, 1), np.uint64)
+        _check(lib().axon_server_completed(self._h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                           ctypes.c_uint32(n.value), ctypes.byref(n)))
+        return [int(x) for x in ids[: n.value]]
+
+    def result(self, rid: int, max_t: int):
+        """(spikes, row digests, latency seconds, T) or None while pending."""
+        spk = np.zeros(self.ncol, np.uint32)
+        dig = np.zeros(max_t, np.uint64)
+        lat, T = ctypes.c_double(), ctypes.c_uint32()
+        st = lib().axon_server_result(self._h, ctypes.c_uint64(rid), _ptr(spk),
+                                      dig.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(lat),
+                                      ctypes.byref(T))
+        if st == 7:  # AXON_ERR_NOT_READY
+            return None
+        _check(st)
+        return spk, dig[: T.value], lat.value, T.value
+
+    def status(self):
+        q, f, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().axon_server_status(self._h, ctypes.byref(q), ctypes.byref(f), ctypes.byref(r)))
+        return q.value, f.value, r.value
+
+
+def _lowbias32(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def row_digest(rows: np.ndarray) -> np.ndarray:
+    """The engine's order-independent 64-bit checksum of each row of u32 words."""
+    rows = np.atleast_2d(np.asarray(rows, np.uint32)).astype(np.uint64)
+    i = np.arange(rows.shape[1], dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        v = rows * (2 * i + 1) + _lowbias32(rows ^ i)
+        return v.sum(axis=1, dtype=np.uint64)
+
+
+def checksum(net: Net, which: str) -> int:
+    v = ctypes.c_uint64()
+    _check(lib().axon_checksum(net._h, BUFFERS.index(which), ctypes.byref(v)))
+    return v.value
+
+
+def probe_autotune(net: Net, stims: np.ndarray, T: int, clamp_steps: int, use_bias: int = 0):
+    """Fastest probe tile for this batch: (tile, {tile: ms})."""
+    stims = np.ascontiguousarray(stims, np.uint32).reshape(-1, net.ncol)
+    best = ctypes.c_uint32()
+    ms = (ctypes.c_double * 4)()
+    _check(lib().axon_probe_autotune(net._h, ctypes.c_uint32(stims.shape[0]), _ptr(stims), ctypes.c_uint32(T),
+                                     ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias), ctypes.byref(best), ms))
+    return best.value, {t: ms[i] for i, t in enumerate((8, 16, 32, 64))}
+
+
 def probe_cue(net: Net, pattern: int, nreplace: int, seed: int) -> np.ndarray:
     """Recall cue: pattern's stimulus columns with nreplace of them swapped for
     other input columns; pattern = -1 draws a random cue."""

@@ -89,9 +89,10 @@ uint32_t argu(int argc, const char* const* argv, const char* key, uint32_t dflt)
 }
 
 struct GraphKey {
-    uint32_t T, clamp, learn, k, facil, useBias;
+    uint32_t T, clamp, learn, k, facil, useBias, stats;
     bool operator<(const GraphKey& o) const {
-        return std::tie(T, clamp, learn, k, facil, useBias) < std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias);
+        return std::tie(T, clamp, learn, k, facil, useBias, stats) <
+               std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias, o.stats);
     }
 };
 
@@ -281,6 +282,7 @@ extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
         {&d.stim, n->NCOL}, {&d.stats, n->NSTATS}, {&d.hist, (size_t)n->MAXT * n->NCOL}, {&d.driveLog, n->NDLOG},
         {&d.refBall, n->NCOL}, {&d.mult, n->N}, {&n->d_small, 64}, {&n->d_ball, n->NCOL}, {&n->d_mask, n->N},
         {&d.cntProp, n->NCOL}, {&d.cntLearn, (size_t)n->NCOL * 4},
+        {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4}, {&d.learnList, (size_t)n->NCOL * 2}, {&d.learnCnt, 2},
     };
     for (auto& b : bufs) {
         if ((st = dev_alloc(n, &p, b.words * 4)) != AXON_OK) return cleanup(st);
@@ -325,7 +327,8 @@ extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
 }
 
 // ================================================================== present
-static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn, hipError_t* err) {
+static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn, bool stats,
+                          hipError_t* err) {
     hipError_t e = axon::launch_clear(n->d, n->stream);
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {  // mm:837-884
         P.step = t;
@@ -338,7 +341,8 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
                   : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
         e = axon::launch_drive(n->d, P, t, n->stream);
         if (e == hipSuccess) e = axon::launch_topk(n->d, P, n->stream);
-        if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, n->stream);
+        if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream);
+        if (e == hipSuccess && learn && P.phase != 3u) e = axon::launch_learn(n->d, P, t, stats, n->stream);
     }
     *err = e;
 }
@@ -371,7 +375,8 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         if ((e = axon::launch_topk(n->d, P, n->stream)) != hipSuccess) break;
         if ((e = hipEventRecord(ev[3], n->stream)) != hipSuccess) break;
         if ((e = hipEventRecord(ev[4], n->stream)) != hipSuccess) break;
-        if ((e = axon::launch_emit(n->d, P, t, learn, n->stream)) != hipSuccess) break;
+        if ((e = axon::launch_emit(n->d, P, t, learn, true, n->stream)) != hipSuccess) break;
+        if (learn && P.phase != 3u && (e = axon::launch_learn(n->d, P, t, true, n->stream)) != hipSuccess) break;
         e = hipEventRecord(ev[5], n->stream);
     }
     if (e != hipSuccess) return e;
@@ -427,13 +432,13 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "profiled launch: %s", hipGetErrorString(e));
     } else if (n->use_graphs && a->T > 0) {
-        const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias};
+        const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias, async ? 0u : 1u};
         auto it = n->graphs.find(key);
         if (it == n->graphs.end()) {
             hipGraph_t g = nullptr;
             HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = hipSuccess;
-            enqueue_steps(n, P, a->T, a->clamp_steps, learn, &e);
+            enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, &e);
             hipError_t e2 = hipStreamEndCapture(n->stream, &g);
             if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "capture launch: %s", hipGetErrorString(e));
             if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -446,7 +451,7 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         HIPCHECK(hipGraphLaunch(it->second, n->stream));
     } else {
         hipError_t e = hipSuccess;
-        enqueue_steps(n, P, a->T, a->clamp_steps, learn, &e);
+        enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, &e);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
     }
     if (async) {
@@ -931,5 +936,485 @@ extern "C" axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint3
     pick(non, nreplace);
     for (uint32_t i = nreplace; i < mem.size(); ++i) stim[mem[i]] = 0xFFFFFFFFu;
     for (uint32_t i = 0; i < nreplace; ++i) stim[non[i]] = 0xFFFFFFFFu;
+    return AXON_OK;
+}
+
+// ================================================================== many networks
+struct axon_multi {
+    std::vector<axon_net*> nets;
+    hipStream_t stream = nullptr;
+    axon::MultiDev m{};
+    std::vector<void*> allocs;
+    uint32_t ncol = 0, nstats = 0;
+    Params P{};
+    uint32_t T = 0, trainClamp = 0, contrast = 0, ltdEvery = 1;
+    hipGraphExec_t graph = nullptr;
+    uint32_t* d_sched = nullptr;
+    size_t sched_rounds = 0;
+};
+
+extern "C" void axon_multi_destroy(axon_multi* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->d_sched) (void)hipFree(b->d_sched);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// Networks must share every dynamics parameter; seeds, stored-pattern counts
+// and epochs may differ.
+extern "C" axon_status axon_multi_create(axon_net* const* nets, uint32_t nnet, axon_multi** out) {
+    if (!nets || !out || nnet == 0) return fail(AXON_ERR_INVALID_ARG, "null argument or no networks");
+    *out = nullptr;
+    const axon_config& c0 = nets[0]->cfg;
+    for (uint32_t i = 0; i < nnet; ++i) {
+        if (!nets[i]) return fail(AXON_ERR_INVALID_ARG, "network %u is null", i);
+        const axon_config& c = nets[i]->cfg;
+        axon_config a = c, b = c0;
+        a.seed = b.seed = 0; a.stim = b.stim = 0; a.epochs = b.epochs = 0;
+        if (memcmp(&a, &b, sizeof a) != 0 || nets[i]->d.topkpar != nets[0]->d.topkpar)
+            return fail(AXON_ERR_INVALID_ARG, "network %u differs from network 0 in a dynamics parameter", i);
+        for (uint32_t j = 0; j < i; ++j)
+            if (nets[j] == nets[i]) return fail(AXON_ERR_INVALID_ARG, "network %u appears twice", i);
+    }
+    HIPCHECK(hipSetDevice(nets[0]->device));
+    axon_multi* b = new axon_multi();
+    b->nets.assign(nets, nets + nnet);
+    b->ncol = nets[0]->NCOL;
+    b->nstats = nets[0]->NSTATS;
+    b->P = nets[0]->P;
+    b->T = c0.T;
+    b->trainClamp = c0.trainclamp;
+    b->contrast = c0.contrast;
+    b->ltdEvery = nets[0]->ltdEvery;
+    auto bail = [&](axon_status s) { axon_multi_destroy(b); return s; };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(AXON_ERR_DEVICE, "hipStreamCreate failed"));
+    auto alloc = [&](void** p, size_t bytes) -> axon_status {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) return fail(AXON_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        b->allocs.push_back(*p);
+        return AXON_OK;
+    };
+    void* p = nullptr;
+    axon_status st;
+    if ((st = alloc(&p, sizeof(NetDev) * nnet)) != AXON_OK) return bail(st);
+    b->m.nets = (const NetDev*)p;
+    if ((st = alloc(&p, 4)) != AXON_OK) return bail(st);
+    b->m.round = (uint32_t*)p;
+    if ((st = alloc(&p, 8 * nnet)) != AXON_OK) return bail(st);
+    b->m.stimTabs = (const uint64_t*)p;
+    b->m.nnet = nnet;
+    std::vector<NetDev> devs(nnet);
+    std::vector<uint64_t> tabs(nnet);
+    for (uint32_t i = 0; i < nnet; ++i) {
+        axon_net* n = nets[i];
+        if (hipStreamSynchronize(n->stream) != hipSuccess) return bail(fail(AXON_ERR_DEVICE, "net stream"));
+        devs[i] = n->d;
+        // clamp patterns at keep = 1.0: every stimulus column of the pattern (loadStim, mm:800-804)
+        std::vector<uint32_t> tab((size_t)n->stimCols.size() * b->ncol, 0u);
+        for (size_t q = 0; q < n->stimCols.size(); ++q)
+            for (uint32_t col : n->stimCols[q]) tab[q * b->ncol + col] = 0xFFFFFFFFu;
+        if ((st = alloc(&p, tab.size() * 4)) != AXON_OK) return bail(st);
+        if (hipMemcpy(p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(AXON_ERR_DEVICE, "stim table copy"));
+        tabs[i] = (uint64_t)(uintptr_t)p;
+    }
+    if (hipMemcpy((void*)b->m.nets, devs.data(), sizeof(NetDev) * nnet, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void*)b->m.stimTabs, tabs.data(), 8 * nnet, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(AXON_ERR_DEVICE, "descriptor copy"));
+    *out = b;
+    return AXON_OK;
+}
+
+// One presentation of every scheduled network (mm:837-884 with learn = 1),
+// followed by the round counter advance.
+static hipError_t multi_enqueue_round(axon_multi* b) {
+    const axon::MultiDev& m = b->m;
+    hipError_t e = axon::launch_multi_begin(m, b->ncol, b->nstats, b->stream);
+    Params P = b->P;
+    P.k = b->P.k;
+    P.facil = 0u;
+    P.useBias = 1u;
+    const uint32_t T = b->T, clampSteps = b->trainClamp;
+    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
+        P.step = t;
+        P.learn = 1u;
+        P.freeze = 0u;
+        P.clamp = (t < clampSteps) ? 1u : 0u;
+        P.depress = (t % 2u == 1u) ? 1u : 0u;
+        P.phase = !b->contrast ? 0u : (t < clampSteps) ? 1u : (((t - clampSteps) % b->ltdEvery == 0u) ? 2u : 3u);
+        e = axon::launch_multi_drive(m, P, t, b->ncol, b->stream);
+        if (e == hipSuccess) e = axon::launch_multi_topk(m, P, b->stream);
+        if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, b->stream);
+        if (e == hipSuccess && P.phase != 3u) e = axon::launch_multi_learn(m, P, t, b->ncol, b->stream);
+    }
+    if (e == hipSuccess) e = axon::launch_multi_end(m, b->stream);
+    return e;
+}
+
+// Trains every network with the reference protocol: epochs[i] x stim_i
+// presentations (pattern k mod stim_i, keep 1.0; mm:1009-1010), then, when
+// extra = 1, the additional presentation of pattern 0 (mm:1033).  Network i's
+// work queue holds its own presentations; all networks advance one
+// presentation per round and drop out when their queue is empty.
+extern "C" axon_status axon_multi_train(axon_multi* b, const uint32_t* epochs, uint32_t extra, double* ms) {
+    if (!b || !epochs) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    const uint32_t nnet = b->m.nnet;
+    std::vector<uint32_t> len(nnet);
+    size_t rounds = 0;
+    for (uint32_t i = 0; i < nnet; ++i) {
+        len[i] = epochs[i] * (uint32_t)b->nets[i]->stimCols.size() + (extra ? 1u : 0u);
+        rounds = std::max<size_t>(rounds, len[i]);
+    }
+    std::vector<uint32_t> sched(rounds * nnet, 0xFFFFFFFFu);
+    for (uint32_t i = 0; i < nnet; ++i) {
+        const uint32_t ns = (uint32_t)b->nets[i]->stimCols.size(), body = len[i] - (extra ? 1u : 0u);
+        for (uint32_t r = 0; r < len[i]; ++r) sched[(size_t)r * nnet + i] = r < body ? r % ns : 0u;
+    }
+    HIPCHECK(hipSetDevice(b->nets[0]->device));
+    for (axon_net* n : b->nets) HIPCHECK(hipStreamSynchronize(n->stream));
+    if (rounds > b->sched_rounds) {
+        if (b->d_sched) (void)hipFree(b->d_sched);
+        b->d_sched = nullptr;
+        HIPCHECK(hipMalloc(&b->d_sched, rounds * nnet * 4));
+        b->sched_rounds = rounds;
+    }
+    HIPCHECK(hipMemcpy(b->d_sched, sched.data(), rounds * nnet * 4, hipMemcpyHostToDevice));
+    b->m.sched = b->d_sched;
+    HIPCHECK(hipMemset(b->m.round, 0, 4));
+    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = multi_enqueue_round(b);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "multi capture: %s", hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+    const bool graphs = b->nets[0]->use_graphs;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (size_t r = 0; r < rounds; ++r) {
+        if (graphs) HIPCHECK(hipGraphLaunch(b->graph, b->stream));
+        else if ((e = multi_enqueue_round(b)) != hipSuccess) return fail(AXON_ERR_DEVICE, "%s", hipGetErrorString(e));
+    }
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return AXON_OK;
+}
+
+// Order-independent 64-bit checksum of an engine buffer (sum over words of
+// w * (2i+1) plus a per-word hash), for comparing states without copying them.
+extern "C" axon_status axon_checksum(axon_net* n, axon_buffer which, uint64_t* sum) {
+    if (!n || !sum) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    void* p; size_t words;
+    axon_status st = buffer_ptr(n, which, &p, &words);
+    if (st != AXON_OK) return st;
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipMemsetAsync(n->d_small, 0, 8, n->stream));
+    HIPCHECK(axon::launch_checksum((const uint32_t*)p, words, (unsigned long long*)n->d_small, n->stream));
+    uint64_t v = 0;
+    HIPCHECK(hipMemcpyAsync(&v, n->d_small, 8, hipMemcpyDeviceToHost, n->stream));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    *sum = v;
+    return AXON_OK;
+}
+
+// ================================================================== serving
+// Continuous batching of recall queries over resident networks.  Each network
+// owns a probe batch (its slots); queued requests are admitted FIFO into free
+// slots while the HBM budget allows, every busy slot advances together, and
+// finished slots are gathered and freed at once, so later requests join
+// while earlier ones are still running.  A slot's computation depends only on
+// its own cue and the network, which is what makes a result the same alone
+// and in any mix.
+#include <deque>
+#include <unordered_map>
+
+namespace {
+struct SrvReq {
+    uint64_t id;
+    uint32_t net, T, clamp, bias;
+    double arrival;
+    uint64_t bytes;
+    std::vector<uint32_t> stim;
+};
+struct SrvSlot {
+    bool busy = false;
+    uint64_t id = 0, bytes = 0;
+    uint32_t T = 0;
+    double arrival = 0;
+};
+struct SrvResult {
+    uint32_t T;
+    double arrival, done;
+    std::vector<uint32_t> spikes;
+    std::vector<uint64_t> digest;
+};
+struct SrvNet {
+    axon_probes* pb = nullptr;
+    std::vector<SrvSlot> slots;
+    uint32_t *h_list = nullptr, *d_list = nullptr;    // admit entries
+    uint32_t* h_t = nullptr;                          // slot step counters
+    uint32_t *h_g = nullptr, *d_g = nullptr;          // gather slots + offsets
+    uint32_t *h_out = nullptr, *d_out = nullptr;      // gathered results
+};
+}  // namespace
+
+struct axon_server {
+    std::vector<axon_net*> nets;
+    std::vector<SrvNet> sn;
+    axon_server_config cfg{};
+    std::deque<SrvReq> queue;
+    std::unordered_map<uint64_t, SrvResult> done;
+    std::vector<uint64_t> last_done;  // ids finished by the last tick
+    uint64_t next_id = 1, reserved = 0;
+    uint32_t ncol = 0, inflight = 0;
+    std::chrono::steady_clock::time_point t0;
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+static uint64_t request_bytes(const axon_server* s, uint32_t T) {
+    const uint64_t NC = s->ncol;
+    return (uint64_t)T * NC * 4 + NC * 32 * 4 + 4 * NC * 4;  // history + accumulators + spikes/drive/stim/gact
+}
+
+extern "C" void axon_server_destroy(axon_server* s) {
+    if (!s) return;
+    for (size_t i = 0; i < s->sn.size(); ++i) {
+        SrvNet& v = s->sn[i];
+        if (s->nets[i]) (void)hipStreamSynchronize(s->nets[i]->stream);
+        for (uint32_t* p : {v.h_list, v.h_t, v.h_g, v.h_out})
+            if (p) (void)hipHostFree(p);
+        for (uint32_t* p : {v.d_list, v.d_g, v.d_out})
+            if (p) (void)hipFree(p);
+        axon_probes_destroy(v.pb);
+    }
+    delete s;
+}
+
+extern "C" axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
+                                          axon_server** out) {
+    if (!nets || !cfg || !out || nnet == 0) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->slots_per_net == 0 || cfg->steps_per_tick == 0) return fail(AXON_ERR_INVALID_ARG, "slots and steps_per_tick >= 1");
+    axon_server* s = new axon_server();
+    s->cfg = *cfg;
+    s->ncol = nets[0]->NCOL;
+    s->nets.assign(nets, nets + nnet);
+    s->sn.resize(nnet);
+    auto bail = [&](axon_status st) { axon_server_destroy(s); return st; };
+    for (uint32_t i = 0; i < nnet; ++i) {
+        if (!nets[i] || nets[i]->NCOL != s->ncol) return bail(fail(AXON_ERR_INVALID_ARG, "network %u missing or of another size", i));
+        SrvNet& v = s->sn[i];
+        axon_status st = axon_probes_create(nets[i], cfg->slots_per_net, cfg->max_t, cfg->tile, &v.pb);
+        if (st != AXON_OK) return bail(st);
+        const uint32_t ns = v.pb->d.nslots;
+        v.slots.resize(ns);
+        const size_t listW = (size_t)ns * (4 + s->ncol), outW = (size_t)ns * (s->ncol + 2 * cfg->max_t);
+        if (hipHostMalloc((void**)&v.h_list, listW * 4, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&v.h_t, (size_t)ns * 4, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&v.h_g, (size_t)ns * 8, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&v.h_out, outW * 4, hipHostMallocDefault) != hipSuccess)
+            return bail(fail(AXON_ERR_OUT_OF_MEMORY, "pinned host buffers"));
+        if (hipMalloc((void**)&v.d_list, listW * 4) != hipSuccess || hipMalloc((void**)&v.d_g, (size_t)ns * 8) != hipSuccess ||
+            hipMalloc((void**)&v.d_out, outW * 4) != hipSuccess)
+            return bail(fail(AXON_ERR_OUT_OF_MEMORY, "server device buffers"));
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return AXON_OK;
+}
+
+extern "C" double axon_server_clock(const axon_server* s) { return s ? s->now() : 0.0; }
+
+extern "C" axon_status axon_server_submit(axon_server* s, uint32_t net, const uint32_t* stim, uint32_t T,
+                                          uint32_t clamp_steps, uint32_t use_bias, double arrival_s, uint64_t* id) {
+    if (!s || !stim || !id) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (net >= s->nets.size()) return fail(AXON_ERR_INVALID_ARG, "network %u out of range", net);
+    if (T == 0 || T > s->cfg.max_t) return fail(AXON_ERR_CAPACITY, "T=%u outside 1..max_t=%u", T, s->cfg.max_t);
+    if (use_bias > 1) return fail(AXON_ERR_INVALID_ARG, "use_bias must be 0 or 1");
+    const uint64_t bytes = request_bytes(s, T);
+    if (s->cfg.hbm_budget && bytes > s->cfg.hbm_budget)
+        return fail(AXON_ERR_BUDGET_EXCEEDED, "request needs %llu bytes, budget is %llu", (unsigned long long)bytes,
+                    (unsigned long long)s->cfg.hbm_budget);
+    SrvReq r;
+    r.id = s->next_id++;
+    r.net = net; r.T = T; r.clamp = clamp_steps; r.bias = use_bias;
+    r.arrival = arrival_s >= 0 ? arrival_s : s->now();
+    r.bytes = bytes;
+    r.stim.assign(stim, stim + s->ncol);
+    s->queue.push_back(std::move(r));
+    *id = s->queue.back().id;
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_server_tick(axon_server* s, uint32_t* completed) {
+    if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
+    const uint32_t nnet = (uint32_t)s->nets.size(), NC = s->ncol;
+    HIPCHECK(hipSetDevice(s->nets[0]->device));
+    // 1. admission: FIFO; a request waits for a free slot on its network, and
+    //    admission stops at the first request the HBM budget cannot take
+    std::vector<uint32_t> nadm(nnet, 0);
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        SrvNet& v = s->sn[it->net];
+        if (s->cfg.hbm_budget && s->reserved + it->bytes > s->cfg.hbm_budget) break;
+        uint32_t slot = UINT32_MAX;
+        for (uint32_t k = 0; k < v.slots.size(); ++k)
+            if (!v.slots[k].busy) { slot = k; break; }
+        if (slot == UINT32_MAX) { ++it; continue; }
+        uint32_t* e = v.h_list + (size_t)nadm[it->net]++ * (4 + NC);
+        e[0] = slot; e[1] = it->T; e[2] = it->clamp; e[3] = it->bias;
+        memcpy(e + 4, it->stim.data(), (size_t)NC * 4);
+        v.slots[slot] = SrvSlot{true, it->id, it->bytes, it->T, it->arrival};
+        s->reserved += it->bytes;
+        s->inflight++;
+        it = s->queue.erase(it);
+    }
+    // 2. enqueue per network: admissions, steps, step-counter read-back
+    for (uint32_t i = 0; i < nnet; ++i) {
+        SrvNet& v = s->sn[i];
+        hipStream_t st = s->nets[i]->stream;
+        if (nadm[i]) {
+            HIPCHECK(hipMemcpyAsync(v.d_list, v.h_list, (size_t)nadm[i] * (4 + NC) * 4, hipMemcpyHostToDevice, st));
+            HIPCHECK(axon::launch_probe_admit(v.pb->d, v.d_list, nadm[i], st));
+        }
+        bool any = false;
+        for (auto& sl : v.slots) any |= sl.busy;
+        if (!any) continue;
+        axon_status r = axon_probes_step(v.pb, s->cfg.steps_per_tick);
+        if (r != AXON_OK) return r;
+        HIPCHECK(hipMemcpyAsync(v.h_t, v.pb->d.st_t, v.slots.size() * 4, hipMemcpyDeviceToHost, st));
+    }
+    for (uint32_t i = 0; i < nnet; ++i) HIPCHECK(hipStreamSynchronize(s->nets[i]->stream));
+    // 3. gather the slots that reached their T
+    std::vector<std::vector<uint32_t>> fin(nnet);
+    for (uint32_t i = 0; i < nnet; ++i) {
+        SrvNet& v = s->sn[i];
+        hipStream_t st = s->nets[i]->stream;
+        uint32_t off = 0;
+        for (uint32_t k = 0; k < v.slots.size(); ++k)
+            if (v.slots[k].busy && v.h_t[k] >= v.slots[k].T) {
+                const uint32_t j = (uint32_t)fin[i].size();
+                fin[i].push_back(k);
+                v.h_g[j] = k;
+                v.h_g[v.slots.size() + j] = off;
+                off += NC + 2 * v.slots[k].T;
+            }
+        const uint32_t nf = (uint32_t)fin[i].size();
+        if (!nf) continue;
+        HIPCHECK(hipMemcpyAsync(v.d_g, v.h_g, v.slots.size() * 8, hipMemcpyHostToDevice, st));
+        HIPCHECK(axon::launch_probe_gather(v.pb->d, v.d_g, v.d_g + v.slots.size(), nf, v.d_out, st));
+        HIPCHECK(hipMemcpyAsync(v.h_out, v.d_out, (size_t)off * 4, hipMemcpyDeviceToHost, st));
+    }
+    uint32_t ndone = 0;
+    s->last_done.clear();
+    for (uint32_t i = 0; i < nnet; ++i) {
+        if (fin[i].empty()) continue;
+        SrvNet& v = s->sn[i];
+        HIPCHECK(hipStreamSynchronize(s->nets[i]->stream));
+        const double tdone = s->now();
+        for (uint32_t j = 0; j < fin[i].size(); ++j) {
+            SrvSlot& sl = v.slots[fin[i][j]];
+            const uint32_t* o = v.h_out + v.h_g[v.slots.size() + j];
+            SrvResult r;
+            r.T = sl.T; r.arrival = sl.arrival; r.done = tdone;
+            r.spikes.assign(o, o + NC);
+            r.digest.resize(sl.T);
+            for (uint32_t t = 0; t < sl.T; ++t) r.digest[t] = (uint64_t)o[NC + 2 * t] | ((uint64_t)o[NC + 2 * t + 1] << 32);
+            s->done.emplace(sl.id, std::move(r));
+            s->last_done.push_back(sl.id);
+            s->reserved -= sl.bytes;
+            s->inflight--;
+            sl = SrvSlot{};
+            ++ndone;
+        }
+    }
+    if (completed) *completed = ndone;
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_server_result(axon_server* s, uint64_t id, uint32_t* spikes, uint64_t* row_digest,
+                                          double* latency_s, uint32_t* T) {
+    if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
+    auto it = s->done.find(id);
+    if (it == s->done.end()) return fail(AXON_ERR_NOT_READY, "request %llu not finished", (unsigned long long)id);
+    const SrvResult& r = it->second;
+    if (spikes) memcpy(spikes, r.spikes.data(), r.spikes.size() * 4);
+    if (row_digest) memcpy(row_digest, r.digest.data(), r.digest.size() * 8);
+    if (latency_s) *latency_s = r.done - r.arrival;
+    if (T) *T = r.T;
+    s->done.erase(it);
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_server_status(const axon_server* s, uint32_t* queued, uint32_t* inflight,
+                                          uint64_t* reserved_bytes) {
+    if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
+    if (queued) *queued = (uint32_t)s->queue.size();
+    if (inflight) *inflight = s->inflight;
+    if (reserved_bytes) *reserved_bytes = s->reserved;
+    return AXON_OK;
+}
+
+// ================================================================== autotuning
+// Probe-tile autotuner: runs the caller's probe batch once per tile size
+// (8, 16, 32, 64 slots per W3 tile read), timing the device work with events,
+// and reports the fastest.  Larger tiles read W3 fewer times but hold more
+// accumulators per lane (lower occupancy); the balance depends on the cue
+// sparsity and the number of probes, so it is measured, not assumed.
+extern "C" axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                           uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
+                                           double ms_per_tile[4]) {
+    if (!n || !stims || !best_tile) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    const uint32_t tiles[4] = {8, 16, 32, 64};
+    double best = 1e30;
+    *best_tile = 32;
+    HIPCHECK(hipSetDevice(n->device));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    axon_status st = AXON_OK;
+    for (int i = 0; i < 4 && st == AXON_OK; ++i) {
+        axon_probes* b = nullptr;
+        if ((st = axon_probes_create(n, count, T, tiles[i], &b)) != AXON_OK) break;
+        std::vector<uint32_t> Ts(b->d.nslots, 0);
+        std::fill(Ts.begin(), Ts.begin() + count, T);
+        std::vector<uint32_t> pad((size_t)(b->d.nslots - count) * n->NCOL, 0);
+        double ms = 0;
+        for (int rep = 0; rep < 2 && st == AXON_OK; ++rep) {  // first pass captures the graph
+            st = axon_probes_load(b, 0, count, stims, Ts.data(), clamp_steps, use_bias);
+            if (st == AXON_OK && count < b->d.nslots)
+                st = axon_probes_load(b, count, b->d.nslots - count, pad.data(), Ts.data() + count, clamp_steps, use_bias);
+            if (st != AXON_OK) break;
+            if (hipEventRecord(e0, n->stream) != hipSuccess) { st = fail(AXON_ERR_DEVICE, "event"); break; }
+            if ((st = axon_probes_step(b, T)) != AXON_OK) break;
+            if (hipEventRecord(e1, n->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                st = fail(AXON_ERR_DEVICE, "event");
+                break;
+            }
+            float f = 0;
+            (void)hipEventElapsedTime(&f, e0, e1);
+            ms = f;
+        }
+        axon_probes_destroy(b);
+        if (st != AXON_OK) break;
+        if (ms_per_tile) ms_per_tile[i] = ms;
+        if (ms < best) { best = ms; *best_tile = tiles[i]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+extern "C" axon_status axon_server_completed(const axon_server* s, uint64_t* ids, uint32_t max, uint32_t* n) {
+    if (!s || !n || (max && !ids)) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    const uint32_t k = (uint32_t)std::min<size_t>(max, s->last_done.size());
+    if (k) memcpy(ids, s->last_done.data(), (size_t)k * 8);
+    *n = (uint32_t)s->last_done.size();
     return AXON_OK;
 }

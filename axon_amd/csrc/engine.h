@@ -45,6 +45,9 @@ struct NetDev {
     // number of tiles plasticity rewrote; accumulated across steps.
     uint32_t* cntProp;   // ncol
     uint32_t* cntLearn;  // ncol * 4
+    uint8_t* colNZ;      // ncol (+pad to 16): column spiked in the last step
+    uint32_t* learnList; // [2][ncol]: columns learning at step t, list t & 1
+    uint32_t* learnCnt;  // [2] list lengths (reset by the step after next's propagation)
     uint32_t ncol, incols, inbr, topkpar;
     uint32_t nstats, ndlog, maxt, pad;
 };
@@ -70,14 +73,34 @@ struct ProbeDev {
     uint32_t k, facil, dthr, boost, mode, biasShift, target, topkpar;
 };
 
+// Lock-step batch of independent networks (same dynamics parameters).
+struct MultiDev {
+    const NetDev* nets;       // [nnet], device copy
+    const uint32_t* sched;    // [rounds][nnet]: pattern of this round, ~0 = idle
+    uint32_t* round;          // device round counter
+    const uint64_t* stimTabs; // [nnet]: device address of the [stim][ncol] clamp patterns
+    uint32_t nnet, pad;
+};
+
 // ---- launchers (kernels.hip) ; all asynchronous on `s` ----
+hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats, hipStream_t s);
+hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
+hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s);
+hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
+                             hipStream_t s);
+hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
+hipError_t launch_multi_end(const MultiDev& m, hipStream_t s);
 hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, hipStream_t s);
 hipError_t launch_probe_reset(const ProbeDev& d, uint32_t first, uint32_t count, hipStream_t s);
+hipError_t launch_probe_admit(const ProbeDev& d, const uint32_t* list, uint32_t n, hipStream_t s);
+hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const uint32_t* offs, uint32_t n,
+                               uint32_t* out, hipStream_t s);
 hipError_t launch_init(const NetDev& d, uint32_t nand, uint32_t seed, hipStream_t s);
 hipError_t launch_clear(const NetDev& d, hipStream_t s);
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s);
 hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s);
-hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, hipStream_t s);
+hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s);
+hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s);
 
 hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s);
 hipError_t launch_within(const NetDev& d, const uint32_t* ball, const uint32_t* postMask, uint32_t useMask,
@@ -87,6 +110,7 @@ hipError_t launch_untouched(const NetDev& d, uint32_t* out, hipStream_t s);
 hipError_t launch_wdiag(const NetDev& d, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
                         uint32_t sel, uint32_t* hist, hipStream_t s);
 hipError_t launch_fill(uint32_t* p, uint32_t value, size_t nwords, hipStream_t s);
+hipError_t launch_checksum(const uint32_t* p, size_t n, unsigned long long* out, hipStream_t s);
 
 hipError_t launch_bstest(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1, uint32_t* a2,
                          uint32_t* a3, const uint32_t* m, size_t nwords, hipStream_t s);

@@ -56,6 +56,12 @@ __device__ __forceinline__ uint32_t half_or(uint32_t x) {
     return x;
 }
 
+// Per-lane choice between two wave-uniform uint4 values (component-wise
+// selects, so the compiler keeps everything in registers).
+__device__ __forceinline__ uint4 pick4(bool hi, const uint4& h, const uint4& l) {
+    return make_uint4(hi ? h.x : l.x, hi ? h.y : l.y, hi ? h.z : l.z, hi ? h.w : l.w);
+}
+
 // ---- exact k-of-32 winners-take-all with margin (kwta_ballot_margin, mm:134-144) ----
 // Lanes 0..31 hold the column's 32 values (lanes 32..63 duplicate them).
 // The reference ranks by (value desc, lane asc): rank_l = #{j: v_j > v_l or
@@ -200,63 +206,76 @@ __global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint32_t
 // (k_clear mm:497, memset of bStat mm:824).  A kernel, not a memset, so the
 // whole presentation can be captured in one hipGraph.
 __global__ __launch_bounds__(256) void k_clear(uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint32_t ncol,
-                                               uint32_t* __restrict__ st, uint32_t nst) {
+                                               uint32_t* __restrict__ st, uint32_t nst, uint8_t* __restrict__ nz,
+                                               uint32_t* __restrict__ lcnt) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < ncol) { a[i] = 0u; b[i] = 0u; }
+    if (i < ncol) { a[i] = 0u; b[i] = 0u; nz[i] = 0u; }
+    if (i < 2u) lcnt[i] = 0u;
     if (i < nst) st[i] = 0u;
 }
 
 // ============================================================== propagation
 // k_drive, mm:150-190.  One wave per column; lane = (j, neuron).
 // Branches whose presynaptic group carries no spike are skipped wave-uniformly
-// (mm:169); the remaining tiles are fetched four at a time so several 1 KiB
+// (mm:169); the remaining tiles are fetched eight at a time so several 1 KiB
 // loads are in flight per wave.  Spike words of a group are wave-uniform and
 // come through scalar loads.
-__global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
+// `gl` is conn[col*32 + (lane & 31)] (loaded by the caller so that the load
+// overlaps the staging of `fired`).  `fired`, when not null, is the LDS copy
+// of the per-column "spiked last step" bytes: a branch is active when any of
+// the 8 columns of its presynaptic group fired, one 8-byte LDS read.
+__device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                          uint32_t col, uint32_t gl, const uint8_t* fired) {
     const uint32_t lane = lane_id();
-    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (col >= d.ncol) return;
     const bool hi = lane >= 32u;
     const uint32_t n = lane & 31u;
     const uint32_t base = col * NSLOT;
 
-    const uint32_t gl = d.conn[base + n];
-    const uint4 sp = reinterpret_cast<const uint4*>(spkIn)[2u * gl + (hi ? 1u : 0u)];
-    const uint32_t anyh = sp.x | sp.y | sp.z | sp.w;
-    uint32_t mask = ballot32((anyh | __shfl_xor(anyh, 32)) != 0u);
+    uint32_t mask;
+    if (fired) {
+        mask = ballot32(*reinterpret_cast<const uint64_t*>(fired + 8u * gl) != 0ull);
+    } else {
+        const uint4 sp = reinterpret_cast<const uint4*>(spkIn)[2u * gl + (hi ? 1u : 0u)];
+        const uint32_t anyh = sp.x | sp.y | sp.z | sp.w;
+        mask = ballot32((anyh | __shfl_xor(anyh, 32)) != 0u);
+    }
     if (lane == 0u) d.cntProp[col] += popc(mask);
 
     const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
     uint32_t acc = 0u, bb = 0u;
+    constexpr int DB = 8;  // active tiles fetched per round trip
     while (mask) {
-        uint32_t sidx[4];
-        uint32_t cnt = 0;
+        uint32_t sidx[DB];
+        uint4 w[DB];
+        uint4 sv[DB];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (mask) { sidx[k] = __builtin_ctz(mask); mask &= mask - 1u; ++cnt; }
-            else sidx[k] = sidx[0];
+        for (int k = 0; k < DB; ++k) {
+            sidx[k] = 32u;
+            w[k] = make_uint4(0u, 0u, 0u, 0u);
+            sv[k] = w[k];
+            if (mask) {
+                const uint32_t sk = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                sidx[k] = sk;
+                w[k] = Wc[(size_t)sk * TILE_U4];
+                const uint32_t g = __builtin_amdgcn_readlane(gl, sk);
+                const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
+                const uint4 lo4 = s4[0], hi4 = s4[1];
+                sv[k] = pick4(hi, hi4, lo4);
+            }
         }
-        uint4 w[4];
-        uint4 sv[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            w[k] = Wc[(size_t)sidx[k] * TILE_U4];
-            const uint32_t g = __builtin_amdgcn_readlane(gl, sidx[k]);
-            const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
-            const uint4 lo4 = s4[0], hi4 = s4[1];
-            sv[k] = hi ? hi4 : lo4;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if ((uint32_t)k >= cnt) break;
-            uint32_t c = popc(w[k].x & sv[k].x) + popc(w[k].y & sv[k].y) + popc(w[k].z & sv[k].z) +
-                         popc(w[k].w & sv[k].w);
-            c += __shfl_xor(c, 32);
-            const bool nmda = c >= P.dthr;
-            if (nmda) bb |= 1u << sidx[k];
-            if (P.mode == 0u) acc += c;
-            else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
-            else acc += nmda ? 1u : 0u;
+        for (int k = 0; k < DB; ++k) {
+            if (sidx[k] < 32u) {
+                uint32_t c = popc(w[k].x & sv[k].x) + popc(w[k].y & sv[k].y) + popc(w[k].z & sv[k].z) +
+                             popc(w[k].w & sv[k].w);
+                c += __shfl_xor(c, 32);
+                const bool nmda = c >= P.dthr;
+                if (nmda) bb |= 1u << sidx[k];
+                if (P.mode == 0u) acc += c;
+                else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
+                else acc += nmda ? 1u : 0u;
+            }
         }
     }
     if (P.facil != 0u && ((spkIn[col] >> n) & 1u) != 0u) acc += P.facil;
@@ -268,34 +287,76 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
     if (lane == 0u) d.drive[col] = drv;
 }
 
+constexpr uint32_t FIRED_LDS_COLS = 16384;  // columns whose fired bytes fit the LDS stage
+
+// Stage the fired bytes of all columns in LDS while this wave's conn entries load.
+__device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds) {
+    if (d.ncol > FIRED_LDS_COLS) return nullptr;
+    const uint4* src = reinterpret_cast<const uint4*>(d.colNZ);
+    for (uint32_t i = threadIdx.x; i < (d.ncol + 15u) / 16u; i += blockDim.x) lds[i] = src[i];
+    __syncthreads();
+    return reinterpret_cast<const uint8_t*>(lds);
+}
+
+__global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
+    __shared__ uint4 fired[FIRED_LDS_COLS / 16];
+    // the learning-column list of step t-1 has been consumed; free it for step t+1
+    if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t gl = col < d.ncol ? d.conn[col * NSLOT + (lane_id() & 31u)] : 0u;
+    const uint8_t* f = stage_fired(d, fired);
+    if (col >= d.ncol) return;
+    drive_col(d, P, spkIn, col, gl, f);
+}
+
 // ============================================================== column top-K
 // theta = target-th largest drive over recurrent columns by three byte-wise
 // histogram passes (k_topk_par mm:301-352, k_topk_serial mm:259-292).  One
-// workgroup per instance.  Histogram bins are accumulated with one LDS atomic
-// per distinct bin per wave; the 256-bin suffix scan and the crossing bin are
-// done by wave 0 in registers.
+// workgroup per instance.  The first four values of every thread are loaded
+// once and kept in registers for all three passes; histogram bins are plain
+// LDS atomics; the 256-bin suffix scan and the crossing bin are done by wave 0
+// in registers.
 // Returns max(theta, 1) to every thread of the block; thread 0 also stores
 // (theta, need) to `theta` when it is not null.
 __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __restrict__ theta, uint32_t ncol,
                                uint32_t inCols, uint32_t target, uint32_t par) {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t sh_need, sh_prefix;
-    const uint32_t tid = threadIdx.x, lane = lane_id();
-    if (tid == 0) { sh_need = target; sh_prefix = 0u; }
-    for (int by = 2; by >= 0; --by) {
+    constexpr int VPT = 4;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), nth = blockDim.x;
+    uint32_t rv[VPT];
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+        const uint32_t c = inCols + tid + (uint32_t)i * nth;
+        rv[i] = c < ncol ? drive[c] : 0u;
+    }
+    const uint32_t cRest = inCols + (uint32_t)VPT * nth;  // values beyond the register cache
+    __shared__ uint32_t sh_or;
+    if (tid == 0) { sh_need = target; sh_prefix = 0u; sh_or = 0u; }
+    // A leading pass whose byte is zero for every value puts them all in bin 0
+    // and leaves (prefix, need) unchanged under both selection rules (except
+    // the serial rule with need == 0, which picks bin 255), so it is skipped.
+    uint32_t orv = 0;
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) orv |= rv[i];
+    for (uint32_t c = cRest + tid; c < ncol; c += nth) orv |= drive[c];
+    for (uint32_t off = 32; off >= 1u; off >>= 1) orv |= __shfl_xor(orv, off);
+    __syncthreads();
+    if (lane == 0u) atomicOr(&sh_or, orv);
+    __syncthreads();
+    int byStart = 2;
+    if (par || target > 0u) byStart = (sh_or >> 16) ? 2 : (sh_or >> 8) ? 1 : 0;
+    for (int by = byStart; by >= 0; --by) {
         if (tid < 256u) hist[tid] = 0u;
         __syncthreads();
         const uint32_t need = sh_need, prefix = sh_prefix;
         const uint32_t shHi = (uint32_t)(by + 1) * 8u, shLo = (uint32_t)by * 8u;
-        for (uint32_t c0 = inCols; c0 < ncol; c0 += blockDim.x) {
-            const uint32_t c = c0 + tid;
-            uint32_t bin = 0;
-            bool act = false;
-            if (c < ncol) {
-                const uint32_t dv = drive[c];
-                act = (dv >> shHi) == (prefix >> shHi);
-                bin = (dv >> shLo) & 255u;
-            }
+        // one LDS atomic per distinct bin per wave (drives cluster in few bins)
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const uint32_t c = inCols + tid + (uint32_t)i * nth;
+            const bool act = c < ncol && (rv[i] >> shHi) == (prefix >> shHi);
+            const uint32_t bin = (rv[i] >> shLo) & 255u;
             uint64_t m = __ballot(act);
             while (m) {
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
@@ -304,6 +365,10 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
                 if (lane == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(same));
                 m &= ~same;
             }
+        }
+        for (uint32_t c = cRest + tid; c < ncol; c += nth) {
+            const uint32_t dv = drive[c];
+            if ((dv >> shHi) == (prefix >> shHi)) atomicAdd(&hist[(dv >> shLo) & 255u], 1u);
         }
         __syncthreads();
         if (tid < 64u) {
@@ -358,141 +423,187 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
     return th;
 }
 
-__global__ __launch_bounds__(256) void k_topk(NetDev d, Params P) {
+__global__ __launch_bounds__(1024) void k_topk(NetDev d, Params P) {
     topk_block(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
-__global__ __launch_bounds__(256) void k_topk_test(const uint32_t* __restrict__ drive, uint32_t* theta,
+__global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__ drive, uint32_t* theta,
                                                    uint32_t ncol, uint32_t inCols, uint32_t target, uint32_t par) {
     topk_block(drive + (size_t)blockIdx.x * ncol, theta + 2 * blockIdx.x, ncol, inCols, target, par);
 }
 
 // ============================================================== readout + plasticity
-// k_emit (mm:192-236), k_bias (mm:240-253) and k_learn (mm:436-495) fused per
-// column: everything after the global theta is local to the column.
-//   LEARN = false: 4 columns per workgroup, one wave each (readout steps).
-//   LEARN = true : 1 column per workgroup; wave 0 emits and updates bias, and
-//                  each of the 4 waves applies plasticity to 8 of the 32
-//                  branches, so a learning column's 32 tile read-modify-writes
-//                  are spread over 4 waves.
-template <bool LEARN>
-__global__ __launch_bounds__(256) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
-                                              uint32_t* __restrict__ spkOut) {
+// k_emit (mm:192-236) and k_bias (mm:240-253) fused: everything after the
+// global theta is local to a column.  A workgroup owns 4 columns, one wave
+// each.  The k-of-32 selection is computed only where a spike can leave (gate
+// passed or input clamped); every other column is silent, and its margin
+// flag -- which k_learn reads only for spiking columns -- is written as 0.
+// The wave also records whether its column fired (read by the next
+// propagation) and whether it learns this step (read by k_learn).
+// STATS = false drops the side outputs (stats and drive-log atomics, LTP/LTD
+// counters); spikes, history, margin, bias and synapses are unaffected.
+template <bool LEARN, bool STATS>
+__device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                          uint32_t* __restrict__ spkOut, uint32_t col0) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t n = lane & 31u;
+    const bool hi = lane >= 32u;
+    const uint32_t th = max(d.theta[0], 1u);
+    {   // ---- phase 1: this wave's column
+        const uint32_t col = col0 + wave;
+        uint32_t b = 0u, learnCol = 0u;
+        if (col < d.ncol) {
+            const uint32_t gid = col * NPC + n;
+            const bool clampCol = P.clamp != 0u && col < P.inCols;
+            const bool gatePass = d.drive[col] >= th;
+            const bool rec = col >= P.inCols;
+            uint32_t vk = 0u, vk1 = 0u;
+            const uint32_t v = d.acc[gid];
+            if (clampCol || gatePass) {
+                const Kwta kw = kwta32(v, P.k, n);
+                vk = kw.vk; vk1 = kw.vk1;
+                b = gatePass ? kw.ball : 0u;
+                if (clampCol) b = d.stim[col];
+            }
+            const bool post = ((b >> n) & 1u) != 0u;
+            if (STATS) {
+                const uint32_t x = (!hi && post && rec) ? popc(d.bbits[gid]) : 0u;
+                const bool mem = !hi && rec && ((d.refBall[col] >> n) & 1u) != 0u;
+                const bool shared = d.mult[gid] > 1u;
+                const uint32_t sumS = wave_sum(mem && shared ? v : 0u);
+                const uint32_t sumE = wave_sum(mem && !shared ? v : 0u);
+                const uint32_t cntS = popc(ballot32(mem && shared));
+                const uint32_t cntE = popc(ballot32(mem && !shared));
+                const uint32_t xs = wave_sum(x);
+                if (lane == 0u) {
+                    const uint32_t t = P.step;
+                    if (xs) atomicAdd(&d.stats[2], xs);
+                    if (cntS) { atomicAdd(&d.driveLog[t * 4u + 0u], sumS); atomicAdd(&d.driveLog[t * 4u + 1u], cntS); }
+                    if (cntE) { atomicAdd(&d.driveLog[t * 4u + 2u], sumE); atomicAdd(&d.driveLog[t * 4u + 3u], cntE); }
+                    if (b != 0u && rec) {
+                        const uint32_t sb = 64u + t * 8u;
+                        atomicAdd(&d.stats[0], 1u);
+                        atomicAdd(&d.stats[1], popc(b));
+                        atomicAdd(&d.stats[sb + 0u], vk - vk1);
+                        atomicAdd(&d.stats[sb + 1u], vk);
+                        atomicAdd(&d.stats[sb + 2u], 1u);
+                        if (vk == vk1) atomicAdd(&d.stats[sb + 3u], 1u);
+                        if (vk == 0u) atomicAdd(&d.stats[sb + 4u], 1u);
+                    }
+                }
+            }
+            learnCol = (LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1))) ? 1u : 0u;
+            if (lane == 0u) {
+                spkOut[col] = b;
+                d.marg[col] = (vk > vk1) ? 1u : 0u;
+                d.colNZ[col] = b != 0u ? 1u : 0u;
+                if (LEARN && learnCol) {  // compact list of this step's learning columns
+                    const uint32_t par = P.step & 1u;
+                    d.learnList[par * d.ncol + atomicAdd(&d.learnCnt[par], 1u)] = col;
+                }
+                if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
+            }
+            if (LEARN && !hi && rec) {  // k_bias
+                const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
+                const int nb = (int)d.bias[gid] + (post ? (int)P.biasUp : -dec);
+                d.bias[gid] = (uint32_t)max(0, nb);
+            }
+        }
+    }
+}
+
+// k_learn (mm:436-495) for the columns emit listed: one listed column per
+// workgroup (grid-stride), the 4 waves rewriting its voted tiles together,
+// 8 branches per wave, so every learning column proceeds in parallel.
+template <bool STATS>
+__device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                           const uint32_t* __restrict__ spkOut) {
     __shared__ uint32_t sLTP[4][32], sLTD[4][32];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
-    const uint32_t col = LEARN ? blockIdx.x : blockIdx.x * 4u + wave;
-    if (!LEARN && col >= d.ncol) return;
-    const bool writer = LEARN ? (wave == 0u) : true;
-    const uint32_t gid = col * NPC + n;
-    const uint32_t th = max(d.theta[0], 1u);
-    const bool clampCol = P.clamp != 0u && col < P.inCols;
-    const bool gatePass = d.drive[col] >= th;
-    if (!writer && !(clampCol || gatePass)) return;  // column silent: nothing to learn
-
-    const uint32_t v = d.acc[gid];
-    const Kwta kw = kwta32(v, P.k, n);
-    uint32_t b = gatePass ? kw.ball : 0u;
-    if (clampCol) b = d.stim[col];
-    const bool post = ((b >> n) & 1u) != 0u;
-
-    if (writer) {
-        const bool rec = col >= P.inCols;
-        const uint32_t x = (!hi && post && rec) ? popc(d.bbits[gid]) : 0u;
-        const bool mem = !hi && rec && ((d.refBall[col] >> n) & 1u) != 0u;
-        const bool shared = d.mult[gid] > 1u;
-        const uint32_t sumS = wave_sum(mem && shared ? v : 0u);
-        const uint32_t sumE = wave_sum(mem && !shared ? v : 0u);
-        const uint32_t cntS = popc(ballot32(mem && shared));
-        const uint32_t cntE = popc(ballot32(mem && !shared));
-        const uint32_t xs = wave_sum(x);
-        if (lane == 0u) {
-            const uint32_t t = P.step;
-            if (xs) atomicAdd(&d.stats[2], xs);
-            if (cntS) { atomicAdd(&d.driveLog[t * 4u + 0u], sumS); atomicAdd(&d.driveLog[t * 4u + 1u], cntS); }
-            if (cntE) { atomicAdd(&d.driveLog[t * 4u + 2u], sumE); atomicAdd(&d.driveLog[t * 4u + 3u], cntE); }
-            spkOut[col] = b;
-            d.marg[col] = (kw.vk > kw.vk1) ? 1u : 0u;
-            if (P.histOn != 0u) d.hist[(size_t)t * d.ncol + col] = b;
-            if (b != 0u && rec) {
-                const uint32_t sb = 64u + t * 8u;
-                atomicAdd(&d.stats[0], 1u);
-                atomicAdd(&d.stats[1], popc(b));
-                atomicAdd(&d.stats[sb + 0u], kw.vk - kw.vk1);
-                atomicAdd(&d.stats[sb + 1u], kw.vk);
-                atomicAdd(&d.stats[sb + 2u], 1u);
-                if (kw.vk == kw.vk1) atomicAdd(&d.stats[sb + 3u], 1u);
-                if (kw.vk == 0u) atomicAdd(&d.stats[sb + 4u], 1u);
+    const uint32_t par = P.step & 1u;
+    const uint32_t nlist = d.learnCnt[par];
+    for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const uint32_t col = d.learnList[par * d.ncol + li];
+        const uint32_t b = spkOut[col];
+        const bool post = ((b >> n) & 1u) != 0u;
+        const uint32_t bb = d.bbits[col * NPC + n];
+        uint32_t voted = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 8u; ++i) {
+            const uint32_t s = wave * 8u + i;
+            if (ballot32(post && ((bb >> s) & 1u) != 0u)) voted |= 1u << i;
+        }
+        if (lane == 0u) d.cntLearn[col * 4u + wave] += popc(voted);
+        uint32_t nLTP = 0, nLTD = 0;
+        const size_t tileBase = (size_t)col * NSLOT * TILE_U4 + lane;
+        while (voted) {
+            constexpr int LB = 4;  // voted tiles whose four planes are in flight together
+            uint32_t si[LB];
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int k = 0; k < LB; ++k) {
+                if (voted) { si[k] = wave * 8u + __builtin_ctz(voted); voted &= voted - 1u; ++cnt; }
+                else si[k] = si[0];
+            }
+            uint4 a0[LB], a1[LB], a2[LB], a3[LB], sp[LB];
+#pragma unroll
+            for (int k = 0; k < LB; ++k) {
+                if ((uint32_t)k < cnt) {
+                    const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
+                    a0[k] = d.W0[idx]; a1[k] = d.W1[idx]; a2[k] = d.W2[idx]; a3[k] = d.W3[idx];
+                    const uint32_t g = __builtin_amdgcn_readfirstlane(d.conn[col * NSLOT + si[k]]);
+                    const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
+                    const uint4 lo4 = s4[0], hi4 = s4[1];
+                    sp[k] = pick4(hi, hi4, lo4);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < LB; ++k) {
+                if ((uint32_t)k < cnt) {
+                    const bool want = post && ((bb >> si[k]) & 1u) != 0u;
+                    learn_word(a0[k].x, a1[k].x, a2[k].x, a3[k].x, sp[k].x, want, P, nLTP, nLTD);
+                    learn_word(a0[k].y, a1[k].y, a2[k].y, a3[k].y, sp[k].y, want, P, nLTP, nLTD);
+                    learn_word(a0[k].z, a1[k].z, a2[k].z, a3[k].z, sp[k].z, want, P, nLTP, nLTD);
+                    learn_word(a0[k].w, a1[k].w, a2[k].w, a3[k].w, sp[k].w, want, P, nLTP, nLTD);
+                    const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
+                    d.W0[idx] = a0[k]; d.W1[idx] = a1[k]; d.W2[idx] = a2[k]; d.W3[idx] = a3[k];
+                }
             }
         }
-        if (LEARN && !hi && rec) {  // k_bias
-            const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
-            const int nb = (int)d.bias[gid] + (post ? (int)P.biasUp : -dec);
-            d.bias[gid] = (uint32_t)max(0, nb);
+        if (STATS) {  // per-neuron totals over both halves and all four waves (mm:491-494)
+            nLTP += __shfl_xor(nLTP, 32);
+            nLTD += __shfl_xor(nLTD, 32);
+            if (!hi) { sLTP[wave][n] = nLTP; sLTD[wave][n] = nLTD; }
+            __syncthreads();
+            if (wave == 0u) {
+                const uint32_t tp = hi ? 0u : sLTP[0][n] + sLTP[1][n] + sLTP[2][n] + sLTP[3][n];
+                const uint32_t td = hi ? 0u : sLTD[0][n] + sLTD[1][n] + sLTD[2][n] + sLTD[3][n];
+                const uint32_t sp_ = wave_sum(tp), sd_ = wave_sum(td);
+                const uint32_t cp = popc(ballot32(tp != 0u)), cd = popc(ballot32(td != 0u));
+                if (lane == 0u) {
+                    if (sp_) atomicAdd(&d.stats[8], sp_);
+                    if (sd_) atomicAdd(&d.stats[9], sd_);
+                    if (cp) atomicAdd(&d.stats[10], cp);
+                    if (cd) atomicAdd(&d.stats[11], cd);
+                }
+            }
+            __syncthreads();
         }
     }
-    if (!LEARN) return;
-    // ---- k_learn for branches 8*wave .. 8*wave+7 ----
-    if (P.phase == 3u || b == 0u) return;
-    if (P.marginGate != 0u && !(kw.vk > kw.vk1)) return;
-    const uint32_t bb = d.bbits[gid];
-    uint32_t voted = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < 8u; ++i) {
-        const uint32_t s = wave * 8u + i;
-        if (ballot32(post && ((bb >> s) & 1u) != 0u)) voted |= 1u << i;
-    }
-    if (lane == 0u) d.cntLearn[col * 4u + wave] += popc(voted);
-    uint32_t nLTP = 0, nLTD = 0;
-    const size_t tileBase = (size_t)col * NSLOT * TILE_U4 + lane;
-    while (voted) {
-        uint32_t si[2];
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if (voted) { si[k] = wave * 8u + __builtin_ctz(voted); voted &= voted - 1u; ++cnt; }
-            else si[k] = si[0];
-        }
-        uint4 a0[2], a1[2], a2[2], a3[2], sp[2];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
-            a0[k] = d.W0[idx]; a1[k] = d.W1[idx]; a2[k] = d.W2[idx]; a3[k] = d.W3[idx];
-            const uint32_t g = __builtin_amdgcn_readfirstlane(d.conn[col * NSLOT + si[k]]);
-            const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
-            const uint4 lo4 = s4[0], hi4 = s4[1];
-            sp[k] = hi ? hi4 : lo4;
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if ((uint32_t)k >= cnt) break;
-            const bool want = post && ((bb >> si[k]) & 1u) != 0u;
-            learn_word(a0[k].x, a1[k].x, a2[k].x, a3[k].x, sp[k].x, want, P, nLTP, nLTD);
-            learn_word(a0[k].y, a1[k].y, a2[k].y, a3[k].y, sp[k].y, want, P, nLTP, nLTD);
-            learn_word(a0[k].z, a1[k].z, a2[k].z, a3[k].z, sp[k].z, want, P, nLTP, nLTD);
-            learn_word(a0[k].w, a1[k].w, a2[k].w, a3[k].w, sp[k].w, want, P, nLTP, nLTD);
-            const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
-            d.W0[idx] = a0[k]; d.W1[idx] = a1[k]; d.W2[idx] = a2[k]; d.W3[idx] = a3[k];
-        }
-    }
-    // per-neuron totals over both halves and all four waves (mm:491-494 counts per neuron)
-    nLTP += __shfl_xor(nLTP, 32);
-    nLTD += __shfl_xor(nLTD, 32);
-    if (!hi) { sLTP[wave][n] = nLTP; sLTD[wave][n] = nLTD; }
-    __syncthreads();
-    if (wave == 0u) {
-        const uint32_t tp = hi ? 0u : sLTP[0][n] + sLTP[1][n] + sLTP[2][n] + sLTP[3][n];
-        const uint32_t td = hi ? 0u : sLTD[0][n] + sLTD[1][n] + sLTD[2][n] + sLTD[3][n];
-        const uint32_t sp_ = wave_sum(tp), sd_ = wave_sum(td);
-        const uint32_t cp = popc(ballot32(tp != 0u)), cd = popc(ballot32(td != 0u));
-        if (lane == 0u) {
-            if (sp_) atomicAdd(&d.stats[8], sp_);
-            if (sd_) atomicAdd(&d.stats[9], sd_);
-            if (cp) atomicAdd(&d.stats[10], cp);
-            if (cd) atomicAdd(&d.stats[11], cd);
-        }
-    }
+}
+
+template <bool LEARN, bool STATS>
+__global__ __launch_bounds__(256) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                              uint32_t* __restrict__ spkOut) {
+    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, blockIdx.x * 4u);
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_learn(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                               const uint32_t* __restrict__ spkOut) {
+    learn_cols<STATS>(d, P, spkIn, spkOut);
 }
 
 // ============================================================== readouts
@@ -703,7 +814,7 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
             if ((bm[p] >> s) & 1u) {
                 const uint4* s4 = reinterpret_cast<const uint4*>(d.spk + (size_t)(slot0 + p) * d.ncol + 8u * g);
                 const uint4 lo4 = s4[0], hi4 = s4[1];
-                const uint4 sv = hi ? hi4 : lo4;
+                const uint4 sv = pick4(hi, hi4, lo4);
                 uint32_t c = popc(w.x & sv.x) + popc(w.y & sv.y) + popc(w.z & sv.z) + popc(w.w & sv.w);
                 c += __shfl_xor(c, 32);
                 const bool nmda = c >= d.dthr;
@@ -789,6 +900,132 @@ __global__ __launch_bounds__(256) void k_probe_reset(ProbeDev d, uint32_t first,
     if (threadIdx.x == 0u) d.st_t[sl] = 0u;
 }
 
+// ============================================================== many networks
+// Lock-step training of independent networks: one launch covers every
+// network that still has work.  The per-network work queues are a device
+// table sched[round][net] (pattern index, or ~0 once the network is done) and
+// a device round counter that the last kernel of a presentation advances, so
+// one captured presentation graph is replayed for every round without host
+// writes in between.
+__device__ __forceinline__ uint32_t mb_pattern(const MultiDev& m, uint32_t net) {
+    return m.sched[(size_t)(*m.round) * m.nnet + net];
+}
+
+__global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
+    const uint32_t net = blockIdx.y;
+    const uint32_t p = mb_pattern(m, net);
+    if (p == 0xFFFFFFFFu) return;
+    const NetDev d = m.nets[net];
+    const uint32_t* tab = reinterpret_cast<const uint32_t*>(m.stimTabs[net]) + (size_t)p * d.ncol;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < d.ncol) { d.stim[i] = tab[i]; d.spk[0][i] = 0u; d.spk[1][i] = 0u; d.colNZ[i] = 0u; }
+    if (i < 2u) d.learnCnt[i] = 0u;
+    if (i < d.nstats) d.stats[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t) {
+    const uint32_t net = blockIdx.y;
+    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
+    __shared__ uint4 fired[FIRED_LDS_COLS / 16];
+    const NetDev d = m.nets[net];
+    if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t gl = col < d.ncol ? d.conn[col * NSLOT + (lane_id() & 31u)] : 0u;
+    const uint8_t* f = stage_fired(d, fired);
+    if (col >= d.ncol) return;
+    drive_col(d, P, (t & 1u) ? d.spk[1] : d.spk[0], col, gl, f);  // no dynamic index into the local copy
+}
+
+__global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
+    const uint32_t net = blockIdx.x;
+    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
+    const NetDev d = m.nets[net];
+    topk_block(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+}
+
+template <bool LEARN>
+__global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t t) {
+    const uint32_t net = blockIdx.y;
+    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
+    const NetDev d = m.nets[net];
+    uint32_t* const s0 = d.spk[0];
+    uint32_t* const s1 = d.spk[1];
+    emit_cols<LEARN, false>(d, P, (t & 1u) ? s1 : s0, (t & 1u) ? s0 : s1, blockIdx.x * 4u);
+}
+
+__global__ __launch_bounds__(256) void k_mb_learn(MultiDev m, Params P, uint32_t t) {
+    const uint32_t net = blockIdx.y;
+    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
+    const NetDev d = m.nets[net];
+    uint32_t* const s0 = d.spk[0];
+    uint32_t* const s1 = d.spk[1];
+    learn_cols<false>(d, P, (t & 1u) ? s1 : s0, (t & 1u) ? s0 : s1);
+}
+
+__global__ void k_mb_end(MultiDev m) {
+    if (threadIdx.x == 0u) *m.round += 1u;
+}
+
+// Order-independent checksum: sum of w_i * (2i + 1) + hash32(w_i ^ i) (mod 2^64).
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ p, size_t n,
+                                                  unsigned long long* out) {
+    unsigned long long acc = 0;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) {
+        const uint32_t w = p[i];
+        acc += (unsigned long long)w * (2ull * i + 1ull) + hash32(w ^ (uint32_t)i ^ (uint32_t)(i >> 32));
+    }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane_id() == 0u) atomicAdd(out, acc);
+}
+
+// Serving: scatter a packed list of admitted requests into their slots.
+// Entry layout: [slot, T, clamp, useBias, stim[ncol]].
+__global__ __launch_bounds__(256) void k_probe_admit(ProbeDev d, const uint32_t* __restrict__ list, uint32_t n) {
+    if (blockIdx.x >= n) return;
+    const uint32_t* e = list + (size_t)blockIdx.x * (4u + d.ncol);
+    const uint32_t sl = e[0];
+    if (sl >= d.nslots) return;
+    for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) {
+        d.stim[(size_t)sl * d.ncol + i] = e[4 + i];
+        d.spk[(size_t)sl * d.ncol + i] = 0u;
+    }
+    for (uint32_t i = threadIdx.x; i < d.gw; i += 256u) d.gact[(size_t)sl * d.gw + i] = 0u;
+    if (threadIdx.x == 0u) {
+        d.st_T[sl] = e[1]; d.st_clamp[sl] = e[2]; d.st_bias[sl] = e[3]; d.st_t[sl] = 0u;
+    }
+}
+
+// Serving: pack finished slots' results: final spikes, then one 64-bit
+// checksum per history row (the k_checksum formula over the row's words).
+// out + offs[i] receives ncol words followed by 2*T words.
+__global__ __launch_bounds__(256) void k_probe_gather(ProbeDev d, const uint32_t* __restrict__ slots,
+                                                      const uint32_t* __restrict__ offs, uint32_t n,
+                                                      uint32_t* __restrict__ out) {
+    __shared__ unsigned long long part[4];
+    if (blockIdx.x >= n) return;
+    const uint32_t sl = slots[blockIdx.x];
+    uint32_t* o = out + offs[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) o[i] = d.spk[(size_t)sl * d.ncol + i];
+    const uint32_t T = d.st_T[sl];
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint32_t* row = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
+        unsigned long long acc = 0;
+        for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) {
+            const uint32_t w = row[i];
+            acc += (unsigned long long)w * (2ull * i + 1ull) + hash32(w ^ i);
+        }
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane_id() == 0u) part[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0u) {
+            const unsigned long long v = part[0] + part[1] + part[2] + part[3];
+            o[d.ncol + 2u * t] = (uint32_t)v;
+            o[d.ncol + 2u * t + 1u] = (uint32_t)(v >> 32);
+        }
+        __syncthreads();
+    }
+}
+
 inline uint32_t blocks_for(size_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 }  // namespace
@@ -812,7 +1049,7 @@ hipError_t launch_fill(uint32_t* p, uint32_t value, size_t n, hipStream_t s) {
 
 hipError_t launch_clear(const NetDev& d, hipStream_t s) {
     const uint32_t n = std::max(d.ncol, d.nstats);
-    k_clear<<<blocks_for(n, 256), 256, 0, s>>>(d.spk[0], d.spk[1], d.ncol, d.stats, d.nstats);
+    k_clear<<<blocks_for(n, 256), 256, 0, s>>>(d.spk[0], d.spk[1], d.ncol, d.stats, d.nstats, d.colNZ, d.learnCnt);
     return hipGetLastError();
 }
 
@@ -823,15 +1060,27 @@ hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_
 }
 
 hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s) {
-    k_topk<<<1, 256, 0, s>>>(d, P);
+    k_topk<<<1, 1024, 0, s>>>(d, P);
     return hipGetLastError();
 }
 
-hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, hipStream_t s) {
+hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s) {
+    const uint32_t* in = d.spk[t & 1u];
+    const uint32_t* out = d.spk[(t & 1u) ^ 1u];
+    const uint32_t g = std::min<uint32_t>(d.ncol, 512u);  // grid-stride over the listed columns
+    if (stats) k_learn<true><<<g, 256, 0, s>>>(d, P, in, out);
+    else k_learn<false><<<g, 256, 0, s>>>(d, P, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s) {
     const uint32_t* in = d.spk[t & 1u];
     uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    if (learn) k_emit<true><<<d.ncol, 256, 0, s>>>(d, P, in, out);
-    else k_emit<false><<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, P, in, out);
+    const uint32_t g = blocks_for(d.ncol, 4);
+    if (learn && stats) k_emit<true, true><<<g, 256, 0, s>>>(d, P, in, out);
+    else if (learn) k_emit<true, false><<<g, 256, 0, s>>>(d, P, in, out);
+    else if (stats) k_emit<false, true><<<g, 256, 0, s>>>(d, P, in, out);
+    else k_emit<false, false><<<g, 256, 0, s>>>(d, P, in, out);
     return hipGetLastError();
 }
 
@@ -877,7 +1126,7 @@ hipError_t launch_kwta_test(const uint32_t* v, uint32_t ncols, uint32_t k, uint3
 
 hipError_t launch_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol, uint32_t incols, uint32_t target,
                             uint32_t par, uint32_t* theta, hipStream_t s) {
-    k_topk_test<<<ninst, 256, 0, s>>>(drive, theta, ncol, incols, target, par);
+    k_topk_test<<<ninst, 1024, 0, s>>>(drive, theta, ncol, incols, target, par);
     return hipGetLastError();
 }
 
@@ -908,4 +1157,58 @@ hipError_t launch_probe_reset(const ProbeDev& d, uint32_t first, uint32_t count,
     return hipGetLastError();
 }
 
+}  // namespace axon
+
+namespace axon {
+
+hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats, hipStream_t s) {
+    dim3 g((std::max(ncol, nstats) + 255u) / 256u, m.nnet);
+    k_mb_begin<<<g, 256, 0, s>>>(m);
+    return hipGetLastError();
+}
+hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
+    dim3 g((ncol + 3u) / 4u, m.nnet);
+    k_mb_drive<<<g, 256, 0, s>>>(m, P, t);
+    return hipGetLastError();
+}
+hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s) {
+    k_mb_topk<<<m.nnet, 1024, 0, s>>>(m, P);
+    return hipGetLastError();
+}
+hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
+                             hipStream_t s) {
+    if (learn) k_mb_emit<true><<<dim3((ncol + 3u) / 4u, m.nnet), 256, 0, s>>>(m, P, t);
+    else k_mb_emit<false><<<dim3((ncol + 3u) / 4u, m.nnet), 256, 0, s>>>(m, P, t);
+    return hipGetLastError();
+}
+hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
+    k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 128u), m.nnet), 256, 0, s>>>(m, P, t);
+    return hipGetLastError();
+}
+hipError_t launch_multi_end(const MultiDev& m, hipStream_t s) {
+    k_mb_end<<<1, 64, 0, s>>>(m);
+    return hipGetLastError();
+}
+
+}  // namespace axon
+
+namespace axon {
+hipError_t launch_checksum(const uint32_t* p, size_t n, unsigned long long* out, hipStream_t s) {
+    k_checksum<<<2048, 256, 0, s>>>(p, n, out);
+    return hipGetLastError();
+}
+}  // namespace axon
+
+namespace axon {
+hipError_t launch_probe_admit(const ProbeDev& d, const uint32_t* list, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_probe_admit<<<n, 256, 0, s>>>(d, list, n);
+    return hipGetLastError();
+}
+hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const uint32_t* offs, uint32_t n,
+                               uint32_t* out, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_probe_gather<<<n, 256, 0, s>>>(d, slots, offs, n, out);
+    return hipGetLastError();
+}
 }  // namespace axon

@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -115,6 +117,153 @@ def gpu_bench(args):
     }
 
 
+def train_alone(net):
+    c = net.config
+    for ep in range(c.epochs):
+        for p in range(c.stim):
+            net.present(p, 1.0, c.T, c.trainclamp, True, False, 1000 + p, outputs=False)
+    net.present(0, 1.0, c.T, c.trainclamp, True, False, 31337, outputs=False)
+    net.sync()
+
+
+def probe_set(net, axon_amd):
+    """60 stored patterns x {0, 10, 25, 50 %} of their 64 cue columns replaced by
+    other input columns, plus 16 random 64-column cues (256 probes)."""
+    cues = []
+    for r in (0, 6, 16, 32):
+        for p in range(net.config.stim):
+            cues.append(axon_amd.probe_cue(net, p, r, 100000 + 1000 * r + p))
+    for i in range(16):
+        cues.append(axon_amd.probe_cue(net, -1, 0, 200000 + i))
+    return np.stack(cues)
+
+
+def c3_probes(args):
+    import axon_amd
+
+    cfg = axon_amd.Config.parse(CONFIG_C + "stim=60 epochs=40 seed=0".split())
+    log("probes: training the load-60 network")
+    net = axon_amd.Net(cfg)
+    train_alone(net)
+    stims = probe_set(net, axon_amd)
+    T, clamp = 35, 5
+    log(f"probes: plain path, {len(stims)} probes one presentation each")
+    net.work_counters(reset=True)
+    t0 = time.perf_counter()
+    plain = [net.present(0, 1.0, T, clamp, False, True, 0, 0, stim=s).hist for s in stims]
+    t_plain = time.perf_counter() - t0
+    tiles_read, _ = net.work_counters(reset=True)
+    log("probes: autotuning the probe tile")
+    best, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
+    pb = axon_amd.Probes(net, len(stims), T, best)
+    pb.run(stims, T, clamp, 0, hist=False)  # warm (graph capture)
+    _, hist, ms = pb.run(stims, T, clamp, 0, hist=True)
+    exact = all(np.array_equal(hist[i], plain[i]) for i in range(len(stims)))
+    pb.close()
+    net.close()
+    syn = SYN_PER_TILE * tiles_read
+    return {"probes": len(stims), "T": T, "plain_gsyn_s": round(syn / t_plain / 1e9, 1),
+            "batched_gsyn_s": round(syn / (ms / 1e3) / 1e9, 1), "speedup": round(t_plain / (ms / 1e3), 2),
+            "plain_ms": round(1e3 * t_plain, 2), "batched_ms": round(ms, 3), "best_tile": best,
+            "tile_sweep_ms": {str(k): round(v, 3) for k, v in sweep.items()}, "bit_exact_vs_plain": bool(exact)}
+
+
+def c2_networks(args):
+    import axon_amd
+
+    loads, seeds = (12, 24, 36, 48, 60), range(5)
+    specs = [(P, sd) for P in loads for sd in seeds]
+    ep = args.net_epochs
+    mk = lambda P, sd: axon_amd.Config.parse(CONFIG_C + f"stim={P} epochs={ep} seed={sd}".split())  # noqa: E731
+    names = ("W0", "W1", "W2", "W3", "BIAS")
+    log(f"networks: plain path, {len(specs)} networks trained one after another")
+    ref, t_plain, tiles_plain = {}, 0.0, 0
+    for P, sd in specs:
+        net = axon_amd.Net(mk(P, sd))
+        t0 = time.perf_counter()
+        train_alone(net)
+        t_plain += time.perf_counter() - t0
+        ref[(P, sd)] = [axon_amd.checksum(net, nm) for nm in names]
+        tiles_plain += net.work_counters()[0]
+        net.close()
+    log("networks: batched, all networks in shared launches")
+    nets = [axon_amd.Net(mk(P, sd)) for P, sd in specs]
+    mb = axon_amd.MultiNet(nets)
+    ms = mb.train([ep] * len(nets), extra=True)
+    exact = all([axon_amd.checksum(n, nm) for nm in names] == ref[spec] for n, spec in zip(nets, specs))
+    tiles = sum(n.work_counters()[0] for n in nets)
+    learn = sum(n.work_counters()[1] for n in nets)
+    mb.close()
+    for n in nets:
+        n.close()
+    return {"networks": len(specs), "epochs": ep, "presentations": sum(ep * P + 1 for P, _ in specs),
+            "plain_s": round(t_plain, 3), "batched_s": round(ms / 1e3, 3), "speedup": round(t_plain / (ms / 1e3), 2),
+            "plain_gsyn_s": round(SYN_PER_TILE * tiles_plain / t_plain / 1e9, 1),
+            "batched_gsyn_s": round(SYN_PER_TILE * tiles / (ms / 1e3) / 1e9, 1),
+            "batched_plasticity_gsyn_s": round(SYN_PER_TILE * learn / (ms / 1e3) / 1e9, 1),
+            "bit_exact_vs_plain": bool(exact and tiles == tiles_plain)}
+
+
+def c4_serving(args):
+    import axon_amd
+
+    loads = (12, 36, 60, 100, 140, 170, 200, 200)
+    log("serving: training 8 resident networks together")
+    nets = [axon_amd.Net(axon_amd.Config.parse(CONFIG_C + f"stim={P} epochs=40 seed={i} maxt=200".split()))
+            for i, P in enumerate(loads)]
+    mb = axon_amd.MultiNet(nets)
+    mb.train([40] * len(nets), extra=True)
+    mb.close()
+    out = {"networks": len(nets), "loads": list(loads), "T_range": [10, 200], "rates": []}
+    for rate in args.serve_rates:
+        n_q = max(200, int(rate * args.serve_seconds))
+        rs = np.random.RandomState(rate)  # MT19937 stream
+        gaps = rs.exponential(1.0 / rate, n_q)
+        arrivals = np.cumsum(gaps)
+        qs = []
+        for i in range(n_q):
+            k = int(rs.randint(0, len(nets)))
+            if rs.randint(0, 16) == 0:
+                cue = axon_amd.probe_cue(nets[k], -1, 0, 300000 + i)
+            else:
+                cue = axon_amd.probe_cue(nets[k], int(rs.randint(0, loads[k])), (0, 6, 16, 32)[rs.randint(0, 4)],
+                                         300000 + i)
+            qs.append((k, cue, int(rs.randint(10, 201))))
+        srv = axon_amd.Server(nets, slots_per_net=args.serve_slots, max_t=200, tile=args.serve_tile,
+                              steps_per_tick=args.serve_steps)
+        log(f"serving: {n_q} queries at {rate}/s")
+        start = srv.clock()
+        ids, lat, res, nxt, last_done = {}, [], {}, 0, start
+        while len(lat) < n_q:
+            now = srv.clock()
+            while nxt < n_q and start + arrivals[nxt] <= now:
+                k, cue, T = qs[nxt]
+                ids[srv.submit(k, cue, T, 5, 0, start + arrivals[nxt])] = nxt
+                nxt += 1
+            if srv.tick():
+                for rid in srv.completed():
+                    r = srv.result(rid, 200)
+                    i = ids.pop(rid)
+                    lat.append(r[2])
+                    if i % max(1, n_q // 24) == 0:
+                        res[i] = r
+                last_done = srv.clock()
+        exact = True
+        for i, r in res.items():
+            k, cue, T = qs[i]
+            h = nets[k].present(0, 1.0, T, 5, False, True, 0, 0, stim=cue)
+            exact &= bool(np.array_equal(r[0], h.spikes) and np.array_equal(r[1], axon_amd.row_digest(h.hist)))
+        srv.close()
+        lat_ms = np.array(lat) * 1e3
+        out["rates"].append({"rate_qps": rate, "queries": n_q, "qps": round(n_q / (last_done - start), 1),
+                             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                             "checked_bit_exact": len(res), "bit_exact_vs_alone": exact})
+    for n in nets:
+        n.close()
+    return out
+
+
 def cpu_bench(args):
     """The oracle (C++ restatement of the reference kernels), column-parallel on
     the host cores, on the first presentations of the same training run."""
@@ -151,6 +300,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="c1,probes,networks,serving",
+                    help="comma list of c1, probes, networks, serving")
+    ap.add_argument("--net-epochs", type=int, default=40)
+    ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-slots", type=int, default=128)
+    ap.add_argument("--serve-tile", type=int, default=16)
+    ap.add_argument("--serve-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (every network, probe batch "
@@ -158,7 +315,14 @@ def main():
         sys.exit(2)
     if os.environ.get("RANK", "0") != "0":
         return
+    sections = set(args.sections.split(","))
     g = gpu_bench(args)
+    batched = {}
+    if "probes" in sections:
+        batched["probes"] = c3_probes(args)
+    if "networks" in sections:
+        batched["networks"] = c2_networks(args)
+    serving = c4_serving(args) if "serving" in sections else None
     cpu = None if args.no_cpu else cpu_bench(args)
     out = {
         "metric": METRIC,
@@ -182,6 +346,8 @@ def main():
         "kernels": {k: {kk: round(vv, 3) if isinstance(vv, float) else vv for kk, vv in v.items()}
                     for k, v in g["kernels"].items()},
         "roofline": g["roofline"],
+        "batched_vs_plain": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

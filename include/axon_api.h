@@ -168,10 +168,65 @@ axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t count, uin
  * hist is count*T*NCOL (packed), ms the device wall time of the steps */
 axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
                             uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes, double* ms);
+/* autotuner: runs this probe batch with every tile size (8/16/32/64), times the
+ * device work, returns the fastest tile and the per-tile milliseconds */
+axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile, double ms_per_tile[4]);
 /* recall cue: pattern's stimulus columns with `nreplace` of them replaced by
  * other input columns (mt19937(seed) draws); pattern 0xFFFFFFFF = a random cue
  * of stimcols input columns.  Extends loadStim (mm:800-804). */
 axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint32_t nreplace, uint32_t seed, uint32_t* stim);
+
+/* ---- many networks trained together ----
+ * Batching extension with no reference counterpart: independent networks
+ * (different seeds and numbers of stored patterns, identical dynamics
+ * parameters) advance in the same launches.  Each network follows exactly the
+ * reference's training protocol -- epochs x stim learning presentations
+ * (pattern k mod stim, keep 1.0, mm:1009-1010) and, with extra = 1, the
+ * additional presentation of pattern 0 (mm:1033) -- so its state is
+ * bit-identical to training it alone.  Uneven lengths are handled by
+ * per-network work queues (a device schedule table). */
+typedef struct axon_multi axon_multi;
+axon_status axon_multi_create(axon_net* const* nets, uint32_t nnet, axon_multi** out);
+void axon_multi_destroy(axon_multi* b);
+axon_status axon_multi_train(axon_multi* b, const uint32_t* epochs /* nnet */, uint32_t extra, double* ms);
+/* order-independent 64-bit checksum of an engine buffer (state comparison) */
+axon_status axon_checksum(axon_net* n, axon_buffer which, uint64_t* sum);
+
+/* ---- serving: continuous batching of recall queries ----
+ * Extension with no reference counterpart.  Trained networks stay resident;
+ * each query is one readout presentation (a probe) of T steps on one network.
+ * axon_server_tick admits queued queries FIFO into free slots while the sum of
+ * in-flight footprints stays within hbm_budget (0 = unlimited), advances every
+ * in-flight query by steps_per_tick steps (networks on separate streams, so
+ * their work overlaps), and collects the finished ones.  A result is the final
+ * spike ballots plus one 64-bit checksum per step's ballot row (the
+ * axon_checksum formula over that row), bit-identical to the same query run
+ * alone.  Latency is completion time minus the arrival time given at submit
+ * (seconds on axon_server_clock; negative = now). */
+typedef struct axon_server axon_server;
+typedef struct axon_server_config {
+    uint32_t slots_per_net; /* probe slots per network */
+    uint32_t max_t;         /* longest query, steps */
+    uint32_t tile;          /* probe tile: 8, 16, 32 or 64 */
+    uint32_t steps_per_tick;
+    uint64_t hbm_budget;    /* bytes of in-flight query state, 0 = no limit */
+    uint32_t reserved[4];
+} axon_server_config;
+axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
+                               axon_server** out);
+void axon_server_destroy(axon_server* s);
+double axon_server_clock(const axon_server* s);
+axon_status axon_server_submit(axon_server* s, uint32_t net, const uint32_t* stim, uint32_t T, uint32_t clamp_steps,
+                               uint32_t use_bias, double arrival_s, uint64_t* id);
+axon_status axon_server_tick(axon_server* s, uint32_t* completed);
+/* ids of the queries the last tick finished (up to max; *n = how many finished) */
+axon_status axon_server_completed(const axon_server* s, uint64_t* ids, uint32_t max, uint32_t* n);
+/* spikes NCOL words, row_digest T words; AXON_ERR_NOT_READY until finished */
+axon_status axon_server_result(axon_server* s, uint64_t id, uint32_t* spikes, uint64_t* row_digest,
+                               double* latency_s, uint32_t* T);
+axon_status axon_server_status(const axon_server* s, uint32_t* queued, uint32_t* inflight,
+                               uint64_t* reserved_bytes);
 
 /* ---- kernel test harnesses (replace the vbs/vtopk self-checks mm:903-994) ---- */
 /* k_bstest mm:424-434: op 0 inc, 1 dec, 2 clip[lo,hi]; nwords u32 per plane, in place */
