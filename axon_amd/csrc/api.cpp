@@ -120,8 +120,8 @@ struct axon_net {
     // per-kernel event timing (axon_net_set_profiling)
     bool profiling = false;
     std::vector<hipEvent_t> prof_ev;  // 2 per kernel per step
-    double prof_ms[3] = {0, 0, 0};
-    uint64_t prof_n[3] = {0, 0, 0};
+    double prof_ms[4] = {0, 0, 0, 0};
+    uint64_t prof_n[4] = {0, 0, 0, 0};
     std::vector<void*> allocs;
 };
 
@@ -282,7 +282,8 @@ extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
         {&d.stim, n->NCOL}, {&d.stats, n->NSTATS}, {&d.hist, (size_t)n->MAXT * n->NCOL}, {&d.driveLog, n->NDLOG},
         {&d.refBall, n->NCOL}, {&d.mult, n->N}, {&n->d_small, 64}, {&n->d_ball, n->NCOL}, {&n->d_mask, n->N},
         {&d.cntProp, n->NCOL}, {&d.cntLearn, (size_t)n->NCOL * 4},
-        {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4}, {&d.learnList, (size_t)n->NCOL * 2}, {&d.learnCnt, 2},
+        {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4}, {&d.learnList, (size_t)n->NCOL * 2},
+        {(uint32_t**)&d.tileList, (size_t)n->NCOL * axon::NSLOT * 2 * 4}, {&d.learnCnt, 2},
     };
     for (auto& b : bufs) {
         if ((st = dev_alloc(n, &p, b.words * 4)) != AXON_OK) return cleanup(st);
@@ -349,14 +350,16 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
 
 // Same launches as enqueue_steps, each bracketed by events so the three
 // kernels' device durations can be summed after the presentation.
-static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn) {
-    const size_t need = (size_t)6 * std::max<uint32_t>(T, 1);
+static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn,
+                                         bool stats) {
+    const size_t need = (size_t)8 * std::max<uint32_t>(T, 1);
     while (n->prof_ev.size() < need) {
         hipEvent_t e;
         hipError_t r = hipEventCreate(&e);
         if (r != hipSuccess) return r;
         n->prof_ev.push_back(e);
     }
+    std::vector<uint8_t> plast(T, 0);  // steps that launch the plasticity kernel
     hipError_t e = axon::launch_clear(n->d, n->stream);
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
         P.step = t;
@@ -367,7 +370,8 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         P.phase = (!learn || !n->contrast) ? 0u
                   : (t < clampSteps)      ? 1u
                   : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
-        hipEvent_t* ev = &n->prof_ev[(size_t)6 * t];
+        hipEvent_t* ev = &n->prof_ev[(size_t)8 * t];
+        plast[t] = learn && P.phase != 3u;
         if ((e = hipEventRecord(ev[0], n->stream)) != hipSuccess) break;
         if ((e = axon::launch_drive(n->d, P, t, n->stream)) != hipSuccess) break;
         if ((e = hipEventRecord(ev[1], n->stream)) != hipSuccess) break;
@@ -375,20 +379,24 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         if ((e = axon::launch_topk(n->d, P, n->stream)) != hipSuccess) break;
         if ((e = hipEventRecord(ev[3], n->stream)) != hipSuccess) break;
         if ((e = hipEventRecord(ev[4], n->stream)) != hipSuccess) break;
-        if ((e = axon::launch_emit(n->d, P, t, learn, true, n->stream)) != hipSuccess) break;
-        if (learn && P.phase != 3u && (e = axon::launch_learn(n->d, P, t, true, n->stream)) != hipSuccess) break;
-        e = hipEventRecord(ev[5], n->stream);
+        if ((e = axon::launch_emit(n->d, P, t, learn, stats, n->stream)) != hipSuccess) break;
+        if ((e = hipEventRecord(ev[5], n->stream)) != hipSuccess) break;
+        if (!plast[t]) continue;
+        if ((e = hipEventRecord(ev[6], n->stream)) != hipSuccess) break;
+        if ((e = axon::launch_learn(n->d, P, t, stats, n->stream)) != hipSuccess) break;
+        e = hipEventRecord(ev[7], n->stream);
     }
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
-    for (uint32_t t = 0; t < T; ++t)
-        for (int k = 0; k < 3; ++k) {
+    for (uint32_t t = 0; t < T; ++t) {
+        for (int k = 0; k < (plast[t] ? 4 : 3); ++k) {
             float ms = 0.f;
-            if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)6 * t + 2 * k], n->prof_ev[(size_t)6 * t + 2 * k + 1])) != hipSuccess)
+            if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)8 * t + 2 * k], n->prof_ev[(size_t)8 * t + 2 * k + 1])) != hipSuccess)
                 return e;
             n->prof_ms[k] += ms;
             n->prof_n[k] += 1;
         }
+    }
     return hipSuccess;
 }
 
@@ -428,8 +436,8 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
 
     const auto t0 = std::chrono::steady_clock::now();
     if (n->profiling) {
-        if (async) return fail(AXON_ERR_INVALID_ARG, "profiling presentations are synchronous");
-        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn);
+        // waits for the device even when async (the event times are read back)
+        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn, !async);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "profiled launch: %s", hipGetErrorString(e));
     } else if (n->use_graphs && a->T > 0) {
         const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias, async ? 0u : 1u};
@@ -583,9 +591,9 @@ extern "C" axon_status axon_net_set_profiling(axon_net* n, uint32_t enable) {
     return AXON_OK;
 }
 
-extern "C" axon_status axon_kernel_times(axon_net* n, double ms[3], uint64_t launches[3], uint32_t reset) {
+extern "C" axon_status axon_kernel_times(axon_net* n, double ms[4], uint64_t launches[4], uint32_t reset) {
     if (!n || !ms || !launches) return fail(AXON_ERR_INVALID_ARG, "null argument");
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         ms[k] = n->prof_ms[k];
         launches[k] = n->prof_n[k];
         if (reset) { n->prof_ms[k] = 0; n->prof_n[k] = 0; }

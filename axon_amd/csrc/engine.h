@@ -41,13 +41,17 @@ struct NetDev {
     uint32_t* refBall;
     uint32_t* mult;
     // Work counters (no reference counterpart): per column, the number of
-    // (column, branch) tiles propagation read, and per (column, wave) the
-    // number of tiles plasticity rewrote; accumulated across steps.
+    // (column, branch) tiles propagation read, and the number of tiles
+    // plasticity rewrote (entry 4 * column); accumulated across steps.
     uint32_t* cntProp;   // ncol
     uint32_t* cntLearn;  // ncol * 4
     uint8_t* colNZ;      // ncol (+pad to 16): column spiked in the last step
-    uint32_t* learnList; // [2][ncol]: columns learning at step t, list t & 1
-    uint32_t* learnCnt;  // [2] list lengths (reset by the step after next's propagation)
+    // Plasticity work of step t, in list t & 1: the learning columns when side
+    // statistics are on (learnList), else the voted tiles (tileList entries
+    // {column * 32 + slot, presynaptic group, learning-neuron mask, 0}).
+    uint32_t* learnList; // [2][ncol]
+    uint4* tileList;     // [2][ncol * 32]
+    uint32_t* learnCnt;  // [2] entries in each list (reset by the step after next's propagation)
     uint32_t ncol, incols, inbr, topkpar;
     uint32_t nstats, ndlog, maxt, pad;
 };

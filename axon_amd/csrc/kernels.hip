@@ -442,6 +442,48 @@ __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__
 // propagation) and whether it learns this step (read by k_learn).
 // STATS = false drops the side outputs (stats and drive-log atomics, LTP/LTD
 // counters); spikes, history, margin, bias and synapses are unaffected.
+// Record a learning column's plasticity work for k_learn.  Branch slot s of
+// the column is voted (mm:446-452) when some spiking neuron has its slot-s
+// NMDA bit set; only voted tiles are rewritten.  Without side statistics each
+// voted tile becomes one list entry, carrying its presynaptic group and the
+// mask of neurons whose slot-s branch learns, so that k_learn spends one wave
+// per tile with no further lookups.  With statistics the column itself is
+// listed (the per-neuron LTP/LTD totals need all of a column's tiles in one
+// workgroup).
+template <bool STATS>
+__device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P, uint32_t col, bool post,
+                                                uint32_t n) {
+    const uint32_t lane = lane_id();
+    const uint32_t par = P.step & 1u;
+    const uint32_t bbv = post ? d.bbits[col * NPC + n] : 0u;
+    if (STATS) {
+        const uint32_t voted = half_or(bbv);
+        if (lane == 0u) {
+            d.cntLearn[col * 4u] += popc(voted);
+            d.learnList[par * d.ncol + atomicAdd(&d.learnCnt[par], 1u)] = col;
+        }
+        return;
+    }
+    uint32_t want = 0u;  // lane s < 32: the neurons whose slot-s branch learns
+#pragma unroll
+    for (uint32_t s = 0; s < 32u; ++s) {
+        const uint32_t m = ballot32(((bbv >> s) & 1u) != 0u);
+        if (lane == s) want = m;
+    }
+    const uint32_t voted = ballot32(want != 0u);
+    uint32_t base = 0u;
+    if (lane == 0u) {
+        base = atomicAdd(&d.learnCnt[par], popc(voted));
+        d.cntLearn[col * 4u] += popc(voted);
+    }
+    base = __shfl(base, 0);
+    if (want != 0u) {
+        const uint32_t at = base + popc(voted & ((1u << lane) - 1u));
+        d.tileList[(size_t)par * d.ncol * NSLOT + at] =
+            make_uint4(col * NSLOT + lane, d.conn[col * NSLOT + lane], want, 0u);
+    }
+}
+
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t* __restrict__ spkOut, uint32_t col0) {
@@ -497,12 +539,9 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
                 spkOut[col] = b;
                 d.marg[col] = (vk > vk1) ? 1u : 0u;
                 d.colNZ[col] = b != 0u ? 1u : 0u;
-                if (LEARN && learnCol) {  // compact list of this step's learning columns
-                    const uint32_t par = P.step & 1u;
-                    d.learnList[par * d.ncol + atomicAdd(&d.learnCnt[par], 1u)] = col;
-                }
                 if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
             }
+            if (LEARN && learnCol) list_plasticity<STATS>(d, P, col, post, n);
             if (LEARN && !hi && rec) {  // k_bias
                 const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
                 const int nb = (int)d.bias[gid] + (post ? (int)P.biasUp : -dec);
@@ -535,7 +574,6 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
             const uint32_t s = wave * 8u + i;
             if (ballot32(post && ((bb >> s) & 1u) != 0u)) voted |= 1u << i;
         }
-        if (lane == 0u) d.cntLearn[col * 4u + wave] += popc(voted);
         uint32_t nLTP = 0, nLTD = 0;
         const size_t tileBase = (size_t)col * NSLOT * TILE_U4 + lane;
         while (voted) {
@@ -592,6 +630,40 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
             __syncthreads();
         }
     }
+}
+
+// k_learn (mm:436-495) over the voted-tile list: wave w of the grid rewrites
+// tiles w, w + nwaves, ... (four bit-planes read, updated, written back).
+__device__ __forceinline__ void learn_tiles(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                            uint32_t w0, uint32_t nwaves) {
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const bool hi = lane >= 32u;
+    const uint32_t par = P.step & 1u;
+    const uint32_t cnt = d.learnCnt[par];
+    const uint4* __restrict__ list = d.tileList + (size_t)par * d.ncol * NSLOT;
+    for (uint32_t i = w0; i < cnt; i += nwaves) {
+        const uint4 e = list[i];
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(e.x);
+        const uint32_t g = __builtin_amdgcn_readfirstlane(e.y);
+        const uint32_t wm = __builtin_amdgcn_readfirstlane(e.z);
+        const size_t idx = (size_t)tile * TILE_U4 + lane;
+        uint4 a0 = d.W0[idx], a1 = d.W1[idx], a2 = d.W2[idx], a3 = d.W3[idx];
+        const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
+        const uint4 lo4 = s4[0], hi4 = s4[1];
+        const uint4 sp = pick4(hi, hi4, lo4);
+        const bool want = ((wm >> n) & 1u) != 0u;
+        uint32_t nLTP = 0, nLTD = 0;
+        learn_word(a0.x, a1.x, a2.x, a3.x, sp.x, want, P, nLTP, nLTD);
+        learn_word(a0.y, a1.y, a2.y, a3.y, sp.y, want, P, nLTP, nLTD);
+        learn_word(a0.z, a1.z, a2.z, a3.z, sp.z, want, P, nLTP, nLTD);
+        learn_word(a0.w, a1.w, a2.w, a3.w, sp.w, want, P, nLTP, nLTD);
+        d.W0[idx] = a0; d.W1[idx] = a1; d.W2[idx] = a2; d.W3[idx] = a3;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_learn_tiles(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    learn_tiles(d, P, spkIn, blockIdx.x * 4u + wave, gridDim.x * 4u);
 }
 
 template <bool LEARN, bool STATS>
@@ -959,7 +1031,8 @@ __global__ __launch_bounds__(256) void k_mb_learn(MultiDev m, Params P, uint32_t
     const NetDev d = m.nets[net];
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    learn_cols<false>(d, P, (t & 1u) ? s1 : s0, (t & 1u) ? s0 : s1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    learn_tiles(d, P, (t & 1u) ? s1 : s0, blockIdx.x * 4u + wave, gridDim.x * 4u);
 }
 
 __global__ void k_mb_end(MultiDev m) {
@@ -1067,9 +1140,9 @@ hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s) {
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    const uint32_t g = std::min<uint32_t>(d.ncol, 512u);  // grid-stride over the listed columns
+    const uint32_t g = std::min<uint32_t>(d.ncol, 512u);  // grid-stride over the listed columns / tiles
     if (stats) k_learn<true><<<g, 256, 0, s>>>(d, P, in, out);
-    else k_learn<false><<<g, 256, 0, s>>>(d, P, in, out);
+    else k_learn_tiles<<<g, 256, 0, s>>>(d, P, in);
     return hipGetLastError();
 }
 
@@ -1182,7 +1255,7 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
     return hipGetLastError();
 }
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
-    k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 128u), m.nnet), 256, 0, s>>>(m, P, t);
+    k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 256u), m.nnet), 256, 0, s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s) {

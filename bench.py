@@ -50,14 +50,16 @@ def kernel_bytes(ncol, steps, prop_tiles, learn_tiles):
     propagation: 1 KiB of W3 per tile read + conn (128 B/column) + bias read
     (4 B/neuron, learning uses the bias) + acc and bbits writes (8 B/neuron)
     + drive (4 B/column).
-    emit+plasticity: 8 KiB per rewritten tile (4 planes read + written) +
-    acc read (4 B/neuron) + bias read-modify-write (8 B/neuron) + drive read,
-    spike, history and margin writes (16 B/column)."""
+    topk: three passes over the column drives (4 B/column each).
+    emit: acc read (4 B/neuron) + bias read-modify-write (8 B/neuron) + drive
+    read, spike, history and margin writes (16 B/column).
+    plasticity: 8 KiB per rewritten tile (4 planes read + written)."""
     n = ncol * 32
     prop = 1024 * prop_tiles + steps * (128 * ncol + 4 * n + 8 * n + 4 * ncol)
-    emit = 8192 * learn_tiles + steps * (4 * n + 8 * n + 16 * ncol)
     topk = steps * 3 * 4 * ncol
-    return {"propagation": prop, "topk": topk, "emit_plasticity": emit}
+    emit = steps * (4 * n + 8 * n + 16 * ncol)
+    plast = 8192 * learn_tiles
+    return {"propagation": prop, "topk": topk, "emit": emit, "plasticity": plast}
 
 
 def gpu_bench(args):
@@ -85,21 +87,21 @@ def gpu_bench(args):
     twin.sync()
     twin.work_counters(reset=True)
     twin.set_profiling(True)
-    train_presentations(twin, args.warmup, args.steps, T, tc, nstim, outputs=True)
+    train_presentations(twin, args.warmup, args.steps, T, tc, nstim)
     ms, launches = twin.kernel_times(reset=True)
     prop2, learn2 = twin.work_counters(reset=True)
     twin.close()
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
     kb = kernel_bytes(ncol, steps, prop, learn)
-    names = ["propagation", "topk", "emit_plasticity"]
+    names = ["propagation", "topk", "emit", "plasticity"]
     kern = {}
     for i, nm in enumerate(names):
         t_s = ms[i] / 1e3
         kern[nm] = {"launches": launches[i], "avg_us": 1e3 * ms[i] / max(launches[i], 1),
                     "bytes_per_launch": kb[nm] / max(launches[i], 1),
                     "GBps": kb[nm] / t_s / 1e9 if t_s > 0 else 0.0}
-    dom = max(range(3), key=lambda i: ms[i])
+    dom = max(range(len(names)), key=lambda i: ms[i])
     dn = names[dom]
     achieved = kern[dn]["GBps"]
     return {
@@ -292,6 +294,23 @@ def cpu_bench(args):
             "timesteps_per_s": round(done * T / dt, 2)}
 
 
+def copy_calibration(args):
+    """Plain device-to-device copies of a known size (256 MiB read + 256 MiB
+    written per launch): the yardstick that scales the FETCH_SIZE / WRITE_SIZE
+    counters into bytes for the traffic figure."""
+    import torch
+
+    nbytes = 256 << 20
+    src = torch.randint(0, 2 ** 31 - 1, (nbytes // 4,), dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst, src))
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                                           "launches": args.steps, "ok": ok}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,7 +320,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="c1,probes,networks,serving",
-                    help="comma list of c1, probes, networks, serving")
+                    help="comma list of c1, probes, networks, serving (copy: counter calibration only)")
     ap.add_argument("--net-epochs", type=int, default=40)
     ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
@@ -316,6 +335,9 @@ def main():
     if os.environ.get("RANK", "0") != "0":
         return
     sections = set(args.sections.split(","))
+    if "copy" in sections:
+        copy_calibration(args)
+        return
     g = gpu_bench(args)
     batched = {}
     if "probes" in sections:

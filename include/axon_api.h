@@ -127,11 +127,13 @@ axon_status axon_stim_cols(const axon_net* n, uint32_t p, uint32_t* out);
  * by plasticity, accumulated since creation or the last reset.  Each tile is
  * 8192 synapses (32 neurons x 256 presynaptic bits). */
 axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset);
-/* enable = 1: presentations run eagerly with HIP events around every kernel;
+/* enable = 1: presentations run eagerly with HIP events around every kernel
+ * and wait for the device before returning (an AXON_PRESENT_ASYNC
+ * presentation still skips its side statistics and outputs);
  * axon_kernel_times returns summed device ms and launch counts for
- * [0] propagation, [1] column top-K, [2] emit (+bias +plasticity). */
+ * [0] propagation, [1] column top-K, [2] emit (k-WTA, bias), [3] plasticity. */
 axon_status axon_net_set_profiling(axon_net* n, uint32_t enable);
-axon_status axon_kernel_times(axon_net* n, double ms[3], uint64_t launches[3], uint32_t reset);
+axon_status axon_kernel_times(axon_net* n, double ms[4], uint64_t launches[4], uint32_t reset);
 
 /* ---- readout kernels ---- */
 axon_status axon_wstat(axon_net* n, uint32_t* functional);  /* k_wstat mm:641-646, wstat() mm:784-791 */
