@@ -56,11 +56,14 @@ __device__ __forceinline__ uint32_t half_or(uint32_t x) {
     return x;
 }
 
-// Per-lane choice between two wave-uniform uint4 values (component-wise
-// selects, so the compiler keeps everything in registers).
-__device__ __forceinline__ uint4 pick4(bool hi, const uint4& h, const uint4& l) {
-    return make_uint4(hi ? h.x : l.x, hi ? h.y : l.y, hi ? h.z : l.z, hi ? h.w : l.w);
+// The 8 spike words of presynaptic group g (columns 8g..8g+7) as this lane's
+// half: words 0-3 in lanes 0-31, words 4-7 in lanes 32-63.  A vector load,
+// so a wave keeps many of them in flight (scalar loads of the same words are
+// waited for one at a time).
+__device__ __forceinline__ uint4 group_words(const uint32_t* __restrict__ spk, uint32_t g, bool hi) {
+    return reinterpret_cast<const uint4*>(spk)[2u * g + (hi ? 1u : 0u)];
 }
+
 
 // ---- exact k-of-32 winners-take-all with margin (kwta_ballot_margin, mm:134-144) ----
 // Lanes 0..31 hold the column's 32 values (lanes 32..63 duplicate them).
@@ -239,7 +242,7 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
         const uint32_t anyh = sp.x | sp.y | sp.z | sp.w;
         mask = ballot32((anyh | __shfl_xor(anyh, 32)) != 0u);
     }
-    if (lane == 0u) d.cntProp[col] += popc(mask);
+    const uint32_t nActive = popc(mask);
 
     const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
     uint32_t acc = 0u, bb = 0u;
@@ -258,10 +261,7 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
                 mask &= mask - 1u;
                 sidx[k] = sk;
                 w[k] = Wc[(size_t)sk * TILE_U4];
-                const uint32_t g = __builtin_amdgcn_readlane(gl, sk);
-                const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
-                const uint4 lo4 = s4[0], hi4 = s4[1];
-                sv[k] = pick4(hi, hi4, lo4);
+                sv[k] = group_words(spkIn, __builtin_amdgcn_readlane(gl, sk), hi);
             }
         }
 #pragma unroll
@@ -284,7 +284,10 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     acc = (acc > bi) ? (acc - bi) : 0u;
     if (!hi) { d.acc[gid] = acc; d.bbits[gid] = bb; }
     const uint32_t drv = half_sum(acc);
-    if (lane == 0u) d.drive[col] = drv;
+    if (lane == 0u) {
+        d.drive[col] = drv;
+        atomicAdd(&d.cntProp[col], nActive);  // last: a memory wait after it would include it
+    }
 }
 
 constexpr uint32_t FIRED_LDS_COLS = 16384;  // columns whose fired bytes fit the LDS stage
@@ -312,68 +315,62 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
 // ============================================================== column top-K
 // theta = target-th largest drive over recurrent columns by three byte-wise
 // histogram passes (k_topk_par mm:301-352, k_topk_serial mm:259-292).  One
-// workgroup per instance.  The first four values of every thread are loaded
-// once and kept in registers for all three passes; histogram bins are plain
-// LDS atomics; the 256-bin suffix scan and the crossing bin are done by wave 0
-// in registers.
+// workgroup of NT threads per instance.  Up to 4096 values are loaded once
+// into registers (4096 / NT per thread) and kept for every pass; further
+// values are re-read per pass.  Histogram bins are direct LDS atomics into two
+// alternating buffers (each pass clears the buffer the next pass fills, so no
+// pass needs a barrier of its own for clearing); the 256-bin suffix scan and
+// the crossing bin are computed by wave 0 in registers.
 // Returns max(theta, 1) to every thread of the block; thread 0 also stores
 // (theta, need) to `theta` when it is not null.
+template <uint32_t NT>
 __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __restrict__ theta, uint32_t ncol,
                                uint32_t inCols, uint32_t target, uint32_t par) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t sh_need, sh_prefix;
-    constexpr int VPT = 4;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), nth = blockDim.x;
+    constexpr uint32_t VPT = 4096u / NT;
+    __shared__ uint32_t hist[2][256];
+    __shared__ uint32_t sh_need, sh_prefix, sh_or;
+    const uint32_t tid = threadIdx.x, lane = lane_id();
     uint32_t rv[VPT];
 #pragma unroll
-    for (int i = 0; i < VPT; ++i) {
-        const uint32_t c = inCols + tid + (uint32_t)i * nth;
+    for (uint32_t i = 0; i < VPT; ++i) {
+        const uint32_t c = inCols + tid + i * NT;
         rv[i] = c < ncol ? drive[c] : 0u;
     }
-    const uint32_t cRest = inCols + (uint32_t)VPT * nth;  // values beyond the register cache
-    __shared__ uint32_t sh_or;
+    const uint32_t cRest = inCols + VPT * NT;  // values beyond the register cache
+    for (uint32_t i = tid; i < 256u; i += NT) hist[0][i] = 0u;
     if (tid == 0) { sh_need = target; sh_prefix = 0u; sh_or = 0u; }
     // A leading pass whose byte is zero for every value puts them all in bin 0
     // and leaves (prefix, need) unchanged under both selection rules (except
     // the serial rule with need == 0, which picks bin 255), so it is skipped.
     uint32_t orv = 0;
 #pragma unroll
-    for (int i = 0; i < VPT; ++i) orv |= rv[i];
-    for (uint32_t c = cRest + tid; c < ncol; c += nth) orv |= drive[c];
-    for (uint32_t off = 32; off >= 1u; off >>= 1) orv |= __shfl_xor(orv, off);
+    for (uint32_t i = 0; i < VPT; ++i) orv |= rv[i];
+    for (uint32_t c = cRest + tid; c < ncol; c += NT) orv |= drive[c];
+    orv = __reduce_or_sync(~0ull, orv);
     __syncthreads();
     if (lane == 0u) atomicOr(&sh_or, orv);
     __syncthreads();
     int byStart = 2;
     if (par || target > 0u) byStart = (sh_or >> 16) ? 2 : (sh_or >> 8) ? 1 : 0;
-    for (int by = byStart; by >= 0; --by) {
-        if (tid < 256u) hist[tid] = 0u;
-        __syncthreads();
+    uint32_t hb = 0;
+    for (int by = byStart; by >= 0; --by, hb ^= 1u) {
         const uint32_t need = sh_need, prefix = sh_prefix;
         const uint32_t shHi = (uint32_t)(by + 1) * 8u, shLo = (uint32_t)by * 8u;
-        // one LDS atomic per distinct bin per wave (drives cluster in few bins)
+        uint32_t* __restrict__ h = hist[hb];
+        // the other buffer was last read before the previous barrier: clear it for the next pass
+        for (uint32_t i = tid; i < 256u; i += NT) hist[hb ^ 1u][i] = 0u;
 #pragma unroll
-        for (int i = 0; i < VPT; ++i) {
-            const uint32_t c = inCols + tid + (uint32_t)i * nth;
-            const bool act = c < ncol && (rv[i] >> shHi) == (prefix >> shHi);
-            const uint32_t bin = (rv[i] >> shLo) & 255u;
-            uint64_t m = __ballot(act);
-            while (m) {
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                const uint32_t lb = __builtin_amdgcn_readlane(bin, leader);
-                const uint64_t same = __ballot(act && bin == lb);
-                if (lane == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(same));
-                m &= ~same;
-            }
+        for (uint32_t i = 0; i < VPT; ++i) {
+            const uint32_t c = inCols + tid + i * NT;
+            if (c < ncol && (rv[i] >> shHi) == (prefix >> shHi)) atomicAdd(&h[(rv[i] >> shLo) & 255u], 1u);
         }
-        for (uint32_t c = cRest + tid; c < ncol; c += nth) {
+        for (uint32_t c = cRest + tid; c < ncol; c += NT) {
             const uint32_t dv = drive[c];
-            if ((dv >> shHi) == (prefix >> shHi)) atomicAdd(&hist[(dv >> shLo) & 255u], 1u);
+            if ((dv >> shHi) == (prefix >> shHi)) atomicAdd(&h[(dv >> shLo) & 255u], 1u);
         }
         __syncthreads();
         if (tid < 64u) {
-            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
-                           h3 = hist[4 * lane + 3];
+            const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2], h3 = h[4 * lane + 3];
             const uint32_t loc = h0 + h1 + h2 + h3;
             uint32_t S = loc;  // inclusive suffix sum over lanes
 #pragma unroll
@@ -389,8 +386,7 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
             if (sc1 >= need) best = 4 * (int)lane + 1;
             if (sc2 >= need) best = 4 * (int)lane + 2;
             if (sc3 >= need) best = 4 * (int)lane + 3;
-#pragma unroll
-            for (uint32_t off = 32; off >= 1u; off >>= 1) best = max(best, __shfl_xor(best, off));
+            best = __reduce_max_sync(~0ull, best);
             auto scan_at = [&](uint32_t i) -> uint32_t {  // i < 256, wave-uniform
                 const uint32_t q = i & 3u, src = i >> 2;
                 const uint32_t v = q == 0 ? sc0 : q == 1 ? sc1 : q == 2 ? sc2 : sc3;
@@ -424,12 +420,12 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
 }
 
 __global__ __launch_bounds__(1024) void k_topk(NetDev d, Params P) {
-    topk_block(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+    topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
 __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__ drive, uint32_t* theta,
                                                    uint32_t ncol, uint32_t inCols, uint32_t target, uint32_t par) {
-    topk_block(drive + (size_t)blockIdx.x * ncol, theta + 2 * blockIdx.x, ncol, inCols, target, par);
+    topk_block<1024>(drive + (size_t)blockIdx.x * ncol, theta + 2 * blockIdx.x, ncol, inCols, target, par);
 }
 
 // ============================================================== readout + plasticity
@@ -451,15 +447,14 @@ __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__
 // listed (the per-neuron LTP/LTD totals need all of a column's tiles in one
 // workgroup).
 template <bool STATS>
-__device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P, uint32_t col, bool post,
-                                                uint32_t n) {
+__device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P, uint32_t col, uint32_t bbv,
+                                                uint32_t cn) {
     const uint32_t lane = lane_id();
     const uint32_t par = P.step & 1u;
-    const uint32_t bbv = post ? d.bbits[col * NPC + n] : 0u;
     if (STATS) {
         const uint32_t voted = half_or(bbv);
         if (lane == 0u) {
-            d.cntLearn[col * 4u] += popc(voted);
+            atomicAdd(&d.cntLearn[col * 4u], popc(voted));
             d.learnList[par * d.ncol + atomicAdd(&d.learnCnt[par], 1u)] = col;
         }
         return;
@@ -474,81 +469,83 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
     uint32_t base = 0u;
     if (lane == 0u) {
         base = atomicAdd(&d.learnCnt[par], popc(voted));
-        d.cntLearn[col * 4u] += popc(voted);
+        atomicAdd(&d.cntLearn[col * 4u], popc(voted));
     }
     base = __shfl(base, 0);
     if (want != 0u) {
         const uint32_t at = base + popc(voted & ((1u << lane) - 1u));
-        d.tileList[(size_t)par * d.ncol * NSLOT + at] =
-            make_uint4(col * NSLOT + lane, d.conn[col * NSLOT + lane], want, 0u);
+        d.tileList[(size_t)par * d.ncol * NSLOT + at] = make_uint4(col * NSLOT + lane, cn, want, 0u);
     }
 }
 
+// Every global value a column's emit can need is loaded up front, so the
+// wave waits for memory once rather than once per use.
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t* __restrict__ spkOut, uint32_t col0) {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
+    const uint32_t col = col0 + wave;
+    if (col >= d.ncol) return;
+    const uint32_t gid = col * NPC + n;
+    const bool clampCol = P.clamp != 0u && col < P.inCols;
+    const bool rec = col >= P.inCols;
     const uint32_t th = max(d.theta[0], 1u);
-    {   // ---- phase 1: this wave's column
-        const uint32_t col = col0 + wave;
-        uint32_t b = 0u, learnCol = 0u;
-        if (col < d.ncol) {
-            const uint32_t gid = col * NPC + n;
-            const bool clampCol = P.clamp != 0u && col < P.inCols;
-            const bool gatePass = d.drive[col] >= th;
-            const bool rec = col >= P.inCols;
-            uint32_t vk = 0u, vk1 = 0u;
-            const uint32_t v = d.acc[gid];
-            if (clampCol || gatePass) {
-                const Kwta kw = kwta32(v, P.k, n);
-                vk = kw.vk; vk1 = kw.vk1;
-                b = gatePass ? kw.ball : 0u;
-                if (clampCol) b = d.stim[col];
-            }
-            const bool post = ((b >> n) & 1u) != 0u;
-            if (STATS) {
-                const uint32_t x = (!hi && post && rec) ? popc(d.bbits[gid]) : 0u;
-                const bool mem = !hi && rec && ((d.refBall[col] >> n) & 1u) != 0u;
-                const bool shared = d.mult[gid] > 1u;
-                const uint32_t sumS = wave_sum(mem && shared ? v : 0u);
-                const uint32_t sumE = wave_sum(mem && !shared ? v : 0u);
-                const uint32_t cntS = popc(ballot32(mem && shared));
-                const uint32_t cntE = popc(ballot32(mem && !shared));
-                const uint32_t xs = wave_sum(x);
-                if (lane == 0u) {
-                    const uint32_t t = P.step;
-                    if (xs) atomicAdd(&d.stats[2], xs);
-                    if (cntS) { atomicAdd(&d.driveLog[t * 4u + 0u], sumS); atomicAdd(&d.driveLog[t * 4u + 1u], cntS); }
-                    if (cntE) { atomicAdd(&d.driveLog[t * 4u + 2u], sumE); atomicAdd(&d.driveLog[t * 4u + 3u], cntE); }
-                    if (b != 0u && rec) {
-                        const uint32_t sb = 64u + t * 8u;
-                        atomicAdd(&d.stats[0], 1u);
-                        atomicAdd(&d.stats[1], popc(b));
-                        atomicAdd(&d.stats[sb + 0u], vk - vk1);
-                        atomicAdd(&d.stats[sb + 1u], vk);
-                        atomicAdd(&d.stats[sb + 2u], 1u);
-                        if (vk == vk1) atomicAdd(&d.stats[sb + 3u], 1u);
-                        if (vk == 0u) atomicAdd(&d.stats[sb + 4u], 1u);
-                    }
-                }
-            }
-            learnCol = (LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1))) ? 1u : 0u;
-            if (lane == 0u) {
-                spkOut[col] = b;
-                d.marg[col] = (vk > vk1) ? 1u : 0u;
-                d.colNZ[col] = b != 0u ? 1u : 0u;
-                if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
-            }
-            if (LEARN && learnCol) list_plasticity<STATS>(d, P, col, post, n);
-            if (LEARN && !hi && rec) {  // k_bias
-                const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
-                const int nb = (int)d.bias[gid] + (post ? (int)P.biasUp : -dec);
-                d.bias[gid] = (uint32_t)max(0, nb);
+    const uint32_t dv = d.drive[col];
+    const uint32_t v = d.acc[gid];
+    const uint32_t stimW = clampCol ? d.stim[col] : 0u;
+    const uint32_t bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
+    const uint32_t cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
+    const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
+    const bool gatePass = dv >= th;
+    uint32_t b = 0u, vk = 0u, vk1 = 0u;
+    if (clampCol || gatePass) {
+        const Kwta kw = kwta32(v, P.k, n);
+        vk = kw.vk; vk1 = kw.vk1;
+        b = gatePass ? kw.ball : 0u;
+        if (clampCol) b = stimW;
+    }
+    const bool post = ((b >> n) & 1u) != 0u;
+    if (STATS) {
+        const uint32_t x = (!hi && post && rec) ? popc(bbv) : 0u;
+        const bool mem = !hi && rec && ((d.refBall[col] >> n) & 1u) != 0u;
+        const bool shared = d.mult[gid] > 1u;
+        const uint32_t sumS = wave_sum(mem && shared ? v : 0u);
+        const uint32_t sumE = wave_sum(mem && !shared ? v : 0u);
+        const uint32_t cntS = popc(ballot32(mem && shared));
+        const uint32_t cntE = popc(ballot32(mem && !shared));
+        const uint32_t xs = wave_sum(x);
+        if (lane == 0u) {
+            const uint32_t t = P.step;
+            if (xs) atomicAdd(&d.stats[2], xs);
+            if (cntS) { atomicAdd(&d.driveLog[t * 4u + 0u], sumS); atomicAdd(&d.driveLog[t * 4u + 1u], cntS); }
+            if (cntE) { atomicAdd(&d.driveLog[t * 4u + 2u], sumE); atomicAdd(&d.driveLog[t * 4u + 3u], cntE); }
+            if (b != 0u && rec) {
+                const uint32_t sb = 64u + t * 8u;
+                atomicAdd(&d.stats[0], 1u);
+                atomicAdd(&d.stats[1], popc(b));
+                atomicAdd(&d.stats[sb + 0u], vk - vk1);
+                atomicAdd(&d.stats[sb + 1u], vk);
+                atomicAdd(&d.stats[sb + 2u], 1u);
+                if (vk == vk1) atomicAdd(&d.stats[sb + 3u], 1u);
+                if (vk == 0u) atomicAdd(&d.stats[sb + 4u], 1u);
             }
         }
     }
+    if (lane == 0u) {
+        spkOut[col] = b;
+        d.marg[col] = (vk > vk1) ? 1u : 0u;
+        d.colNZ[col] = b != 0u ? 1u : 0u;
+        if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
+    }
+    if (LEARN && !hi && rec) {  // k_bias
+        const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
+        const int nb = (int)bia + (post ? (int)P.biasUp : -dec);
+        d.bias[gid] = (uint32_t)max(0, nb);
+    }
+    const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
+    if (learnCol) list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn);
 }
 
 // k_learn (mm:436-495) for the columns emit listed: one listed column per
@@ -591,10 +588,7 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
                 if ((uint32_t)k < cnt) {
                     const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
                     a0[k] = d.W0[idx]; a1[k] = d.W1[idx]; a2[k] = d.W2[idx]; a3[k] = d.W3[idx];
-                    const uint32_t g = __builtin_amdgcn_readfirstlane(d.conn[col * NSLOT + si[k]]);
-                    const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
-                    const uint4 lo4 = s4[0], hi4 = s4[1];
-                    sp[k] = pick4(hi, hi4, lo4);
+                    sp[k] = group_words(spkIn, __builtin_amdgcn_readfirstlane(d.conn[col * NSLOT + si[k]]), hi);
                 }
             }
 #pragma unroll
@@ -648,9 +642,7 @@ __device__ __forceinline__ void learn_tiles(const NetDev& d, const Params& P, co
         const uint32_t wm = __builtin_amdgcn_readfirstlane(e.z);
         const size_t idx = (size_t)tile * TILE_U4 + lane;
         uint4 a0 = d.W0[idx], a1 = d.W1[idx], a2 = d.W2[idx], a3 = d.W3[idx];
-        const uint4* s4 = reinterpret_cast<const uint4*>(spkIn + 8u * g);
-        const uint4 lo4 = s4[0], hi4 = s4[1];
-        const uint4 sp = pick4(hi, hi4, lo4);
+        const uint4 sp = group_words(spkIn, g, hi);
         const bool want = ((wm >> n) & 1u) != 0u;
         uint32_t nLTP = 0, nLTD = 0;
         learn_word(a0.x, a1.x, a2.x, a3.x, sp.x, want, P, nLTP, nLTD);
@@ -834,12 +826,17 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // registers (PT of them), and the probe loop is unrolled so a probe whose
 // group is silent costs one scalar branch.  Group activity comes from a bit
 // per presynaptic group (gact) that k_probe_select writes with the spikes.
+// DB branch tiles (and all their probes' spike words) are in flight together;
+// the per-slot state is loaded in one batch before the branch loop and after
+// it, so a wave waits for memory a few times, not once per slot.
 //
 // Blocks are numbered so that the blocks one XCD receives (linear id L,
 // L+8, L+16, ...) walk all probe tiles of the same four columns back to back,
 // so the tile's W3 bytes are reused from that XCD's L2.
 template <int PT>
 __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles) {
+    constexpr int DB = 1;                // branch tiles in flight together
+    constexpr int PC = PT < 8 ? PT : 8;  // probes whose spike words are in flight together
     const uint32_t L = blockIdx.x;
     const uint32_t q = L >> 3;
     const uint32_t tile = q % ntiles;
@@ -853,62 +850,94 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
     const uint32_t slot0 = tile * PT;
     const uint32_t gl = d.conn[base + n];
 
-    uint32_t bm[PT];
-    uint32_t anyb = 0, live = 0;
+    // slots past the end read the last slot's state and are masked off
+    uint32_t stt[PT], stT[PT], gaw[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
-        const uint32_t sl = slot0 + p;
-        uint32_t m = 0;
-        if (sl < d.nslots && d.st_t[sl] < d.st_T[sl]) {
-            live |= 1u << (p & 31);
-            const uint32_t w = d.gact[(size_t)sl * d.gw + (gl >> 5)];
-            m = ballot32(((w >> (gl & 31u)) & 1u) != 0u);
-        }
-        bm[p] = m;
-        anyb |= m;
+        const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
+        stt[p] = d.st_t[sl];
+        stT[p] = d.st_T[sl];
+        gaw[p] = d.gact[(size_t)sl * d.gw + (gl >> 5)];
     }
-    if (PT <= 32 && live == 0u) return;
+    uint32_t bm[PT];
+    uint32_t anyb = 0;
+    uint64_t live = 0;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const bool lv = slot0 + (uint32_t)p < d.nslots && stt[p] < stT[p];
+        live |= (uint64_t)(lv ? 1u : 0u) << p;
+        bm[p] = lv ? ballot32(((gaw[p] >> (gl & 31u)) & 1u) != 0u) : 0u;
+        anyb |= bm[p];
+    }
+    if (live == 0u) return;
 
     uint32_t acc[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) acc[p] = 0u;
     const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
     uint32_t mask = anyb;
-    uint32_t s = mask ? __builtin_ctz(mask) : 0u;
-    uint4 w = mask ? Wc[(size_t)s * TILE_U4] : make_uint4(0u, 0u, 0u, 0u);
     while (mask) {
-        mask &= mask - 1u;
-        const uint32_t sn = mask ? __builtin_ctz(mask) : s;
-        const uint4 wn = Wc[(size_t)sn * TILE_U4];  // next tile in flight while this one is used
-        const uint32_t g = __builtin_amdgcn_readlane(gl, s);
+        uint32_t sidx[DB], g[DB];
+        uint4 w[DB];
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            if ((bm[p] >> s) & 1u) {
-                const uint4* s4 = reinterpret_cast<const uint4*>(d.spk + (size_t)(slot0 + p) * d.ncol + 8u * g);
-                const uint4 lo4 = s4[0], hi4 = s4[1];
-                const uint4 sv = pick4(hi, hi4, lo4);
-                uint32_t c = popc(w.x & sv.x) + popc(w.y & sv.y) + popc(w.z & sv.z) + popc(w.w & sv.w);
-                c += __shfl_xor(c, 32);
-                const bool nmda = c >= d.dthr;
-                if (d.mode == 0u) acc[p] += c;
-                else if (d.mode == 1u) acc[p] += nmda ? (c + d.boost) : c;
-                else acc[p] += nmda ? 1u : 0u;
+        for (int k = 0; k < DB; ++k) {
+            sidx[k] = 32u;
+            g[k] = 0u;
+            if (mask) {
+                const uint32_t sk = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                sidx[k] = sk;
+                w[k] = Wc[(size_t)sk * TILE_U4];
+                g[k] = __builtin_amdgcn_readlane(gl, sk);
             }
         }
-        s = sn;
-        w = wn;
+#pragma unroll
+        for (int p0 = 0; p0 < PT; p0 += PC) {
+            uint4 sv[DB][PC];
+#pragma unroll
+            for (int k = 0; k < DB; ++k) {
+                if (sidx[k] >= 32u) continue;
+#pragma unroll
+                for (int pp = 0; pp < PC; ++pp)
+                    if ((bm[p0 + pp] >> sidx[k]) & 1u)
+                        sv[k][pp] = group_words(d.spk + (size_t)(slot0 + p0 + pp) * d.ncol, g[k], hi);
+            }
+#pragma unroll
+            for (int k = 0; k < DB; ++k) {
+                if (sidx[k] >= 32u) continue;
+#pragma unroll
+                for (int pp = 0; pp < PC; ++pp) {
+                    const int p = p0 + pp;
+                    if ((bm[p] >> sidx[k]) & 1u) {
+                        const uint4 v = sv[k][pp];
+                        uint32_t c = popc(w[k].x & v.x) + popc(w[k].y & v.y) + popc(w[k].z & v.z) +
+                                     popc(w[k].w & v.w);
+                        c += __shfl_xor(c, 32);
+                        const bool nmda = c >= d.dthr;
+                        if (d.mode == 0u) acc[p] += c;
+                        else if (d.mode == 1u) acc[p] += nmda ? (c + d.boost) : c;
+                        else acc[p] += nmda ? 1u : 0u;
+                    }
+                }
+            }
+        }
     }
     const uint32_t gid = base + n;
+    const uint32_t bi = d.bias[gid] >> d.biasShift;
+    uint32_t ub[PT], fs[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
-        const uint32_t sl = slot0 + p;
-        if (sl >= d.nslots || d.st_t[sl] >= d.st_T[sl]) continue;
+        const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
+        ub[p] = d.st_bias[sl];
+        fs[p] = d.facil != 0u ? d.spk[(size_t)sl * d.ncol + col] : 0u;
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        if (((live >> p) & 1u) == 0u) continue;
+        const uint32_t sl = slot0 + (uint32_t)p;
         uint32_t a = acc[p];
-        if (d.facil != 0u && ((d.spk[(size_t)sl * d.ncol + col] >> n) & 1u) != 0u) a += d.facil;
-        if (d.st_bias[sl] != 0u) {
-            const uint32_t bi = d.bias[gid] >> d.biasShift;
-            a = (a > bi) ? (a - bi) : 0u;
-        }
+        if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
+        if (ub[p] != 0u) a = (a > bi) ? (a - bi) : 0u;
         if (!hi) d.acc[(size_t)sl * d.ncol * NPC + gid] = a;
         const uint32_t drv = half_sum(a);
         if (lane == 0u) d.drive[(size_t)sl * d.ncol + col] = drv;
@@ -927,7 +956,7 @@ __global__ __launch_bounds__(256) void k_probe_select(ProbeDev d) {
     if (t >= d.st_T[sl]) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
     const size_t so = (size_t)sl * d.ncol;
-    const uint32_t th = topk_block(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
+    const uint32_t th = topk_block<256>(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
     for (uint32_t i = tid; i < d.gw; i += 256u) sg[i] = 0u;
     __syncthreads();
     const bool clampPhase = t < d.st_clamp[sl];
@@ -1012,7 +1041,7 @@ __global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
     const uint32_t net = blockIdx.x;
     if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
     const NetDev d = m.nets[net];
-    topk_block(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+    topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
 template <bool LEARN>
