@@ -225,6 +225,13 @@ class Net:
         _check(lib().axon_work_counters(self._h, out, ctypes.c_uint32(int(reset))))
         return int(out[0]), int(out[1])
 
+    def line_counters(self, reset: bool = False):
+        """(128-byte W3 lines read by propagation, word lines rewritten by plasticity,
+        each 4 planes x 128 bytes read and written).  reset also clears work_counters."""
+        out = (ctypes.c_uint64 * 2)()
+        _check(lib().axon_line_counters(self._h, out, ctypes.c_uint32(int(reset))))
+        return int(out[0]), int(out[1])
+
     def set_profiling(self, enable: bool) -> None:
         _check(lib().axon_net_set_profiling(self._h, ctypes.c_uint32(int(enable))))
 

@@ -263,7 +263,7 @@ extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
     n->MAXT = c->max_t;
     n->NSTATS = std::max<uint32_t>(1024u, 64u + 8u * n->MAXT);
     n->NDLOG = std::max<uint32_t>(256u, 4u * n->MAXT);
-    n->PLANE_U4 = (size_t)n->NCOL * axon::NSLOT * axon::TILE_U4;
+    n->PLANE_U4 = (size_t)n->NCOL * axon::NSLOT * (axon::TILE_WORDS / 4u);
     if (const char* e = getenv("AXON_NO_GRAPH")) n->use_graphs = atoi(e) == 0;
 
     auto cleanup = [&](axon_status s) { axon_net_destroy(n); return s; };
@@ -271,17 +271,18 @@ extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
         return cleanup(fail(AXON_ERR_DEVICE, "hipStreamCreate failed"));
     NetDev& d = n->d;
     void* p = nullptr;
-    uint4** planes[4] = {&d.W0, &d.W1, &d.W2, &d.W3};
+    uint32_t** planes[4] = {&d.W0, &d.W1, &d.W2, &d.W3};
     for (int i = 0; i < 4; ++i) {
         if ((st = dev_alloc(n, &p, n->PLANE_U4 * 16)) != AXON_OK) return cleanup(st);
-        *planes[i] = (uint4*)p;
+        *planes[i] = (uint32_t*)p;
     }
     struct { uint32_t** ptr; size_t words; } bufs[] = {
         {&d.conn, (size_t)n->NCOL * axon::NSLOT}, {&d.bias, n->N}, {&d.spk[0], n->NCOL}, {&d.spk[1], n->NCOL},
         {&d.acc, n->N}, {&d.bbits, n->N}, {&d.drive, n->NCOL}, {&d.theta, 16}, {&d.marg, n->NCOL},
         {&d.stim, n->NCOL}, {&d.stats, n->NSTATS}, {&d.hist, (size_t)n->MAXT * n->NCOL}, {&d.driveLog, n->NDLOG},
         {&d.refBall, n->NCOL}, {&d.mult, n->N}, {&n->d_small, 64}, {&n->d_ball, n->NCOL}, {&n->d_mask, n->N},
-        {&d.cntProp, n->NCOL}, {&d.cntLearn, (size_t)n->NCOL * 4},
+        {&d.cntProp, (size_t)n->NCOL * 2}, {&d.cntLearn, (size_t)n->NCOL * 2},
+        {(uint32_t**)&d.firedW, (size_t)n->NCOL * axon::NSLOT / 4}, {(uint32_t**)&d.tileBand, (size_t)n->NCOL * axon::NSLOT / 4},
         {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4}, {&d.learnList, (size_t)n->NCOL * 2},
         {(uint32_t**)&d.tileList, (size_t)n->NCOL * axon::NSLOT * 2 * 4}, {&d.learnCnt, 2},
     };
@@ -521,6 +522,19 @@ extern "C" axon_status axon_buffer_words(const axon_net* n, axon_buffer w, size_
     return buffer_ptr(n, w, &p, words);
 }
 
+// The weight planes are exchanged with the host in the reference's layout
+// (per tile: [half][neuron][4 words], mm:170-172); on the device each tile is
+// word-major ([word][neuron]).  Tiles are 1 KiB and sit at the same offsets
+// in both layouts, so a conversion permutes within whole tiles.
+static bool is_plane(axon_buffer w) {
+    return w == AXON_BUF_W0 || w == AXON_BUF_W1 || w == AXON_BUF_W2 || w == AXON_BUF_W3;
+}
+static inline size_t ref_to_dev_word(size_t i) {  // index within the plane
+    const size_t tile = i >> 8, r = i & 255u;
+    const size_t h = r >> 7, nn = (r >> 2) & 31u, k = r & 3u;
+    return (tile << 8) | ((h * 4u + k) << 5) | nn;
+}
+
 extern "C" axon_status axon_read_buffer(axon_net* n, axon_buffer w, void* dst, size_t bytes) {
     if (!n || !dst) return fail(AXON_ERR_INVALID_ARG, "null argument");
     void* p; size_t words;
@@ -529,7 +543,15 @@ extern "C" axon_status axon_read_buffer(axon_net* n, axon_buffer w, void* dst, s
     if (bytes > words * 4) return fail(AXON_ERR_INVALID_ARG, "read of %zu bytes exceeds buffer (%zu)", bytes, words * 4);
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
-    HIPCHECK(hipMemcpy(dst, p, bytes, hipMemcpyDeviceToHost));
+    if (!is_plane(w)) {
+        HIPCHECK(hipMemcpy(dst, p, bytes, hipMemcpyDeviceToHost));
+        return AXON_OK;
+    }
+    const size_t tw = axon::TILE_WORDS, ntw = (bytes / 4 + tw - 1) / tw * tw;  // whole tiles covering the read
+    std::vector<uint32_t> dv(ntw), rv(ntw);
+    HIPCHECK(hipMemcpy(dv.data(), p, ntw * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ntw; ++i) rv[i] = dv[ref_to_dev_word(i)];
+    memcpy(dst, rv.data(), bytes);
     return AXON_OK;
 }
 
@@ -541,7 +563,19 @@ extern "C" axon_status axon_write_buffer(axon_net* n, axon_buffer w, const void*
     if (bytes > words * 4) return fail(AXON_ERR_INVALID_ARG, "write of %zu bytes exceeds buffer (%zu)", bytes, words * 4);
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
-    HIPCHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    if (!is_plane(w)) {
+        HIPCHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        return AXON_OK;
+    }
+    // read-modify-write of the covered tiles, then forget which tiles were known to lie in the band
+    const size_t tw = axon::TILE_WORDS, ntw = (bytes / 4 + tw - 1) / tw * tw;
+    std::vector<uint32_t> dv(ntw), rv(ntw);
+    HIPCHECK(hipMemcpy(dv.data(), p, ntw * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ntw; ++i) rv[i] = dv[ref_to_dev_word(i)];
+    memcpy(rv.data(), src, bytes);
+    for (size_t i = 0; i < ntw; ++i) dv[ref_to_dev_word(i)] = rv[i];
+    HIPCHECK(hipMemcpy(p, dv.data(), ntw * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(n->d.tileBand, 0, (size_t)n->NCOL * axon::NSLOT));
     return AXON_OK;
 }
 
@@ -568,19 +602,44 @@ extern "C" axon_status axon_net_set_topk_parallel(axon_net* n, uint32_t par) {
     return AXON_OK;
 }
 
+// Sums the per-column counter pairs: out[0] += entries 2c, out[1] += 2c + 1.
+static axon_status counter_pairs(axon_net* n, uint32_t* dev, uint64_t out[2], uint32_t reset) {
+    std::vector<uint32_t> v((size_t)n->NCOL * 2);
+    HIPCHECK(hipMemcpy(v.data(), dev, v.size() * 4, hipMemcpyDeviceToHost));
+    out[0] = 0; out[1] = 0;
+    for (size_t i = 0; i < v.size(); ++i) out[i & 1u] += v[i];
+    if (reset) HIPCHECK(hipMemset(dev, 0, v.size() * 4));
+    return AXON_OK;
+}
+
 extern "C" axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset) {
     if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
-    std::vector<uint32_t> p(n->NCOL), l((size_t)n->NCOL * 4);
-    HIPCHECK(hipMemcpy(p.data(), n->d.cntProp, p.size() * 4, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(l.data(), n->d.cntLearn, l.size() * 4, hipMemcpyDeviceToHost));
-    out[0] = 0; out[1] = 0;
-    for (uint32_t x : p) out[0] += x;
-    for (uint32_t x : l) out[1] += x;
+    uint64_t p[2], l[2];
+    axon_status st = counter_pairs(n, n->d.cntProp, p, 0);
+    if (st == AXON_OK) st = counter_pairs(n, n->d.cntLearn, l, 0);
+    if (st != AXON_OK) return st;
+    out[0] = p[0]; out[1] = l[0];
     if (reset) {
-        HIPCHECK(hipMemset(n->d.cntProp, 0, p.size() * 4));
-        HIPCHECK(hipMemset(n->d.cntLearn, 0, l.size() * 4));
+        HIPCHECK(hipMemset(n->d.cntProp, 0, (size_t)n->NCOL * 8));
+        HIPCHECK(hipMemset(n->d.cntLearn, 0, (size_t)n->NCOL * 8));
+    }
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t reset) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    uint64_t p[2], l[2];
+    axon_status st = counter_pairs(n, n->d.cntProp, p, 0);
+    if (st == AXON_OK) st = counter_pairs(n, n->d.cntLearn, l, 0);
+    if (st != AXON_OK) return st;
+    out[0] = p[1]; out[1] = l[1];
+    if (reset) {
+        HIPCHECK(hipMemset(n->d.cntProp, 0, (size_t)n->NCOL * 8));
+        HIPCHECK(hipMemset(n->d.cntLearn, 0, (size_t)n->NCOL * 8));
     }
     return AXON_OK;
 }
@@ -767,7 +826,7 @@ extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t 
     axon::ProbeDev& d = b->d;
     d.W3 = n->d.W3; d.conn = n->d.conn; d.bias = n->d.bias;
     d.nslots = ns; d.ncol = n->NCOL; d.incols = n->cfg.incols; d.maxt = max_t;
-    d.gw = (n->NCOL / axon::COLS_PER_GRP + 31) / 32;
+    d.gw = (n->NCOL + 31) / 32;
     d.k = n->kTest ? n->kTest : n->K;  // readout k (mm:818)
     d.facil = n->facil;
     d.dthr = n->P.dthr; d.boost = n->P.boost; d.mode = n->P.mode; d.biasShift = n->P.biasShift;

@@ -12,7 +12,12 @@ constexpr uint32_t NSLOT = 32;     // dendritic branches per neuron, mm:34
 constexpr uint32_t PRESYN = 256;   // presynaptic neurons per branch, mm:36
 constexpr uint32_t FANIN = 8192;   // mm:37
 constexpr uint32_t COLS_PER_GRP = 8;
-constexpr uint32_t TILE_U4 = 64;   // uint4 per (column, branch) tile per plane: [j][lane]
+// A (column, branch) tile is 32 neurons x 256 presynaptic bits per plane,
+// stored word-major: plane[(tile * 8 + w) * 32 + n] holds the 32 bits of
+// presynaptic column 8g + w (g = the branch's group) for neuron n, so one
+// presynaptic column's word for all 32 neurons is one 128-byte line, and a
+// kernel reads only the lines of presynaptic columns that spiked.
+constexpr uint32_t TILE_WORDS = 256;  // uint32 per tile per plane
 
 // Per-step kernel parameters.  Same 26 words as the reference Params
 // (mm:56-83); the fields no kernel reads (learn, freeze, theta) are kept so
@@ -25,7 +30,7 @@ struct Params {
 
 // Device pointers of one network instance.
 struct NetDev {
-    uint4* W0; uint4* W1; uint4* W2; uint4* W3;
+    uint32_t* W0; uint32_t* W1; uint32_t* W2; uint32_t* W3;  // word-major tiles
     uint32_t* conn;
     uint32_t* bias;
     uint32_t* spk[2];
@@ -40,15 +45,25 @@ struct NetDev {
     uint32_t* driveLog;
     uint32_t* refBall;
     uint32_t* mult;
-    // Work counters (no reference counterpart): per column, the number of
-    // (column, branch) tiles propagation read, and the number of tiles
-    // plasticity rewrote (entry 4 * column); accumulated across steps.
-    uint32_t* cntProp;   // ncol
-    uint32_t* cntLearn;  // ncol * 4
+    // Work counters (no reference counterpart), accumulated across steps:
+    // per column, [2c] (column, branch) tiles propagation evaluated and
+    // [2c + 1] 128-byte W3 lines it read; the same pair for the tiles and word
+    // lines (4 planes each) plasticity rewrote.
+    uint32_t* cntProp;   // ncol * 2
+    uint32_t* cntLearn;  // ncol * 2
     uint8_t* colNZ;      // ncol (+pad to 16): column spiked in the last step
+    // Per (column, branch) tile: the 8-bit mask of the branch group's columns
+    // that spiked in the step propagation last read (written by propagation,
+    // read by plasticity), and whether every counter of the tile is known to
+    // lie in [bandLo, bandHi] (set by a full-tile plasticity pass with the
+    // band clip, cleared by host writes of the planes).
+    uint8_t* firedW;     // ncol * 32
+    uint8_t* tileBand;   // ncol * 32
     // Plasticity work of step t, in list t & 1: the learning columns when side
     // statistics are on (learnList), else the voted tiles (tileList entries
-    // {column * 32 + slot, presynaptic group, learning-neuron mask, 0}).
+    // {column * 32 + slot, presynaptic group, learning-neuron mask, flags};
+    // flags = fired-column mask of the group | 0x100 when all 8 words of
+    // the tile must be rewritten).
     uint32_t* learnList; // [2][ncol]
     uint4* tileList;     // [2][ncol * 32]
     uint32_t* learnCnt;  // [2] entries in each list (reset by the step after next's propagation)
@@ -59,11 +74,11 @@ struct NetDev {
 // Device pointers of a probe batch: `nslots` independent readout
 // presentations against one network (its W3, conn and bias).
 struct ProbeDev {
-    const uint4* W3;
+    const uint32_t* W3;
     const uint32_t* conn;
     const uint32_t* bias;
     uint32_t* spk;       // [slot][ncol]   spikes of the last completed step
-    uint32_t* gact;      // [slot][gw]     1 bit per presynaptic group with any spike
+    uint32_t* gact;      // [slot][gw]     1 bit per column that spiked in the last step
     uint32_t* acc;       // [slot][ncol*32]
     uint32_t* drive;     // [slot][ncol]
     uint32_t* stim;      // [slot][ncol]

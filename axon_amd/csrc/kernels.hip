@@ -56,6 +56,39 @@ __device__ __forceinline__ uint32_t half_or(uint32_t x) {
     return x;
 }
 
+// Word-major tile addressing (see TILE_WORDS): word w of neuron n of a tile.
+__device__ __forceinline__ size_t word_idx(uint32_t tile, uint32_t w, uint32_t n) {
+    return ((size_t)tile * 8u + w) * 32u + n;
+}
+// Words 4h..4h+3 of neuron n (the presynaptic columns 8g+4h..8g+4h+3): one
+// lane's share of a tile when lanes 32h..32h+31 cover half h.
+__device__ __forceinline__ uint4 lane_words(const uint32_t* __restrict__ plane, uint32_t tile, uint32_t h,
+                                            uint32_t n) {
+    const uint32_t* p = plane + word_idx(tile, 4u * h, n);
+    return make_uint4(p[0], p[32], p[64], p[96]);
+}
+// Range-checked buffer access: a lane whose byte offset is at or past `bytes`
+// reads 0 and writes nothing, and issues no memory request.  Used to load or
+// store only the words a lane needs without branching around the access.
+constexpr uint32_t SKIP = 0x80000000u;  // an offset past every range used here
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
+}
+
+// 8-bit mask of the nonzero bytes of x (bit i = byte i).
+__device__ __forceinline__ uint32_t byte_mask(uint64_t x) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) m |= (((x >> (8 * i)) & 0xFFull) != 0ull ? 1u : 0u) << i;
+    return m;
+}
+
 // The 8 spike words of presynaptic group g (columns 8g..8g+7) as this lane's
 // half: words 0-3 in lanes 0-31, words 4-7 in lanes 32-63.  A vector load,
 // so a wave keeps many of them in flight (scalar loads of the same words are
@@ -169,10 +202,12 @@ __device__ __forceinline__ void learn_word(uint32_t& a0, uint32_t& a1, uint32_t&
 }
 
 // ============================================================== init
-// k_init_w, mm:86-104
-__global__ __launch_bounds__(256) void k_init_w(uint4* __restrict__ W0, uint4* __restrict__ W1,
-                                                uint4* __restrict__ W2, uint4* __restrict__ W3, uint32_t nAnd,
-                                                uint32_t seed, uint32_t n) {
+// k_init_w, mm:86-104.  Thread gg produces the values of the reference's
+// uint4 number gg (tile gg / 64, half (gg / 32) & 1, neuron gg & 31) and
+// stores its four words at their word-major places.
+__global__ __launch_bounds__(256) void k_init_w(uint32_t* __restrict__ W0, uint32_t* __restrict__ W1,
+                                                uint32_t* __restrict__ W2, uint32_t* __restrict__ W3,
+                                                uint32_t nAnd, uint32_t seed, uint32_t n) {
     const uint32_t gg = blockIdx.x * 256u + threadIdx.x;
     if (gg >= n) return;
     const uint32_t g = gg ^ (seed * 0x9E3779B9u);
@@ -185,10 +220,16 @@ __global__ __launch_bounds__(256) void k_init_w(uint4* __restrict__ W0, uint4* _
     }
     const uint4 c = make_uint4(hash32(g * 8u + 11u), hash32(g * 8u + 13u), hash32(g * 8u + 17u),
                                hash32(g * 8u + 19u));
-    W3[gg] = m;
-    W2[gg] = make_uint4(~m.x | c.x, ~m.y | c.y, ~m.z | c.z, ~m.w | c.w);
-    W1[gg] = c;
-    W0[gg] = make_uint4(~c.x, ~c.y, ~c.z, ~c.w);
+    const uint32_t mw[4] = {m.x, m.y, m.z, m.w}, cw[4] = {c.x, c.y, c.z, c.w};
+    const uint32_t tile = gg >> 6, h = (gg >> 5) & 1u, nn = gg & 31u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {
+        const size_t i = word_idx(tile, 4u * h + k, nn);
+        W3[i] = mw[k];
+        W2[i] = ~mw[k] | cw[k];
+        W1[i] = cw[k];
+        W0[i] = ~cw[k];
+    }
 }
 
 // k_init_conn, mm:109-117
@@ -227,66 +268,126 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t* __restrict__ a, uint32_
 // overlaps the staging of `fired`).  `fired`, when not null, is the LDS copy
 // of the per-column "spiked last step" bytes: a branch is active when any of
 // the 8 columns of its presynaptic group fired, one 8-byte LDS read.
+// The per-column inputs that do not depend on which branches are active,
+// loaded unconditionally before anything waits for memory: the lane's branch
+// group (conn), its neuron's bias word, the column's previous spike word and
+// its two work counters (only this column's wave writes them, so they are
+// updated by a plain read-modify-write instead of an atomic).
+struct DriveIn { uint32_t gl, biasw, prev; uint2 cnt; };
+__device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t* __restrict__ spkIn, uint32_t col) {
+    const uint32_t n = lane_id() & 31u;
+    return DriveIn{d.conn[col * NSLOT + n], d.bias[col * NPC + n], spkIn[col],
+                   reinterpret_cast<const uint2*>(d.cntProp)[col]};
+}
+
+constexpr uint32_t ITEMS_PER_HALF = NSLOT * 8u;  // LDS item list capacity of a half-wave
+
 __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
-                                          uint32_t col, uint32_t gl, const uint8_t* fired) {
+                                          uint32_t col, const DriveIn& in, const uint8_t* fired,
+                                          uint32_t* items) {
     const uint32_t lane = lane_id();
     const bool hi = lane >= 32u;
     const uint32_t n = lane & 31u;
     const uint32_t base = col * NSLOT;
+    const uint32_t gid = base + n;
+    const uint32_t gl = in.gl;
+    const uint32_t biasw = (P.useBias != 0u) ? in.biasw : 0u;
+    const uint32_t prev = (P.facil != 0u) ? in.prev : 0u;
 
-    uint32_t mask;
+    // lane s < 32: which of slot s's eight presynaptic columns spiked
+    uint32_t fwl;
     if (fired) {
-        mask = ballot32(*reinterpret_cast<const uint64_t*>(fired + 8u * gl) != 0ull);
+        fwl = byte_mask(*reinterpret_cast<const uint64_t*>(fired + 8u * gl));
     } else {
-        const uint4 sp = reinterpret_cast<const uint4*>(spkIn)[2u * gl + (hi ? 1u : 0u)];
-        const uint32_t anyh = sp.x | sp.y | sp.z | sp.w;
-        mask = ballot32((anyh | __shfl_xor(anyh, 32)) != 0u);
+        const uint4 a = reinterpret_cast<const uint4*>(spkIn)[2u * gl];
+        const uint4 b = reinterpret_cast<const uint4*>(spkIn)[2u * gl + 1u];
+        fwl = (a.x ? 1u : 0u) | (a.y ? 2u : 0u) | (a.z ? 4u : 0u) | (a.w ? 8u : 0u) | (b.x ? 16u : 0u) |
+              (b.y ? 32u : 0u) | (b.z ? 64u : 0u) | (b.w ? 128u : 0u);
     }
+    const uint32_t mask = ballot32(fwl != 0u);
+    if (!hi) d.firedW[base + n] = (uint8_t)fwl;
     const uint32_t nActive = popc(mask);
 
-    const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
-    uint32_t acc = 0u, bb = 0u;
-    constexpr int DB = 8;  // active tiles fetched per round trip
-    while (mask) {
-        uint32_t sidx[DB];
-        uint4 w[DB];
-        uint4 sv[DB];
+    // Work items are (branch, spiking presynaptic column) pairs, one 128-byte
+    // W3 line each.  The active branches are dealt alternately to the two
+    // half-waves; each half lists all items of its branches, in order, in LDS
+    // (lane s writes branch s's items at a scanned offset), so a single dword
+    // load instruction serves one item of each half and a branch's count c
+    // (summed over its items) completes inside one half.  Item word: branch |
+    // word << 5 | group << 8 | 1 << 31 on the branch's last item.
+    const uint32_t rank = popc(mask & ((1u << n) - 1u));
+    const uint32_t hh = rank & 1u;
+    const uint32_t ni = popc(fwl);
+    const uint32_t x0 = hh == 0u ? ni : 0u, x1 = hh == 1u ? ni : 0u;
+    uint32_t s0 = x0, s1 = x1;  // inclusive scans over the 32 branches (each half computes the same)
 #pragma unroll
-        for (int k = 0; k < DB; ++k) {
-            sidx[k] = 32u;
-            w[k] = make_uint4(0u, 0u, 0u, 0u);
-            sv[k] = w[k];
-            if (mask) {
-                const uint32_t sk = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                sidx[k] = sk;
-                w[k] = Wc[(size_t)sk * TILE_U4];
-                sv[k] = group_words(spkIn, __builtin_amdgcn_readlane(gl, sk), hi);
+    for (uint32_t off = 1; off < 32u; off <<= 1) {
+        const uint32_t t0 = __shfl_up(s0, off, 32), t1 = __shfl_up(s1, off, 32);
+        if (n >= off) { s0 += t0; s1 += t1; }
+    }
+    const uint32_t nA = __builtin_amdgcn_readlane(s0, 31), nB = __builtin_amdgcn_readlane(s1, 31);
+    if (!hi && fwl != 0u) {
+        uint32_t* L = items + hh * ITEMS_PER_HALF;
+        uint32_t pos = hh ? s1 - x1 : s0 - x0;
+        uint32_t m = fwl;
+        while (m) {
+            const uint32_t w = __builtin_ctz(m);
+            m &= m - 1u;
+            L[pos++] = n | (w << 5) | (gl << 8) | (m == 0u ? 0x80000000u : 0u);
+        }
+    }
+    const uint32_t nLines = nA + nB;
+    const uint32_t myN = hi ? nB : nA;
+    const uint32_t* L = items + (hi ? ITEMS_PER_HALF : 0u);  // read after this wave's writes above
+    const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);  // wave-uniform: the resources live in SGPRs
+    const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, NSLOT * TILE_WORDS * 4u);
+    const __amdgpu_buffer_rsrc_t sr = buf_rsrc(spkIn, d.ncol * 4u);
+    uint32_t acc = 0u, bb = 0u, c = 0u;
+    constexpr uint32_t DB = 8;  // items per half fetched per round trip
+    const uint32_t R = max(nA, nB);
+    for (uint32_t r = 0; r < R; r += DB) {
+        const uint32_t kn = min(R - r, DB);  // item places used this round (uniform)
+        const uint4 i0 = *reinterpret_cast<const uint4*>(L + r);
+        const uint4 i1 = *reinterpret_cast<const uint4*>(L + r + 4u);
+        const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+        uint32_t wv[DB], sv[DB];
+#pragma unroll
+        for (uint32_t k = 0; k < DB; ++k) {
+            wv[k] = 0u; sv[k] = 0u;
+            if (k < kn) {  // no load instructions for unused item places
+                const bool v = r + k < myN;
+                const uint32_t e = it[k];
+                const uint32_t br = e & 31u, w = (e >> 5) & 7u, g = (e >> 8) & 0x7FFFFFu;
+                wv[k] = buf_ld(w3, v ? ((br * 8u + w) * 32u + n) * 4u : SKIP);
+                sv[k] = buf_ld(sr, v ? (8u * g + w) * 4u : SKIP);
             }
         }
 #pragma unroll
-        for (int k = 0; k < DB; ++k) {
-            if (sidx[k] < 32u) {
-                uint32_t c = popc(w[k].x & sv[k].x) + popc(w[k].y & sv[k].y) + popc(w[k].z & sv[k].z) +
-                             popc(w[k].w & sv[k].w);
-                c += __shfl_xor(c, 32);
-                const bool nmda = c >= P.dthr;
-                if (nmda) bb |= 1u << sidx[k];
-                if (P.mode == 0u) acc += c;
-                else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
-                else acc += nmda ? 1u : 0u;
+        for (uint32_t k = 0; k < DB; ++k) {
+            if (k < kn && r + k < myN) {
+                const uint32_t e = it[k];
+                c += popc(wv[k] & sv[k]);
+                if (e >> 31) {  // last item of this half's branch: its count is complete
+                    const bool nmda = c >= P.dthr;
+                    if (nmda) bb |= 1u << (e & 31u);
+                    if (P.mode == 0u) acc += c;
+                    else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
+                    else acc += nmda ? 1u : 0u;
+                    c = 0u;
+                }
             }
         }
     }
-    if (P.facil != 0u && ((spkIn[col] >> n) & 1u) != 0u) acc += P.facil;
-    const uint32_t gid = base + n;
-    const uint32_t bi = (P.useBias != 0u) ? (d.bias[gid] >> P.biasShift) : 0u;
+    acc += __shfl_xor(acc, 32);  // the two halves' branches
+    bb |= __shfl_xor(bb, 32);
+    if (P.facil != 0u && ((prev >> n) & 1u) != 0u) acc += P.facil;
+    const uint32_t bi = biasw >> P.biasShift;
     acc = (acc > bi) ? (acc - bi) : 0u;
     if (!hi) { d.acc[gid] = acc; d.bbits[gid] = bb; }
     const uint32_t drv = half_sum(acc);
     if (lane == 0u) {
         d.drive[col] = drv;
-        atomicAdd(&d.cntProp[col], nActive);  // last: a memory wait after it would include it
+        reinterpret_cast<uint2*>(d.cntProp)[col] = make_uint2(in.cnt.x + nActive, in.cnt.y + nLines);
     }
 }
 
@@ -303,13 +404,14 @@ __device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* ld
 
 __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
     __shared__ uint4 fired[FIRED_LDS_COLS / 16];
+    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
     // the learning-column list of step t-1 has been consumed; free it for step t+1
     if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
     const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t gl = col < d.ncol ? d.conn[col * NSLOT + (lane_id() & 31u)] : 0u;
+    const DriveIn in = drive_inputs(d, spkIn, min(col, d.ncol - 1u));
     const uint8_t* f = stage_fired(d, fired);
     if (col >= d.ncol) return;
-    drive_col(d, P, spkIn, col, gl, f);
+    drive_col(d, P, spkIn, col, in, f, items[threadIdx.x >> 6]);
 }
 
 // ============================================================== column top-K
@@ -446,15 +548,25 @@ __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__
 // per tile with no further lookups.  With statistics the column itself is
 // listed (the per-neuron LTP/LTD totals need all of a column's tiles in one
 // workgroup).
+// Whether a voted tile needs all 8 words rewritten: the update touches words
+// of silent presynaptic columns (the legacy rule with depression, hetero-
+// synaptic LTD), or the band clip may still move counters the tile's spikes
+// do not touch.  Otherwise only the spiking columns' words can change.
+__device__ __forceinline__ bool full_tile_update(const Params& P, uint32_t band) {
+    return P.phase == 0u || (P.phase == 1u && P.hetero != 0u) || (P.bandHi != 0u && band == 0u);
+}
+
 template <bool STATS>
 __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P, uint32_t col, uint32_t bbv,
-                                                uint32_t cn) {
+                                                uint32_t cn, uint32_t fwb, uint32_t band, uint2 lc) {
     const uint32_t lane = lane_id();
     const uint32_t par = P.step & 1u;
     if (STATS) {
         const uint32_t voted = half_or(bbv);
+        const bool v = lane < 32u && ((voted >> lane) & 1u) != 0u;
+        const uint32_t lines = wave_sum(v ? (full_tile_update(P, band) ? 8u : popc(fwb)) : 0u);
         if (lane == 0u) {
-            atomicAdd(&d.cntLearn[col * 4u], popc(voted));
+            reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
             d.learnList[par * d.ncol + atomicAdd(&d.learnCnt[par], 1u)] = col;
         }
         return;
@@ -466,15 +578,19 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
         if (lane == s) want = m;
     }
     const uint32_t voted = ballot32(want != 0u);
+    const bool full = full_tile_update(P, band);
+    const uint32_t lines = wave_sum(want != 0u ? (full ? 8u : popc(fwb)) : 0u);
     uint32_t base = 0u;
     if (lane == 0u) {
         base = atomicAdd(&d.learnCnt[par], popc(voted));
-        atomicAdd(&d.cntLearn[col * 4u], popc(voted));
+        // only this column's wave writes its counters: a plain read-modify-write
+        reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
     }
     base = __shfl(base, 0);
     if (want != 0u) {
         const uint32_t at = base + popc(voted & ((1u << lane) - 1u));
-        d.tileList[(size_t)par * d.ncol * NSLOT + at] = make_uint4(col * NSLOT + lane, cn, want, 0u);
+        d.tileList[(size_t)par * d.ncol * NSLOT + at] =
+            make_uint4(col * NSLOT + lane, cn, want, fwb | (full ? 0x100u : 0u));
     }
 }
 
@@ -497,6 +613,9 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
     const uint32_t stimW = clampCol ? d.stim[col] : 0u;
     const uint32_t bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
     const uint32_t cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
+    const uint32_t fwb = LEARN ? d.firedW[col * NSLOT + n] : 0u;
+    const uint32_t band = LEARN ? d.tileBand[col * NSLOT + n] : 0u;
+    const uint2 lc = LEARN ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
     const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
     const bool gatePass = dv >= th;
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
@@ -545,12 +664,45 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
         d.bias[gid] = (uint32_t)max(0, nb);
     }
     const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
-    if (learnCol) list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn);
+    if (learnCol) list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn, fwb, band, lc);
 }
 
-// k_learn (mm:436-495) for the columns emit listed: one listed column per
-// workgroup (grid-stride), the 4 waves rewriting its voted tiles together,
-// 8 branches per wave, so every learning column proceeds in parallel.
+// One voted tile of k_learn (mm:436-495).  Lane (h, n) owns words 4h..4h+3
+// of neuron n in each of the four planes; with `fl` bit 8 clear only the
+// words of presynaptic columns that spiked (fl bits 0-7) are read and
+// written, the others being unchanged by the update.  A full pass that
+// applies the band clip leaves every counter of the tile inside the band.
+__device__ __forceinline__ void learn_tile(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                           uint32_t tile, uint32_t g, uint32_t wm, uint32_t fl, uint32_t& nLTP,
+                                           uint32_t& nLTD) {
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const bool hi = lane >= 32u;
+    const bool full = (fl & 0x100u) != 0u;
+    const uint32_t my = full ? 0xFu : ((fl >> (hi ? 4u : 0u)) & 0xFu);
+    const uint4 sp4 = group_words(spkIn, g, hi);
+    const uint32_t sp[4] = {sp4.x, sp4.y, sp4.z, sp4.w};
+    const size_t t0 = (size_t)__builtin_amdgcn_readfirstlane(tile) * TILE_WORDS;
+    const __amdgpu_buffer_rsrc_t r0 = buf_rsrc(d.W0 + t0, TILE_WORDS * 4u), r1 = buf_rsrc(d.W1 + t0, TILE_WORDS * 4u),
+                                 r2 = buf_rsrc(d.W2 + t0, TILE_WORDS * 4u), r3 = buf_rsrc(d.W3 + t0, TILE_WORDS * 4u);
+    const uint32_t o0 = ((hi ? 4u : 0u) * 32u + n) * 4u;
+    uint32_t a[4][4], o[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {  // words this lane leaves alone are neither read nor written
+        o[k] = ((my >> k) & 1u) ? o0 + 128u * k : SKIP;
+        a[k][0] = buf_ld(r0, o[k]); a[k][1] = buf_ld(r1, o[k]); a[k][2] = buf_ld(r2, o[k]); a[k][3] = buf_ld(r3, o[k]);
+    }
+    const bool want = ((wm >> n) & 1u) != 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {
+        learn_word(a[k][0], a[k][1], a[k][2], a[k][3], (my >> k) & 1u ? sp[k] : 0u, want, P, nLTP, nLTD);
+        buf_st(r0, o[k], a[k][0]); buf_st(r1, o[k], a[k][1]); buf_st(r2, o[k], a[k][2]); buf_st(r3, o[k], a[k][3]);
+    }
+    if (full && P.bandHi != 0u && lane == 0u) d.tileBand[tile] = 1u;
+}
+
+// k_learn for the columns emit listed (side statistics on): one listed column
+// per workgroup (grid-stride), its voted tiles dealt round-robin to the four
+// waves; the per-neuron LTP/LTD totals are reduced in LDS (mm:491-494).
 template <bool STATS>
 __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                            const uint32_t* __restrict__ spkOut) {
@@ -564,45 +716,25 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
         const uint32_t col = d.learnList[par * d.ncol + li];
         const uint32_t b = spkOut[col];
         const bool post = ((b >> n) & 1u) != 0u;
-        const uint32_t bb = d.bbits[col * NPC + n];
-        uint32_t voted = 0;
+        const uint32_t bb = post ? d.bbits[col * NPC + n] : 0u;
+        const uint32_t cn = d.conn[col * NSLOT + n];
+        const uint32_t fwb = d.firedW[col * NSLOT + n];
+        const uint32_t band = d.tileBand[col * NSLOT + n];
+        uint32_t want = 0u;  // lane s < 32: the neurons whose slot-s branch learns
 #pragma unroll
-        for (uint32_t i = 0; i < 8u; ++i) {
-            const uint32_t s = wave * 8u + i;
-            if (ballot32(post && ((bb >> s) & 1u) != 0u)) voted |= 1u << i;
+        for (uint32_t s = 0; s < 32u; ++s) {
+            const uint32_t m = ballot32(((bb >> s) & 1u) != 0u);
+            if (lane == s) want = m;
         }
+        const uint32_t fl = fwb | (full_tile_update(P, band) ? 0x100u : 0u);
+        uint32_t voted = ballot32(want != 0u);
         uint32_t nLTP = 0, nLTD = 0;
-        const size_t tileBase = (size_t)col * NSLOT * TILE_U4 + lane;
-        while (voted) {
-            constexpr int LB = 4;  // voted tiles whose four planes are in flight together
-            uint32_t si[LB];
-            uint32_t cnt = 0;
-#pragma unroll
-            for (int k = 0; k < LB; ++k) {
-                if (voted) { si[k] = wave * 8u + __builtin_ctz(voted); voted &= voted - 1u; ++cnt; }
-                else si[k] = si[0];
-            }
-            uint4 a0[LB], a1[LB], a2[LB], a3[LB], sp[LB];
-#pragma unroll
-            for (int k = 0; k < LB; ++k) {
-                if ((uint32_t)k < cnt) {
-                    const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
-                    a0[k] = d.W0[idx]; a1[k] = d.W1[idx]; a2[k] = d.W2[idx]; a3[k] = d.W3[idx];
-                    sp[k] = group_words(spkIn, __builtin_amdgcn_readfirstlane(d.conn[col * NSLOT + si[k]]), hi);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < LB; ++k) {
-                if ((uint32_t)k < cnt) {
-                    const bool want = post && ((bb >> si[k]) & 1u) != 0u;
-                    learn_word(a0[k].x, a1[k].x, a2[k].x, a3[k].x, sp[k].x, want, P, nLTP, nLTD);
-                    learn_word(a0[k].y, a1[k].y, a2[k].y, a3[k].y, sp[k].y, want, P, nLTP, nLTD);
-                    learn_word(a0[k].z, a1[k].z, a2[k].z, a3[k].z, sp[k].z, want, P, nLTP, nLTD);
-                    learn_word(a0[k].w, a1[k].w, a2[k].w, a3[k].w, sp[k].w, want, P, nLTP, nLTD);
-                    const size_t idx = tileBase + (size_t)si[k] * TILE_U4;
-                    d.W0[idx] = a0[k]; d.W1[idx] = a1[k]; d.W2[idx] = a2[k]; d.W3[idx] = a3[k];
-                }
-            }
+        for (uint32_t r = 0; voted; ++r) {
+            const uint32_t s = __builtin_ctz(voted);
+            voted &= voted - 1u;
+            if ((r & 3u) != wave) continue;
+            learn_tile(d, P, spkIn, col * NSLOT + s, __builtin_amdgcn_readlane(cn, s),
+                       __builtin_amdgcn_readlane(want, s), __builtin_amdgcn_readlane(fl, s), nLTP, nLTD);
         }
         if (STATS) {  // per-neuron totals over both halves and all four waves (mm:491-494)
             nLTP += __shfl_xor(nLTP, 32);
@@ -626,30 +758,23 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
     }
 }
 
-// k_learn (mm:436-495) over the voted-tile list: wave w of the grid rewrites
-// tiles w, w + nwaves, ... (four bit-planes read, updated, written back).
+// k_learn over the voted-tile list: wave w of the grid rewrites tiles w,
+// w + nwaves, ...
 __device__ __forceinline__ void learn_tiles(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                             uint32_t w0, uint32_t nwaves) {
-    const uint32_t lane = lane_id(), n = lane & 31u;
-    const bool hi = lane >= 32u;
     const uint32_t par = P.step & 1u;
-    const uint32_t cnt = d.learnCnt[par];
     const uint4* __restrict__ list = d.tileList + (size_t)par * d.ncol * NSLOT;
-    for (uint32_t i = w0; i < cnt; i += nwaves) {
-        const uint4 e = list[i];
-        const uint32_t tile = __builtin_amdgcn_readfirstlane(e.x);
-        const uint32_t g = __builtin_amdgcn_readfirstlane(e.y);
-        const uint32_t wm = __builtin_amdgcn_readfirstlane(e.z);
-        const size_t idx = (size_t)tile * TILE_U4 + lane;
-        uint4 a0 = d.W0[idx], a1 = d.W1[idx], a2 = d.W2[idx], a3 = d.W3[idx];
-        const uint4 sp = group_words(spkIn, g, hi);
-        const bool want = ((wm >> n) & 1u) != 0u;
+    // the first entry is read together with the count, unconditionally (the
+    // index is clamped into the list's ncol * 32 entries; used only if w0 < count)
+    uint4 e = list[min(w0, d.ncol * NSLOT - 1u)];
+    const uint32_t cnt = d.learnCnt[par];
+    asm volatile("" ::"v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w), "v"(cnt));  // both in flight before either is used
+    for (uint32_t i = w0; i < cnt;) {
         uint32_t nLTP = 0, nLTD = 0;
-        learn_word(a0.x, a1.x, a2.x, a3.x, sp.x, want, P, nLTP, nLTD);
-        learn_word(a0.y, a1.y, a2.y, a3.y, sp.y, want, P, nLTP, nLTD);
-        learn_word(a0.z, a1.z, a2.z, a3.z, sp.z, want, P, nLTP, nLTD);
-        learn_word(a0.w, a1.w, a2.w, a3.w, sp.w, want, P, nLTP, nLTD);
-        d.W0[idx] = a0; d.W1[idx] = a1; d.W2[idx] = a2; d.W3[idx] = a3;
+        learn_tile(d, P, spkIn, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
+                   __builtin_amdgcn_readfirstlane(e.z), __builtin_amdgcn_readfirstlane(e.w), nLTP, nLTD);
+        i += nwaves;
+        if (i < cnt) e = list[i];
     }
 }
 
@@ -672,7 +797,7 @@ __global__ __launch_bounds__(256) void k_learn(NetDev d, Params P, const uint32_
 
 // ============================================================== readouts
 // k_wstat, mm:641-646: functional synapse count (popcount of W3)
-__global__ __launch_bounds__(256) void k_wstat(const uint4* __restrict__ W3, size_t n, uint32_t* out) {
+__global__ __launch_bounds__(256) void k_wstat(const uint4* __restrict__ W3, size_t n, uint32_t* out) {  // any layout
     uint32_t s = 0;
     for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) {
         const uint4 w = W3[i];
@@ -699,7 +824,7 @@ __global__ __launch_bounds__(256) void k_within(NetDev d, const uint32_t* __rest
             const uint32_t g = d.conn[base + s];
             for (uint32_t j = 0; j < 2u; ++j) {
                 const uint4 ap = reinterpret_cast<const uint4*>(ball)[2u * g + j];
-                const uint4 w = d.W3[(size_t)(base + s) * TILE_U4 + j * 32u + n];
+                const uint4 w = lane_words(d.W3, base + s, j, n);
                 fIn += popc(w.x & ap.x) + popc(w.y & ap.y) + popc(w.z & ap.z) + popc(w.w & ap.w);
                 pIn += popc(ap.x) + popc(ap.y) + popc(ap.z) + popc(ap.w);
                 fOut += popc(w.x & ~ap.x) + popc(w.y & ~ap.y) + popc(w.z & ~ap.z) + popc(w.w & ~ap.w);
@@ -725,7 +850,7 @@ __global__ __launch_bounds__(256) void k_indeg(NetDev d, uint32_t* __restrict__ 
     uint32_t dd = 0;
     for (uint32_t s = d.inbr; s < NSLOT; ++s)
         for (uint32_t j = 0; j < 2u; ++j) {
-            const uint4 w = d.W3[(size_t)(col * NSLOT + s) * TILE_U4 + j * 32u + n];
+            const uint4 w = lane_words(d.W3, col * NSLOT + s, j, n);
             dd += popc(w.x) + popc(w.y) + popc(w.z) + popc(w.w);
         }
     out[gid] = dd;
@@ -739,8 +864,9 @@ __global__ __launch_bounds__(256) void k_untouched(NetDev d, uint32_t* __restric
     uint32_t u = 0;
     for (uint32_t s = d.inbr; s < NSLOT; ++s)
         for (uint32_t j = 0; j < 2u; ++j) {
-            const size_t idx = (size_t)(col * NSLOT + s) * TILE_U4 + j * 32u + n;
-            const uint4 a0 = d.W0[idx], a1 = d.W1[idx], a2 = d.W2[idx], a3 = d.W3[idx];
+            const uint32_t tl = col * NSLOT + s;
+            const uint4 a0 = lane_words(d.W0, tl, j, n), a1 = lane_words(d.W1, tl, j, n),
+                        a2 = lane_words(d.W2, tl, j, n), a3 = lane_words(d.W3, tl, j, n);
             auto f = [](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
                 return popc(((~x3) & x2 & (~x1) & x0) | ((~x3) & x2 & x1 & (~x0)));
             };
@@ -767,8 +893,9 @@ __global__ __launch_bounds__(256) void k_wdiag(NetDev d, const uint32_t* __restr
         if (sel == 1u && !isAff) continue;
         const uint32_t g = d.conn[base + s];
         for (uint32_t j = 0; j < 2u; ++j) {
-            const size_t idx = (size_t)(base + s) * TILE_U4 + j * 32u + n;
-            const uint4 A0 = d.W0[idx], A1 = d.W1[idx], A2 = d.W2[idx], A3 = d.W3[idx];
+            const uint32_t tl = base + s;
+            const uint4 A0 = lane_words(d.W0, tl, j, n), A1 = lane_words(d.W1, tl, j, n),
+                        A2 = lane_words(d.W2, tl, j, n), A3 = lane_words(d.W3, tl, j, n);
             const uint32_t a0[4] = {A0.x, A0.y, A0.z, A0.w}, a1[4] = {A1.x, A1.y, A1.z, A1.w},
                            a2[4] = {A2.x, A2.y, A2.z, A2.w}, a3[4] = {A3.x, A3.y, A3.z, A3.w};
             for (int q = 0; q < 4; ++q) {
@@ -851,22 +978,25 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
     const uint32_t gl = d.conn[base + n];
 
     // slots past the end read the last slot's state and are masked off
+    // gact holds one bit per column: the byte of group gl is bits 8gl..8gl+7
     uint32_t stt[PT], stT[PT], gaw[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
         stt[p] = d.st_t[sl];
         stT[p] = d.st_T[sl];
-        gaw[p] = d.gact[(size_t)sl * d.gw + (gl >> 5)];
+        gaw[p] = d.gact[(size_t)sl * d.gw + (gl >> 2)];
     }
     uint32_t bm[PT];
-    uint32_t anyb = 0;
+    uint32_t anyb = 0, fwu = 0;  // fwu (lane s < 32): columns of slot s's group that spiked in any live probe
     uint64_t live = 0;
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         const bool lv = slot0 + (uint32_t)p < d.nslots && stt[p] < stT[p];
         live |= (uint64_t)(lv ? 1u : 0u) << p;
-        bm[p] = lv ? ballot32(((gaw[p] >> (gl & 31u)) & 1u) != 0u) : 0u;
+        const uint32_t fb = lv ? (gaw[p] >> (8u * (gl & 3u))) & 0xFFu : 0u;
+        fwu |= fb;
+        bm[p] = ballot32(fb != 0u);
         anyb |= bm[p];
     }
     if (live == 0u) return;
@@ -874,22 +1004,31 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
     uint32_t acc[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) acc[p] = 0u;
-    const uint4* __restrict__ Wc = d.W3 + (size_t)base * TILE_U4 + lane;
+    const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);
+    const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, NSLOT * TILE_WORDS * 4u);
     uint32_t mask = anyb;
     while (mask) {
-        uint32_t sidx[DB], g[DB];
-        uint4 w[DB];
+        uint32_t sidx[DB], g[DB], fw[DB];
 #pragma unroll
         for (int k = 0; k < DB; ++k) {
-            sidx[k] = 32u;
-            g[k] = 0u;
+            sidx[k] = 32u; g[k] = 0u; fw[k] = 0u;
             if (mask) {
                 const uint32_t sk = __builtin_ctz(mask);
                 mask &= mask - 1u;
                 sidx[k] = sk;
-                w[k] = Wc[(size_t)sk * TILE_U4];
+                fw[k] = __builtin_amdgcn_readlane(fwu, sk);
                 g[k] = __builtin_amdgcn_readlane(gl, sk);
             }
+        }
+        uint4 w[DB];
+#pragma unroll
+        for (int k = 0; k < DB; ++k) {  // the W3 lines of columns that spiked in some probe of the tile
+            const uint32_t my = (fw[k] >> (hi ? 4u : 0u)) & 0xFu;
+            const uint32_t o = ((sidx[k] * 8u + (hi ? 4u : 0u)) * 32u + n) * 4u;
+            w[k].x = buf_ld(w3, (my & 1u) ? o : SKIP);
+            w[k].y = buf_ld(w3, (my & 2u) ? o + 128u : SKIP);
+            w[k].z = buf_ld(w3, (my & 4u) ? o + 256u : SKIP);
+            w[k].w = buf_ld(w3, (my & 8u) ? o + 384u : SKIP);
         }
 #pragma unroll
         for (int p0 = 0; p0 < PT; p0 += PC) {
@@ -949,7 +1088,6 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
 // spike and history writes and the slot's group-activity bits.  The margin,
 // stats and drive-log side outputs of k_emit are not produced for probes.
 __global__ __launch_bounds__(256) void k_probe_select(ProbeDev d) {
-    __shared__ uint32_t sg[4096];
     const uint32_t sl = blockIdx.x;
     if (sl >= d.nslots) return;
     const uint32_t t = d.st_t[sl];
@@ -957,8 +1095,6 @@ __global__ __launch_bounds__(256) void k_probe_select(ProbeDev d) {
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
     const size_t so = (size_t)sl * d.ncol;
     const uint32_t th = topk_block<256>(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
-    for (uint32_t i = tid; i < d.gw; i += 256u) sg[i] = 0u;
-    __syncthreads();
     const bool clampPhase = t < d.st_clamp[sl];
     const uint32_t* __restrict__ acc = d.acc + so * NPC;
     uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
@@ -977,22 +1113,16 @@ __global__ __launch_bounds__(256) void k_probe_select(ProbeDev d) {
             if (lane == i) b = kw.ball;
         }
         if (inb) { d.spk[so + c] = b; hist[c] = b; }
-        const uint64_t nz = __ballot(inb && b != 0u);
-        if (lane == 0u && nz) {
-            uint32_t bits = 0;
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if ((nz >> (8 * j)) & 0xFFull) bits |= 1u << j;
-            const uint32_t g0 = c0 >> 3;
-            atomicOr(&sg[g0 >> 5], bits << (g0 & 31u));
-        }
+        const uint64_t nz = __ballot(inb && b != 0u);  // one bit per column of this wave's 64
+        uint32_t* ga = d.gact + (size_t)sl * d.gw + (c0 >> 5);
+        if (lane == 0u) ga[0] = (uint32_t)nz;
+        if (lane == 1u && c0 + 32u < d.ncol) ga[1] = (uint32_t)(nz >> 32);
     }
     __syncthreads();
-    for (uint32_t i = tid; i < d.gw; i += 256u) d.gact[(size_t)sl * d.gw + i] = sg[i];
     if (tid == 0u) d.st_t[sl] = t + 1u;
 }
 
-// Start of a probe: zero spikes and group bits, step counter to 0.
+// Start of a probe: zero spikes and spiking-column bits, step counter to 0.
 __global__ __launch_bounds__(256) void k_probe_reset(ProbeDev d, uint32_t first, uint32_t count) {
     const uint32_t sl = first + blockIdx.x;
     if (blockIdx.x >= count || sl >= d.nslots) return;
@@ -1028,13 +1158,15 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     const uint32_t net = blockIdx.y;
     if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
     __shared__ uint4 fired[FIRED_LDS_COLS / 16];
+    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
     const NetDev d = m.nets[net];
     if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
     const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t gl = col < d.ncol ? d.conn[col * NSLOT + (lane_id() & 31u)] : 0u;
+    const uint32_t* in = (t & 1u) ? d.spk[1] : d.spk[0];  // no dynamic index into the local copy
+    const DriveIn di = drive_inputs(d, in, min(col, d.ncol - 1u));
     const uint8_t* f = stage_fired(d, fired);
     if (col >= d.ncol) return;
-    drive_col(d, P, (t & 1u) ? d.spk[1] : d.spk[0], col, gl, f);  // no dynamic index into the local copy
+    drive_col(d, P, in, col, di, f, items[threadIdx.x >> 6]);
 }
 
 __global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
@@ -1134,7 +1266,7 @@ inline uint32_t blocks_for(size_t n, uint32_t per) { return (uint32_t)((n + per 
 
 // ============================================================== launchers
 hipError_t launch_init(const NetDev& d, uint32_t nand, uint32_t seed, hipStream_t s) {
-    const uint32_t planeU4 = d.ncol * NSLOT * TILE_U4;
+    const uint32_t planeU4 = d.ncol * NSLOT * (TILE_WORDS / 4u);  // one thread per reference uint4
     k_init_w<<<blocks_for(planeU4, 256), 256, 0, s>>>(d.W0, d.W1, d.W2, d.W3, nand, seed, planeU4);
     const uint32_t nconn = d.ncol * NSLOT;
     k_init_conn<<<blocks_for(nconn, 256), 256, 0, s>>>(d.conn, d.ncol / COLS_PER_GRP, d.incols / COLS_PER_GRP,
@@ -1187,8 +1319,8 @@ hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn,
 }
 
 hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s) {
-    const size_t n = (size_t)d.ncol * NSLOT * TILE_U4;
-    k_wstat<<<2048, 256, 0, s>>>(d.W3, n, out);
+    const size_t n = (size_t)d.ncol * NSLOT * (TILE_WORDS / 4u);
+    k_wstat<<<2048, 256, 0, s>>>(reinterpret_cast<const uint4*>(d.W3), n, out);
     return hipGetLastError();
 }
 

@@ -44,21 +44,23 @@ def train_presentations(net, start, count, T, tc, nstim, outputs=False):
         net.present(p, 1.0, T, tc, True, False, 1000 + p, outputs=outputs)
 
 
-def kernel_bytes(ncol, steps, prop_tiles, learn_tiles):
+def kernel_bytes(ncol, steps, prop_lines, learn_lines):
     """Algorithmic HBM bytes per kernel over `steps` learning timesteps.
 
-    propagation: 1 KiB of W3 per tile read + conn (128 B/column) + bias read
+    The synapse planes are word-major tiles, so a kernel touches only the
+    128-byte lines of presynaptic columns that spiked (device line counters).
+    propagation: 128 B per W3 line read + conn (128 B/column) + bias read
     (4 B/neuron, learning uses the bias) + acc and bbits writes (8 B/neuron)
-    + drive (4 B/column).
+    + drive (4 B/column) + fired-column masks (32 B/column).
     topk: three passes over the column drives (4 B/column each).
     emit: acc read (4 B/neuron) + bias read-modify-write (8 B/neuron) + drive
     read, spike, history and margin writes (16 B/column).
-    plasticity: 8 KiB per rewritten tile (4 planes read + written)."""
+    plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read + written)."""
     n = ncol * 32
-    prop = 1024 * prop_tiles + steps * (128 * ncol + 4 * n + 8 * n + 4 * ncol)
+    prop = 128 * prop_lines + steps * (128 * ncol + 4 * n + 8 * n + 4 * ncol + 32 * ncol)
     topk = steps * 3 * 4 * ncol
     emit = steps * (4 * n + 8 * n + 16 * ncol)
-    plast = 8192 * learn_tiles
+    plast = 1024 * learn_lines
     return {"propagation": prop, "topk": topk, "emit": emit, "plasticity": plast}
 
 
@@ -77,6 +79,7 @@ def gpu_bench(args):
     train_presentations(net, args.warmup, args.steps, T, tc, nstim)
     net.sync()
     dt = time.perf_counter() - t0
+    lines = net.line_counters()
     prop, learn = net.work_counters(reset=True)
     net.close()
 
@@ -93,7 +96,7 @@ def gpu_bench(args):
     twin.close()
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
-    kb = kernel_bytes(ncol, steps, prop, learn)
+    kb = kernel_bytes(ncol, steps, lines[0], lines[1])
     names = ["propagation", "topk", "emit", "plasticity"]
     kern = {}
     for i, nm in enumerate(names):

@@ -127,6 +127,11 @@ axon_status axon_stim_cols(const axon_net* n, uint32_t p, uint32_t* out);
  * by plasticity, accumulated since creation or the last reset.  Each tile is
  * 8192 synapses (32 neurons x 256 presynaptic bits). */
 axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset);
+/* No reference counterpart: the HBM lines behind those tiles.  out[0]: 128-byte
+ * W3 lines propagation read (one per spiking presynaptic column of an active
+ * tile), out[1]: word lines plasticity rewrote (each 4 planes x 128 bytes read
+ * and written).  Reset clears these and the work counters together. */
+axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t reset);
 /* enable = 1: presentations run eagerly with HIP events around every kernel
  * and wait for the device before returning (an AXON_PRESENT_ASYNC
  * presentation still skips its side statistics and outputs);
