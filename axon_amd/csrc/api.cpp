@@ -349,8 +349,8 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
     *err = e;
 }
 
-// Same launches as enqueue_steps, each bracketed by events so the three
-// kernels' device durations can be summed after the presentation.
+// Same launches as enqueue_steps, each carrying start / end timing events,
+// so the four kernels' device durations can be summed after the presentation.
 static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn,
                                          bool stats) {
     const size_t need = (size_t)8 * std::max<uint32_t>(T, 1);
@@ -373,19 +373,11 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
                   : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
         hipEvent_t* ev = &n->prof_ev[(size_t)8 * t];
         plast[t] = learn && P.phase != 3u;
-        if ((e = hipEventRecord(ev[0], n->stream)) != hipSuccess) break;
-        if ((e = axon::launch_drive(n->d, P, t, n->stream)) != hipSuccess) break;
-        if ((e = hipEventRecord(ev[1], n->stream)) != hipSuccess) break;
-        if ((e = hipEventRecord(ev[2], n->stream)) != hipSuccess) break;
-        if ((e = axon::launch_topk(n->d, P, n->stream)) != hipSuccess) break;
-        if ((e = hipEventRecord(ev[3], n->stream)) != hipSuccess) break;
-        if ((e = hipEventRecord(ev[4], n->stream)) != hipSuccess) break;
-        if ((e = axon::launch_emit(n->d, P, t, learn, stats, n->stream)) != hipSuccess) break;
-        if ((e = hipEventRecord(ev[5], n->stream)) != hipSuccess) break;
+        if ((e = axon::launch_drive(n->d, P, t, n->stream, ev[0], ev[1])) != hipSuccess) break;
+        if ((e = axon::launch_topk(n->d, P, n->stream, ev[2], ev[3])) != hipSuccess) break;
+        if ((e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, ev[4], ev[5])) != hipSuccess) break;
         if (!plast[t]) continue;
-        if ((e = hipEventRecord(ev[6], n->stream)) != hipSuccess) break;
-        if ((e = axon::launch_learn(n->d, P, t, stats, n->stream)) != hipSuccess) break;
-        e = hipEventRecord(ev[7], n->stream);
+        e = axon::launch_learn(n->d, P, t, stats, n->stream, ev[6], ev[7]);
     }
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;

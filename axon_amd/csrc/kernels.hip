@@ -10,6 +10,7 @@
 //
 // All arithmetic is u32 integer with the reference's wrap-around semantics.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 
@@ -1287,35 +1288,43 @@ hipError_t launch_clear(const NetDev& d, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s) {
+// With timing events, the launch carries them in its dispatch packet
+// (hipExtLaunchKernel), so they bracket the kernel itself and not the queue
+// traffic around it.
+template <class F, class... A>
+static hipError_t launch(F k, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... a) {
+    if (e0) hipExtLaunchKernelGGL(k, g, b, 0, s, e0, e1, 0, a...);
+    else hipLaunchKernelGGL(k, g, b, 0, s, a...);
+    return hipGetLastError();
+}
+
+hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
-    k_drive<<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, P, in);
-    return hipGetLastError();
+    return launch(k_drive, dim3(blocks_for(d.ncol, 4)), dim3(256), s, e0, e1, d, P, in);
 }
 
-hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s) {
-    k_topk<<<1, 1024, 0, s>>>(d, P);
-    return hipGetLastError();
+hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    return launch(k_topk, dim3(1), dim3(1024), s, e0, e1, d, P);
 }
 
-hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s) {
+hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    const uint32_t g = std::min<uint32_t>(d.ncol, 512u);  // grid-stride over the listed columns / tiles
-    if (stats) k_learn<true><<<g, 256, 0, s>>>(d, P, in, out);
-    else k_learn_tiles<<<g, 256, 0, s>>>(d, P, in);
-    return hipGetLastError();
+    const dim3 g(std::min<uint32_t>(d.ncol, 512u));  // grid-stride over the listed columns / tiles
+    if (stats) return launch(k_learn<true>, g, dim3(256), s, e0, e1, d, P, in, out);
+    return launch(k_learn_tiles, g, dim3(256), s, e0, e1, d, P, in);
 }
 
-hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s) {
+hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
+                       hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
     uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    const uint32_t g = blocks_for(d.ncol, 4);
-    if (learn && stats) k_emit<true, true><<<g, 256, 0, s>>>(d, P, in, out);
-    else if (learn) k_emit<true, false><<<g, 256, 0, s>>>(d, P, in, out);
-    else if (stats) k_emit<false, true><<<g, 256, 0, s>>>(d, P, in, out);
-    else k_emit<false, false><<<g, 256, 0, s>>>(d, P, in, out);
-    return hipGetLastError();
+    const dim3 g(blocks_for(d.ncol, 4)), b(256);
+    if (learn && stats) return launch(k_emit<true, true>, g, b, s, e0, e1, d, P, in, out);
+    if (learn) return launch(k_emit<true, false>, g, b, s, e0, e1, d, P, in, out);
+    if (stats) return launch(k_emit<false, true>, g, b, s, e0, e1, d, P, in, out);
+    return launch(k_emit<false, false>, g, b, s, e0, e1, d, P, in, out);
 }
 
 hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s) {

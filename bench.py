@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "synaptic events/s (GSyn/s) & % HBM peak, batched vs plain; p99 recall latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per propagation launch of this workload, measured with rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (means over all propagation dispatches of the
+# Configuration C training run), each scaled by what the counter reads for a
+# 256 MiB device copy (x2.000 and x1.000); profiles/k_drive_c1_traffic.md.
+MEASURED_TRAFFIC = {"propagation": 6592026}
 SYN_PER_TILE = 8192
 CONFIG_C = ("nand=4 target=64 k=24 dthr=1 mode=1 inbr=4 stimcols=64 contrast=1 margingate=0 "
             "ltdrep=1 ltdevery=2 biasup=73 biasdown=1 T=10 clampsteps=5").split()
@@ -116,7 +121,8 @@ def gpu_bench(args):
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(dn),
                      "bytes_per_launch": round(kern[dn]["bytes_per_launch"]),
                      "avg_launch_us": round(kern[dn]["avg_us"], 3)},
     }
