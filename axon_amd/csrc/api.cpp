@@ -999,26 +999,38 @@ extern "C" axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint3
 }
 
 // ================================================================== many networks
+// Networks are trained in up to MULTI_GROUPS lock-step groups, one stream
+// each: networks with the same queue length share a group (merged down to the
+// limit, shortest first), so a group's launches never carry idle networks for
+// long, and the groups' kernels overlap on the device (one group's
+// latency-bound top-K runs beside another's propagation).
+constexpr uint32_t MULTI_GROUPS = 4;
+
+struct MultiGroup {
+    axon::MultiDev m{};
+    hipGraphExec_t graph = nullptr;
+    size_t rounds = 0;
+    uint32_t stream = 0;
+};
+
 struct axon_multi {
     std::vector<axon_net*> nets;
-    hipStream_t stream = nullptr;
-    axon::MultiDev m{};
-    std::vector<void*> allocs;
+    std::vector<NetDev> devs;      // host copies of the networks' descriptors
+    std::vector<uint64_t> tabs;    // device clamp-pattern tables, one per network
+    hipStream_t streams[MULTI_GROUPS] = {};
+    std::vector<void*> allocs;     // clamp tables (lifetime of the batch)
     uint32_t ncol = 0, nstats = 0;
     Params P{};
     uint32_t T = 0, trainClamp = 0, contrast = 0, ltdEvery = 1;
-    hipGraphExec_t graph = nullptr;
-    uint32_t* d_sched = nullptr;
-    size_t sched_rounds = 0;
 };
 
 extern "C" void axon_multi_destroy(axon_multi* b) {
     if (!b) return;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    if (b->d_sched) (void)hipFree(b->d_sched);
+    for (hipStream_t s : b->streams)
+        if (s) (void)hipStreamSynchronize(s);
     for (void* p : b->allocs) (void)hipFree(p);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (hipStream_t s : b->streams)
+        if (s) (void)hipStreamDestroy(s);
     delete b;
 }
 
@@ -1049,52 +1061,35 @@ extern "C" axon_status axon_multi_create(axon_net* const* nets, uint32_t nnet, a
     b->contrast = c0.contrast;
     b->ltdEvery = nets[0]->ltdEvery;
     auto bail = [&](axon_status s) { axon_multi_destroy(b); return s; };
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(AXON_ERR_DEVICE, "hipStreamCreate failed"));
-    auto alloc = [&](void** p, size_t bytes) -> axon_status {
-        hipError_t e = hipMalloc(p, bytes);
-        if (e != hipSuccess) return fail(AXON_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-        b->allocs.push_back(*p);
-        return AXON_OK;
-    };
-    void* p = nullptr;
-    axon_status st;
-    if ((st = alloc(&p, sizeof(NetDev) * nnet)) != AXON_OK) return bail(st);
-    b->m.nets = (const NetDev*)p;
-    if ((st = alloc(&p, 4)) != AXON_OK) return bail(st);
-    b->m.round = (uint32_t*)p;
-    if ((st = alloc(&p, 8 * nnet)) != AXON_OK) return bail(st);
-    b->m.stimTabs = (const uint64_t*)p;
-    b->m.nnet = nnet;
-    std::vector<NetDev> devs(nnet);
-    std::vector<uint64_t> tabs(nnet);
+    for (hipStream_t& st : b->streams)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(AXON_ERR_DEVICE, "hipStreamCreate failed"));
+    b->devs.resize(nnet);
+    b->tabs.resize(nnet);
     for (uint32_t i = 0; i < nnet; ++i) {
         axon_net* n = nets[i];
         if (hipStreamSynchronize(n->stream) != hipSuccess) return bail(fail(AXON_ERR_DEVICE, "net stream"));
-        devs[i] = n->d;
+        b->devs[i] = n->d;
         // clamp patterns at keep = 1.0: every stimulus column of the pattern (loadStim, mm:800-804)
         std::vector<uint32_t> tab((size_t)n->stimCols.size() * b->ncol, 0u);
         for (size_t q = 0; q < n->stimCols.size(); ++q)
             for (uint32_t col : n->stimCols[q]) tab[q * b->ncol + col] = 0xFFFFFFFFu;
-        if ((st = alloc(&p, tab.size() * 4)) != AXON_OK) return bail(st);
+        void* p = nullptr;
+        if (hipMalloc(&p, tab.size() * 4) != hipSuccess) return bail(fail(AXON_ERR_OUT_OF_MEMORY, "clamp table"));
+        b->allocs.push_back(p);
         if (hipMemcpy(p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(AXON_ERR_DEVICE, "stim table copy"));
-        tabs[i] = (uint64_t)(uintptr_t)p;
+        b->tabs[i] = (uint64_t)(uintptr_t)p;
     }
-    if (hipMemcpy((void*)b->m.nets, devs.data(), sizeof(NetDev) * nnet, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy((void*)b->m.stimTabs, tabs.data(), 8 * nnet, hipMemcpyHostToDevice) != hipSuccess)
-        return bail(fail(AXON_ERR_DEVICE, "descriptor copy"));
     *out = b;
     return AXON_OK;
 }
 
-// One presentation of every scheduled network (mm:837-884 with learn = 1),
-// followed by the round counter advance.
-static hipError_t multi_enqueue_round(axon_multi* b) {
-    const axon::MultiDev& m = b->m;
-    hipError_t e = axon::launch_multi_begin(m, b->ncol, b->nstats, b->stream);
+// One presentation of every scheduled network of a group (mm:837-884 with
+// learn = 1), followed by the group's round counter advance.
+static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev& m, hipStream_t s) {
+    hipError_t e = axon::launch_multi_begin(m, b->ncol, b->nstats, s);
     Params P = b->P;
-    P.k = b->P.k;
     P.facil = 0u;
     P.useBias = 1u;
     const uint32_t T = b->T, clampSteps = b->trainClamp;
@@ -1105,64 +1100,122 @@ static hipError_t multi_enqueue_round(axon_multi* b) {
         P.clamp = (t < clampSteps) ? 1u : 0u;
         P.depress = (t % 2u == 1u) ? 1u : 0u;
         P.phase = !b->contrast ? 0u : (t < clampSteps) ? 1u : (((t - clampSteps) % b->ltdEvery == 0u) ? 2u : 3u);
-        e = axon::launch_multi_drive(m, P, t, b->ncol, b->stream);
-        if (e == hipSuccess) e = axon::launch_multi_topk(m, P, b->stream);
-        if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, b->stream);
-        if (e == hipSuccess && P.phase != 3u) e = axon::launch_multi_learn(m, P, t, b->ncol, b->stream);
+        e = axon::launch_multi_drive(m, P, t, b->ncol, s);
+        if (e == hipSuccess) e = axon::launch_multi_topk(m, P, s);
+        if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, s);
+        if (e == hipSuccess && P.phase != 3u) e = axon::launch_multi_learn(m, P, t, b->ncol, s);
     }
-    if (e == hipSuccess) e = axon::launch_multi_end(m, b->stream);
+    if (e == hipSuccess) e = axon::launch_multi_end(m, s);
     return e;
 }
 
 // Trains every network with the reference protocol: epochs[i] x stim_i
 // presentations (pattern k mod stim_i, keep 1.0; mm:1009-1010), then, when
 // extra = 1, the additional presentation of pattern 0 (mm:1033).  Network i's
-// work queue holds its own presentations; all networks advance one
-// presentation per round and drop out when their queue is empty.
+// work queue holds its own presentations; within a group all networks
+// advance one presentation per round and drop out when their queue is empty.
 extern "C" axon_status axon_multi_train(axon_multi* b, const uint32_t* epochs, uint32_t extra, double* ms) {
     if (!b || !epochs) return fail(AXON_ERR_INVALID_ARG, "null argument");
-    const uint32_t nnet = b->m.nnet;
+    const uint32_t nnet = (uint32_t)b->nets.size();
     std::vector<uint32_t> len(nnet);
-    size_t rounds = 0;
-    for (uint32_t i = 0; i < nnet; ++i) {
+    for (uint32_t i = 0; i < nnet; ++i)
         len[i] = epochs[i] * (uint32_t)b->nets[i]->stimCols.size() + (extra ? 1u : 0u);
-        rounds = std::max<size_t>(rounds, len[i]);
-    }
-    std::vector<uint32_t> sched(rounds * nnet, 0xFFFFFFFFu);
-    for (uint32_t i = 0; i < nnet; ++i) {
-        const uint32_t ns = (uint32_t)b->nets[i]->stimCols.size(), body = len[i] - (extra ? 1u : 0u);
-        for (uint32_t r = 0; r < len[i]; ++r) sched[(size_t)r * nnet + i] = r < body ? r % ns : 0u;
+    // groups: equal queue lengths together, longest first; merge the shortest down to the limit
+    std::vector<std::vector<uint32_t>> members;
+    {
+        std::vector<uint32_t> order(nnet);
+        for (uint32_t i = 0; i < nnet; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return len[x] > len[y]; });
+        for (uint32_t i : order) {
+            if (len[i] == 0u) continue;
+            if (members.empty() || len[members.back()[0]] != len[i]) members.emplace_back();
+            members.back().push_back(i);
+        }
+        while (members.size() > MULTI_GROUPS) {
+            auto last = members.back();
+            members.pop_back();
+            members.back().insert(members.back().end(), last.begin(), last.end());
+        }
     }
     HIPCHECK(hipSetDevice(b->nets[0]->device));
     for (axon_net* n : b->nets) HIPCHECK(hipStreamSynchronize(n->stream));
-    if (rounds > b->sched_rounds) {
-        if (b->d_sched) (void)hipFree(b->d_sched);
-        b->d_sched = nullptr;
-        HIPCHECK(hipMalloc(&b->d_sched, rounds * nnet * 4));
-        b->sched_rounds = rounds;
-    }
-    HIPCHECK(hipMemcpy(b->d_sched, sched.data(), rounds * nnet * 4, hipMemcpyHostToDevice));
-    b->m.sched = b->d_sched;
-    HIPCHECK(hipMemset(b->m.round, 0, 4));
-    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
-    hipGraph_t g = nullptr;
-    HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = multi_enqueue_round(b);
-    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "multi capture: %s", hipGetErrorString(e));
-    if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
-    hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e3 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+
+    std::vector<MultiGroup> groups(members.size());
+    std::vector<void*> tmp;  // this call's descriptors, schedules and counters
+    auto release = [&]() {
+        for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
+        for (MultiGroup& g : groups)
+            if (g.graph) (void)hipGraphExecDestroy(g.graph);
+        for (void* p : tmp) (void)hipFree(p);
+    };
+    auto dalloc = [&](size_t bytes, const void* src) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        if (src && hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        if (!src && hipMemset(p, 0, bytes) != hipSuccess) return nullptr;
+        return p;
+    };
     const bool graphs = b->nets[0]->use_graphs;
-    HIPCHECK(hipStreamSynchronize(b->stream));
-    const auto t0 = std::chrono::steady_clock::now();
-    for (size_t r = 0; r < rounds; ++r) {
-        if (graphs) HIPCHECK(hipGraphLaunch(b->graph, b->stream));
-        else if ((e = multi_enqueue_round(b)) != hipSuccess) return fail(AXON_ERR_DEVICE, "%s", hipGetErrorString(e));
+    size_t maxRounds = 0;
+    for (size_t gi = 0; gi < members.size(); ++gi) {
+        MultiGroup& g = groups[gi];
+        const std::vector<uint32_t>& mem = members[gi];
+        const uint32_t k = (uint32_t)mem.size();
+        for (uint32_t i : mem) g.rounds = std::max<size_t>(g.rounds, len[i]);
+        maxRounds = std::max(maxRounds, g.rounds);
+        std::vector<NetDev> devs(k);
+        std::vector<uint64_t> tabs(k);
+        std::vector<uint32_t> sched(g.rounds * k, 0xFFFFFFFFu);
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t i = mem[j];
+            devs[j] = b->devs[i];
+            tabs[j] = b->tabs[i];
+            const uint32_t ns = (uint32_t)b->nets[i]->stimCols.size(), body = len[i] - (extra ? 1u : 0u);
+            for (uint32_t r = 0; r < len[i]; ++r) sched[(size_t)r * k + j] = r < body ? r % ns : 0u;
+        }
+        g.m.nets = (const NetDev*)dalloc(sizeof(NetDev) * k, devs.data());
+        g.m.stimTabs = (const uint64_t*)dalloc(8 * k, tabs.data());
+        g.m.sched = (const uint32_t*)dalloc(sched.size() * 4, sched.data());
+        g.m.round = (uint32_t*)dalloc(4, nullptr);
+        g.m.nnet = k;
+        g.stream = (uint32_t)gi;
+        if (!g.m.nets || !g.m.stimTabs || !g.m.sched || !g.m.round) {
+            release();
+            return fail(AXON_ERR_OUT_OF_MEMORY, "multi-network descriptors");
+        }
+        if (!graphs) continue;
+        hipStream_t s = b->streams[gi];
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            release();
+            return fail(AXON_ERR_DEVICE, "hipStreamBeginCapture failed");
+        }
+        hipError_t e = multi_enqueue_round(b, g.m, s);
+        hipError_t e2 = hipStreamEndCapture(s, &gr);
+        hipError_t e3 = (e == hipSuccess && e2 == hipSuccess) ? hipGraphInstantiate(&g.graph, gr, nullptr, nullptr, 0)
+                                                             : hipErrorUnknown;
+        if (gr) (void)hipGraphDestroy(gr);
+        if (e3 != hipSuccess) {
+            release();
+            return fail(AXON_ERR_DEVICE, "multi capture: %s / %s", hipGetErrorString(e), hipGetErrorString(e2));
+        }
     }
-    HIPCHECK(hipStreamSynchronize(b->stream));
-    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipSuccess;
+    for (size_t r = 0; r < maxRounds && e == hipSuccess; ++r)
+        for (MultiGroup& g : groups) {
+            if (r >= g.rounds) continue;
+            hipStream_t s = b->streams[g.stream];
+            e = graphs ? hipGraphLaunch(g.graph, s) : multi_enqueue_round(b, g.m, s);
+            if (e != hipSuccess) break;
+        }
+    for (hipStream_t s : b->streams)
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const auto t1 = std::chrono::steady_clock::now();
+    release();
+    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "multi train: %s", hipGetErrorString(e));
+    if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return AXON_OK;
 }
 

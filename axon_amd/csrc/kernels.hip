@@ -596,8 +596,10 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
 }
 
 // Every global value a column's emit can need is loaded up front, so the
-// wave waits for memory once rather than once per use.
-template <bool LEARN, bool STATS>
+// wave waits for memory once rather than once per use.  EAGER = false (many
+// networks at once, where bytes matter more than a wave's latency) loads the
+// plasticity inputs only in the columns that learn.
+template <bool LEARN, bool STATS, bool EAGER = true>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t* __restrict__ spkOut, uint32_t col0) {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
@@ -612,11 +614,12 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
     const uint32_t dv = d.drive[col];
     const uint32_t v = d.acc[gid];
     const uint32_t stimW = clampCol ? d.stim[col] : 0u;
-    const uint32_t bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
-    const uint32_t cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
-    const uint32_t fwb = LEARN ? d.firedW[col * NSLOT + n] : 0u;
-    const uint32_t band = LEARN ? d.tileBand[col * NSLOT + n] : 0u;
-    const uint2 lc = LEARN ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
+    constexpr bool EL = LEARN && EAGER;
+    uint32_t bbv = (EL || STATS) ? d.bbits[gid] : 0u;
+    uint32_t cn = EL ? d.conn[col * NSLOT + n] : 0u;
+    uint32_t fwb = EL ? d.firedW[col * NSLOT + n] : 0u;
+    uint32_t band = EL ? d.tileBand[col * NSLOT + n] : 0u;
+    uint2 lc = EL ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
     const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
     const bool gatePass = dv >= th;
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
@@ -665,7 +668,16 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
         d.bias[gid] = (uint32_t)max(0, nb);
     }
     const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
-    if (learnCol) list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn, fwb, band, lc);
+    if (learnCol) {
+        if (!EAGER) {
+            if (!STATS) bbv = d.bbits[gid];
+            cn = d.conn[col * NSLOT + n];
+            fwb = d.firedW[col * NSLOT + n];
+            band = d.tileBand[col * NSLOT + n];
+            lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
+        }
+        list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn, fwb, band, lc);
+    }
 }
 
 // One voted tile of k_learn (mm:436-495).  Lane (h, n) owns words 4h..4h+3
@@ -1184,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
     const NetDev d = m.nets[net];
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    emit_cols<LEARN, false>(d, P, (t & 1u) ? s1 : s0, (t & 1u) ? s0 : s1, blockIdx.x * 4u);
+    emit_cols<LEARN, false, false>(d, P, (t & 1u) ? s1 : s0, (t & 1u) ? s0 : s1, blockIdx.x * 4u);
 }
 
 __global__ __launch_bounds__(256) void k_mb_learn(MultiDev m, Params P, uint32_t t) {
