@@ -281,7 +281,9 @@ __device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t*
                    reinterpret_cast<const uint2*>(d.cntProp)[col]};
 }
 
-constexpr uint32_t ITEMS_PER_HALF = NSLOT * 8u;  // LDS item list capacity of a half-wave
+// LDS item list capacity of a half-wave: branches are dealt alternately to the
+// halves, so a half holds at most 16 branches of at most 8 items.
+constexpr uint32_t ITEMS_PER_HALF = (NSLOT / 2u) * 8u;
 
 __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t col, const DriveIn& in, const uint8_t* fired,
@@ -392,7 +394,9 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     }
 }
 
-constexpr uint32_t FIRED_LDS_COLS = 16384;  // columns whose fired bytes fit the LDS stage
+constexpr uint32_t FIRED_LDS_COLS = 16384;  // up to this many columns the fired bytes are staged in LDS
+// Dynamic LDS bytes of the fired-byte stage (0: read from global memory).
+inline uint32_t fired_lds_bytes(uint32_t ncol) { return ncol <= FIRED_LDS_COLS ? (ncol + 15u) / 16u * 16u : 0u; }
 
 // Stage the fired bytes of all columns in LDS while this wave's conn entries load.
 __device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds) {
@@ -404,7 +408,7 @@ __device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* ld
 }
 
 __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
-    __shared__ uint4 fired[FIRED_LDS_COLS / 16];
+    extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
     // the learning-column list of step t-1 has been consumed; free it for step t+1
     if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
@@ -1170,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
 __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t) {
     const uint32_t net = blockIdx.y;
     if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
-    __shared__ uint4 fired[FIRED_LDS_COLS / 16];
+    extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
     const NetDev d = m.nets[net];
     if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
@@ -1304,19 +1308,19 @@ hipError_t launch_clear(const NetDev& d, hipStream_t s) {
 // (hipExtLaunchKernel), so they bracket the kernel itself and not the queue
 // traffic around it.
 template <class F, class... A>
-static hipError_t launch(F k, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... a) {
-    if (e0) hipExtLaunchKernelGGL(k, g, b, 0, s, e0, e1, 0, a...);
-    else hipLaunchKernelGGL(k, g, b, 0, s, a...);
+static hipError_t launch(F k, dim3 g, dim3 b, uint32_t lds, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... a) {
+    if (e0) hipExtLaunchKernelGGL(k, g, b, lds, s, e0, e1, 0, a...);
+    else hipLaunchKernelGGL(k, g, b, lds, s, a...);
     return hipGetLastError();
 }
 
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
-    return launch(k_drive, dim3(blocks_for(d.ncol, 4)), dim3(256), s, e0, e1, d, P, in);
+    return launch(k_drive, dim3(blocks_for(d.ncol, 4)), dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
 }
 
 hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    return launch(k_topk, dim3(1), dim3(1024), s, e0, e1, d, P);
+    return launch(k_topk, dim3(1), dim3(1024), 0u, s, e0, e1, d, P);
 }
 
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s, hipEvent_t e0,
@@ -1324,8 +1328,8 @@ hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* out = d.spk[(t & 1u) ^ 1u];
     const dim3 g(std::min<uint32_t>(d.ncol, 512u));  // grid-stride over the listed columns / tiles
-    if (stats) return launch(k_learn<true>, g, dim3(256), s, e0, e1, d, P, in, out);
-    return launch(k_learn_tiles, g, dim3(256), s, e0, e1, d, P, in);
+    if (stats) return launch(k_learn<true>, g, dim3(256), 0u, s, e0, e1, d, P, in, out);
+    return launch(k_learn_tiles, g, dim3(256), 0u, s, e0, e1, d, P, in);
 }
 
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
@@ -1333,10 +1337,10 @@ hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn,
     const uint32_t* in = d.spk[t & 1u];
     uint32_t* out = d.spk[(t & 1u) ^ 1u];
     const dim3 g(blocks_for(d.ncol, 4)), b(256);
-    if (learn && stats) return launch(k_emit<true, true>, g, b, s, e0, e1, d, P, in, out);
-    if (learn) return launch(k_emit<true, false>, g, b, s, e0, e1, d, P, in, out);
-    if (stats) return launch(k_emit<false, true>, g, b, s, e0, e1, d, P, in, out);
-    return launch(k_emit<false, false>, g, b, s, e0, e1, d, P, in, out);
+    if (learn && stats) return launch(k_emit<true, true>, g, b, 0u, s, e0, e1, d, P, in, out);
+    if (learn) return launch(k_emit<true, false>, g, b, 0u, s, e0, e1, d, P, in, out);
+    if (stats) return launch(k_emit<false, true>, g, b, 0u, s, e0, e1, d, P, in, out);
+    return launch(k_emit<false, false>, g, b, 0u, s, e0, e1, d, P, in, out);
 }
 
 hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s) {
@@ -1423,7 +1427,7 @@ hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats,
 }
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
     dim3 g((ncol + 3u) / 4u, m.nnet);
-    k_mb_drive<<<g, 256, 0, s>>>(m, P, t);
+    k_mb_drive<<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s) {
