@@ -64,9 +64,11 @@ struct NetDev {
     // {column * 32 + slot, presynaptic group, learning-neuron mask, flags};
     // flags = fired-column mask of the group | 0x100 when all 8 words of
     // the tile must be rewritten).
-    uint32_t* learnList; // [2][ncol]
-    uint4* tileList;     // [2][ncol * 32]
-    uint32_t* learnCnt;  // [2] entries in each list (reset by the step after next's propagation)
+    // Each parity holds 8 lists (column c in list c % 8) so that columns take
+    // list slots on 8 different counters.
+    uint32_t* learnList; // [2][8][ncol / 8]
+    uint4* tileList;     // [2][8][ncol / 8 * 32]
+    uint32_t* learnCnt;  // [2][8] list lengths, 32 words apart (reset by the step after next's propagation)
     uint32_t ncol, incols, inbr, topkpar;
     uint32_t nstats, ndlog, maxt, pad;
 };

@@ -57,6 +57,14 @@ __device__ __forceinline__ uint32_t half_or(uint32_t x) {
     return x;
 }
 
+// Plasticity work lists: per step parity, NLISTS lists (column c goes to list
+// c % NLISTS) so the list-slot atomics of different columns hit different
+// counters; each counter sits on its own 128-byte line.
+constexpr uint32_t NLISTS = 8;
+__device__ __forceinline__ uint32_t* list_count(const NetDev& d, uint32_t par, uint32_t l) {
+    return d.learnCnt + (par * NLISTS + l) * 32u;
+}
+
 // Word-major tile addressing (see TILE_WORDS): word w of neuron n of a tile.
 __device__ __forceinline__ size_t word_idx(uint32_t tile, uint32_t w, uint32_t n) {
     return ((size_t)tile * 8u + w) * 32u + n;
@@ -255,7 +263,7 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t* __restrict__ a, uint32_
                                                uint32_t* __restrict__ lcnt) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < ncol) { a[i] = 0u; b[i] = 0u; nz[i] = 0u; }
-    if (i < 2u) lcnt[i] = 0u;
+    if (i < 2u * NLISTS) lcnt[i * 32u] = 0u;  // the plasticity list counters (list_count)
     if (i < nst) st[i] = 0u;
 }
 
@@ -411,7 +419,7 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
     // the learning-column list of step t-1 has been consumed; free it for step t+1
-    if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
+    if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
     const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
     const DriveIn in = drive_inputs(d, spkIn, min(col, d.ncol - 1u));
     const uint8_t* f = stage_fired(d, fired);
@@ -572,7 +580,8 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
         const uint32_t lines = wave_sum(v ? (full_tile_update(P, band) ? 8u : popc(fwb)) : 0u);
         if (lane == 0u) {
             reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
-            d.learnList[par * d.ncol + atomicAdd(&d.learnCnt[par], 1u)] = col;
+            const uint32_t l = col % NLISTS, cap = d.ncol / NLISTS;
+            d.learnList[(par * NLISTS + l) * cap + atomicAdd(list_count(d, par, l), 1u)] = col;
         }
         return;
     }
@@ -587,14 +596,15 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
     const uint32_t lines = wave_sum(want != 0u ? (full ? 8u : popc(fwb)) : 0u);
     uint32_t base = 0u;
     if (lane == 0u) {
-        base = atomicAdd(&d.learnCnt[par], popc(voted));
+        base = atomicAdd(list_count(d, par, col % NLISTS), popc(voted));
         // only this column's wave writes its counters: a plain read-modify-write
         reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
     }
     base = __shfl(base, 0);
     if (want != 0u) {
         const uint32_t at = base + popc(voted & ((1u << lane) - 1u));
-        d.tileList[(size_t)par * d.ncol * NSLOT + at] =
+        const size_t cap = (size_t)d.ncol / NLISTS * NSLOT;  // a list's columns x 32 tiles
+        d.tileList[(par * NLISTS + col % NLISTS) * cap + at] =
             make_uint4(col * NSLOT + lane, cn, want, fwb | (full ? 0x100u : 0u));
     }
 }
@@ -728,9 +738,11 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
     const uint32_t par = P.step & 1u;
-    const uint32_t nlist = d.learnCnt[par];
-    for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
-        const uint32_t col = d.learnList[par * d.ncol + li];
+    const uint32_t l = blockIdx.x % NLISTS, cap = d.ncol / NLISTS;  // workgroup b serves list b % NLISTS
+    if (blockIdx.x / NLISTS >= gridDim.x / NLISTS) return;  // workgroup-uniform: no barrier is skipped by part of it
+    const uint32_t nlist = *list_count(d, par, l);
+    for (uint32_t li = blockIdx.x / NLISTS; li < nlist; li += gridDim.x / NLISTS) {
+        const uint32_t col = d.learnList[(par * NLISTS + l) * cap + li];
         const uint32_t b = spkOut[col];
         const bool post = ((b >> n) & 1u) != 0u;
         const uint32_t bb = post ? d.bbits[col * NPC + n] : 0u;
@@ -775,16 +787,19 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
     }
 }
 
-// k_learn over the voted-tile list: wave w of the grid rewrites tiles w,
-// w + nwaves, ...
+// k_learn over the voted-tile lists: wave w of the grid serves list w %
+// NLISTS and rewrites its tiles w / NLISTS, w / NLISTS + nwaves / NLISTS, ...
 __device__ __forceinline__ void learn_tiles(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
-                                            uint32_t w0, uint32_t nwaves) {
+                                            uint32_t wg, uint32_t nwg) {
     const uint32_t par = P.step & 1u;
-    const uint4* __restrict__ list = d.tileList + (size_t)par * d.ncol * NSLOT;
+    const uint32_t l = wg % NLISTS, w0 = wg / NLISTS, nwaves = nwg / NLISTS;
+    if (w0 >= nwaves) return;  // a grid that is not a multiple of NLISTS waves leaves these idle
+    const uint32_t cap = d.ncol / NLISTS * NSLOT;
+    const uint4* __restrict__ list = d.tileList + (size_t)(par * NLISTS + l) * cap;
     // the first entry is read together with the count, unconditionally (the
-    // index is clamped into the list's ncol * 32 entries; used only if w0 < count)
-    uint4 e = list[min(w0, d.ncol * NSLOT - 1u)];
-    const uint32_t cnt = d.learnCnt[par];
+    // index is clamped into the list; used only if w0 < count)
+    uint4 e = list[min(w0, cap - 1u)];
+    const uint32_t cnt = *list_count(d, par, l);
     asm volatile("" ::"v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w), "v"(cnt));  // both in flight before either is used
     for (uint32_t i = w0; i < cnt;) {
         uint32_t nLTP = 0, nLTD = 0;
@@ -1167,7 +1182,7 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
     const uint32_t* tab = reinterpret_cast<const uint32_t*>(m.stimTabs[net]) + (size_t)p * d.ncol;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < d.ncol) { d.stim[i] = tab[i]; d.spk[0][i] = 0u; d.spk[1][i] = 0u; d.colNZ[i] = 0u; }
-    if (i < 2u) d.learnCnt[i] = 0u;
+    if (i < 2u * NLISTS) *list_count(d, i / NLISTS, i % NLISTS) = 0u;
     if (i < d.nstats) d.stats[i] = 0u;
 }
 
@@ -1177,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
     const NetDev d = m.nets[net];
-    if (blockIdx.x == 0u && threadIdx.x == 0u) d.learnCnt[(P.step & 1u) ^ 1u] = 0u;
+    if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
     const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t* in = (t & 1u) ? d.spk[1] : d.spk[0];  // no dynamic index into the local copy
     const DriveIn di = drive_inputs(d, in, min(col, d.ncol - 1u));
