@@ -293,6 +293,10 @@ __device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t*
 // halves, so a half holds at most 16 branches of at most 8 items.
 constexpr uint32_t ITEMS_PER_HALF = (NSLOT / 2u) * 8u;
 
+// Propagation (k_drive, mm:150-190) for one column per wave, the form for a
+// single network, where a timestep is latency-bound: the column's active
+// branches are dealt alternately to the two half-waves, halving the items a
+// half works through.
 __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t col, const DriveIn& in, const uint8_t* fired,
                                           uint32_t* items) {
@@ -399,6 +403,117 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     if (lane == 0u) {
         d.drive[col] = drv;
         reinterpret_cast<uint2*>(d.cntProp)[col] = make_uint2(in.cnt.x + nActive, in.cnt.y + nLines);
+    }
+}
+
+constexpr uint32_t ITEMS_PER_COL = NSLOT * 8u;  // LDS item list capacity of a column
+
+// Propagation (k_drive, mm:150-190) for two columns per wave, the form for
+// many networks at once, where throughput counts: half h of the wave is
+// column colA + h, and lane n of a half is neuron n (and, for the per-branch
+// lookups, branch n); no work is duplicated between the halves.
+//
+// Work items are (branch, spiking presynaptic column) pairs, one 128-byte W3
+// line each.  Each half lists its column's items in branch order in LDS (lane
+// n writes branch n's items at a scanned offset); one dword load instruction
+// then serves one item of each half, and a branch's count c, summed over its
+// items, completes inside the half.  Item word: branch | word << 5 |
+// group << 8 | 1 << 31 on the branch's last item.
+__device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                           uint32_t colA, const DriveIn& in, const uint8_t* fired,
+                                           uint32_t* items) {
+    const uint32_t lane = lane_id();
+    const uint32_t h = lane >> 5, n = lane & 31u;
+    const bool hi = h != 0u;
+    const uint32_t col = colA + h;
+    const uint32_t gid = col * NPC + n;
+    const uint32_t gl = in.gl;
+    const uint32_t biasw = (P.useBias != 0u) ? in.biasw : 0u;
+    const uint32_t prev = (P.facil != 0u) ? in.prev : 0u;
+
+    // which of branch n's eight presynaptic columns spiked
+    uint32_t fwl;
+    if (fired) {
+        fwl = byte_mask(*reinterpret_cast<const uint64_t*>(fired + 8u * gl));
+    } else {
+        const uint4 a = reinterpret_cast<const uint4*>(spkIn)[2u * gl];
+        const uint4 b = reinterpret_cast<const uint4*>(spkIn)[2u * gl + 1u];
+        fwl = (a.x ? 1u : 0u) | (a.y ? 2u : 0u) | (a.z ? 4u : 0u) | (a.w ? 8u : 0u) | (b.x ? 16u : 0u) |
+              (b.y ? 32u : 0u) | (b.z ? 64u : 0u) | (b.w ? 128u : 0u);
+    }
+    d.firedW[col * NSLOT + n] = (uint8_t)fwl;
+    const uint32_t mask = (uint32_t)(__ballot(fwl != 0u) >> (32u * h));  // this column's active branches
+    const uint32_t nActive = popc(mask);
+
+    const uint32_t ni = popc(fwl);
+    uint32_t sc = ni;  // inclusive scan of the item counts over the half's branches
+#pragma unroll
+    for (uint32_t off = 1; off < 32u; off <<= 1) {
+        const uint32_t t = __shfl_up(sc, off, 32);
+        if (n >= off) sc += t;
+    }
+    const uint32_t nA = __builtin_amdgcn_readlane(sc, 31), nB = __builtin_amdgcn_readlane(sc, 63);
+    uint32_t* L = items + h * ITEMS_PER_COL;  // written here, read back below by the same wave
+    if (fwl != 0u) {
+        uint32_t pos = sc - ni;
+        uint32_t m = fwl;
+        while (m) {
+            const uint32_t w = __builtin_ctz(m);
+            m &= m - 1u;
+            L[pos++] = n | (w << 5) | (gl << 8) | (m == 0u ? 0x80000000u : 0u);
+        }
+    }
+    const uint32_t myN = hi ? nB : nA;
+    // the two columns' 64 tiles of W3 (64 KiB) and the spike words, as range-checked resources
+    const uint32_t ubase = __builtin_amdgcn_readfirstlane(colA * NSLOT);
+    const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, 2u * NSLOT * TILE_WORDS * 4u);
+    const __amdgpu_buffer_rsrc_t sr = buf_rsrc(spkIn, d.ncol * 4u);
+    uint32_t acc = 0u, bb = 0u, c = 0u;
+    constexpr uint32_t DB = 8;  // items per half fetched per round trip
+    const uint32_t R = max(nA, nB);
+    for (uint32_t r = 0; r < R; r += DB) {
+        const uint32_t kn = min(R - r, DB);  // item places used this round (uniform)
+        const uint4 i0 = *reinterpret_cast<const uint4*>(L + r);
+        const uint4 i1 = *reinterpret_cast<const uint4*>(L + r + 4u);
+        const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+        uint32_t wv[DB], sv[DB];
+#pragma unroll
+        for (uint32_t k = 0; k < DB; ++k) {
+            wv[k] = 0u; sv[k] = 0u;
+            if (k < kn) {  // no load instructions for unused item places
+                const bool v = r + k < myN;
+                const uint32_t e = it[k];
+                const uint32_t br = e & 31u, w = (e >> 5) & 7u, g = (e >> 8) & 0x7FFFFFu;
+                wv[k] = buf_ld(w3, v ? (((h * NSLOT + br) * 8u + w) * 32u + n) * 4u : SKIP);
+                sv[k] = buf_ld(sr, v ? (8u * g + w) * 4u : SKIP);
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < DB; ++k) {
+            if (k < kn && r + k < myN) {
+                const uint32_t e = it[k];
+                c += popc(wv[k] & sv[k]);
+                if (e >> 31) {  // last item of the branch: its count is complete
+                    const bool nmda = c >= P.dthr;
+                    if (nmda) bb |= 1u << (e & 31u);
+                    if (P.mode == 0u) acc += c;
+                    else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
+                    else acc += nmda ? 1u : 0u;
+                    c = 0u;
+                }
+            }
+        }
+    }
+    if (P.facil != 0u && ((prev >> n) & 1u) != 0u) acc += P.facil;
+    const uint32_t bi = biasw >> P.biasShift;
+    acc = (acc > bi) ? (acc - bi) : 0u;
+    d.acc[gid] = acc;
+    d.bbits[gid] = bb;
+    const uint32_t drv = half_sum(acc);
+    if (n == 0u) {
+        d.drive[col] = drv;
+        // only this column's half writes its counters: a plain read-modify-write
+        reinterpret_cast<uint2*>(d.cntProp)[col] = make_uint2(in.cnt.x + nActive, in.cnt.y + myN);
     }
 }
 
@@ -1190,15 +1305,15 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     const uint32_t net = blockIdx.y;
     if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
-    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
+    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_COL];
     const NetDev d = m.nets[net];
     if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
-    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
     const uint32_t* in = (t & 1u) ? d.spk[1] : d.spk[0];  // no dynamic index into the local copy
-    const DriveIn di = drive_inputs(d, in, min(col, d.ncol - 1u));
+    const DriveIn di = drive_inputs(d, in, min(colA + (lane_id() >> 5), d.ncol - 1u));
     const uint8_t* f = stage_fired(d, fired);
-    if (col >= d.ncol) return;
-    drive_col(d, P, in, col, di, f, items[threadIdx.x >> 6]);
+    if (colA >= d.ncol) return;
+    drive_pair(d, P, in, colA, di, f, items[threadIdx.x >> 6]);
 }
 
 __global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
@@ -1441,7 +1556,7 @@ hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats,
     return hipGetLastError();
 }
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
-    dim3 g((ncol + 3u) / 4u, m.nnet);
+    dim3 g((ncol + 7u) / 8u, m.nnet);
     k_mb_drive<<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
     return hipGetLastError();
 }
