@@ -114,29 +114,31 @@ __device__ __forceinline__ uint4 group_words(const uint32_t* __restrict__ spk, u
 // rank k-1 / k (0 when that rank does not exist).  Here the m-th largest value
 // is found by a bitwise radix select over ballots, which gives the same sets:
 //   winners = {v > vk} plus the (k - #{v > vk}) lowest lanes with v == vk.
-__device__ __forceinline__ uint32_t kth_largest(uint32_t v, uint32_t m, int hb) {
+// h selects the 32-lane half that holds the column (its ballot bits).
+__device__ __forceinline__ uint32_t hball(bool p, uint32_t h) { return (uint32_t)(__ballot(p) >> (32u * h)); }
+__device__ __forceinline__ uint32_t kth_largest(uint32_t v, uint32_t m, int hb, uint32_t h) {
     uint32_t x = 0;
     for (int b = hb; b >= 0; --b) {
         uint32_t t = x | (1u << b);
-        if (popc(ballot32(v >= t)) >= m) x = t;
+        if (popc(hball(v >= t, h)) >= m) x = t;
     }
     return x;
 }
 struct Kwta { uint32_t ball, vk, vk1; };
-__device__ __forceinline__ Kwta kwta32(uint32_t v, uint32_t k, uint32_t n) {
+__device__ __forceinline__ Kwta kwta32(uint32_t v, uint32_t k, uint32_t n, uint32_t h = 0u) {
     const uint32_t ov = half_or(v);
     const int hb = ov ? 31 - __clz(ov) : -1;
     Kwta r;
-    r.vk = (k >= 1u && k <= 32u) ? kth_largest(v, k, hb) : 0u;
-    r.vk1 = (k <= 31u) ? kth_largest(v, k + 1u, hb) : 0u;
+    r.vk = (k >= 1u && k <= 32u) ? kth_largest(v, k, hb, h) : 0u;
+    r.vk1 = (k <= 31u) ? kth_largest(v, k + 1u, hb, h) : 0u;
     if (k == 0u) r.ball = 0u;
     else if (k >= 32u) r.ball = 0xFFFFFFFFu;
     else {
-        const uint32_t gt = ballot32(v > r.vk);
-        const uint32_t eq = ballot32(v == r.vk);
+        const uint32_t gt = hball(v > r.vk, h);
+        const uint32_t eq = hball(v == r.vk, h);
         const uint32_t need = k - popc(gt);
         const bool win = (v > r.vk) || (v == r.vk && popc(eq & ((1u << n) - 1u)) < need);
-        r.ball = ballot32(win);
+        r.ball = hball(win, h);
     }
     return r;
 }
@@ -725,10 +727,9 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
 }
 
 // Every global value a column's emit can need is loaded up front, so the
-// wave waits for memory once rather than once per use.  EAGER = false (many
-// networks at once, where bytes matter more than a wave's latency) loads the
-// plasticity inputs only in the columns that learn.
-template <bool LEARN, bool STATS, bool EAGER = true>
+// wave waits for memory once rather than once per use (a single network's
+// timestep is latency-bound; emit_pair is the form for many networks).
+template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t* __restrict__ spkOut, uint32_t col0) {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
@@ -743,12 +744,11 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
     const uint32_t dv = d.drive[col];
     const uint32_t v = d.acc[gid];
     const uint32_t stimW = clampCol ? d.stim[col] : 0u;
-    constexpr bool EL = LEARN && EAGER;
-    uint32_t bbv = (EL || STATS) ? d.bbits[gid] : 0u;
-    uint32_t cn = EL ? d.conn[col * NSLOT + n] : 0u;
-    uint32_t fwb = EL ? d.firedW[col * NSLOT + n] : 0u;
-    uint32_t band = EL ? d.tileBand[col * NSLOT + n] : 0u;
-    uint2 lc = EL ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
+    const uint32_t bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
+    const uint32_t cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
+    const uint32_t fwb = LEARN ? d.firedW[col * NSLOT + n] : 0u;
+    const uint32_t band = LEARN ? d.tileBand[col * NSLOT + n] : 0u;
+    const uint2 lc = LEARN ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
     const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
     const bool gatePass = dv >= th;
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
@@ -797,16 +797,7 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
         d.bias[gid] = (uint32_t)max(0, nb);
     }
     const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
-    if (learnCol) {
-        if (!EAGER) {
-            if (!STATS) bbv = d.bbits[gid];
-            cn = d.conn[col * NSLOT + n];
-            fwb = d.firedW[col * NSLOT + n];
-            band = d.tileBand[col * NSLOT + n];
-            lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
-        }
-        list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn, fwb, band, lc);
-    }
+    if (learnCol) list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn, fwb, band, lc);
 }
 
 // One voted tile of k_learn (mm:436-495).  Lane (h, n) owns words 4h..4h+3
@@ -1323,6 +1314,75 @@ __global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
     topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
+// k_emit + k_bias + the plasticity listing (as emit_cols) for two columns
+// per wave, the form for many networks at once: half h is column colA + h,
+// lane n of a half is neuron n (and branch n); the plasticity inputs are read
+// only in columns that learn.  No side statistics.
+template <bool LEARN>
+__device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint32_t* __restrict__ spkOut,
+                                          uint32_t colA) {
+    const uint32_t lane = lane_id(), h = lane >> 5, n = lane & 31u;
+    const uint32_t col = colA + h;
+    const uint32_t gid = col * NPC + n;
+    const bool clampCol = P.clamp != 0u && col < P.inCols;
+    const bool rec = col >= P.inCols;
+    const uint32_t th = max(d.theta[0], 1u);
+    const uint32_t dv = d.drive[col];
+    const uint32_t v = d.acc[gid];
+    const uint32_t stimW = clampCol ? d.stim[col] : 0u;
+    const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
+    const bool gatePass = dv >= th;
+    uint32_t b = 0u, vk = 0u, vk1 = 0u;
+    if (clampCol || gatePass) {
+        const Kwta kw = kwta32(v, P.k, n, h);
+        vk = kw.vk; vk1 = kw.vk1;
+        b = gatePass ? kw.ball : 0u;
+        if (clampCol) b = stimW;
+    }
+    const bool post = ((b >> n) & 1u) != 0u;
+    if (n == 0u) {
+        spkOut[col] = b;
+        d.marg[col] = (vk > vk1) ? 1u : 0u;
+        d.colNZ[col] = b != 0u ? 1u : 0u;
+        if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
+    }
+    if (LEARN && rec) {  // k_bias
+        const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
+        const int nb = (int)bia + (post ? (int)P.biasUp : -dec);
+        d.bias[gid] = (uint32_t)max(0, nb);
+    }
+    const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
+    if (!learnCol) return;  // half-uniform
+    const uint32_t bbv = post ? d.bbits[gid] : 0u;
+    const uint32_t cn = d.conn[col * NSLOT + n];
+    const uint32_t fwb = d.firedW[col * NSLOT + n];
+    const uint32_t band = d.tileBand[col * NSLOT + n];
+    const uint2 lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
+    uint32_t want = 0u;  // lane n: the neurons whose branch-n dendrite learns
+#pragma unroll
+    for (uint32_t s = 0; s < 32u; ++s) {
+        const uint32_t m = hball(((bbv >> s) & 1u) != 0u, h);
+        if (n == s) want = m;
+    }
+    const uint32_t voted = hball(want != 0u, h);
+    const bool full = full_tile_update(P, band);
+    uint32_t lines = want != 0u ? (full ? 8u : popc(fwb)) : 0u;
+    lines = half_sum(lines);
+    const uint32_t par = P.step & 1u;
+    uint32_t base = 0u;
+    if (n == 0u) {
+        base = atomicAdd(list_count(d, par, col % NLISTS), popc(voted));
+        reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
+    }
+    base = __shfl(base, (int)(32u * h));
+    if (want != 0u) {
+        const uint32_t at = base + popc(voted & ((1u << n) - 1u));
+        const size_t cap = (size_t)d.ncol / NLISTS * NSLOT;
+        d.tileList[(par * NLISTS + col % NLISTS) * cap + at] =
+            make_uint4(col * NSLOT + n, cn, want, fwb | (full ? 0x100u : 0u));
+    }
+}
+
 template <bool LEARN>
 __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t t) {
     const uint32_t net = blockIdx.y;
@@ -1330,7 +1390,9 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
     const NetDev d = m.nets[net];
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    emit_cols<LEARN, false, false>(d, P, (t & 1u) ? s1 : s0, (t & 1u) ? s0 : s1, blockIdx.x * 4u);
+    const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
+    if (colA >= d.ncol) return;
+    emit_pair<LEARN>(d, P, (t & 1u) ? s0 : s1, colA);
 }
 
 __global__ __launch_bounds__(256) void k_mb_learn(MultiDev m, Params P, uint32_t t) {
@@ -1566,8 +1628,8 @@ hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s) 
 }
 hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
                              hipStream_t s) {
-    if (learn) k_mb_emit<true><<<dim3((ncol + 3u) / 4u, m.nnet), 256, 0, s>>>(m, P, t);
-    else k_mb_emit<false><<<dim3((ncol + 3u) / 4u, m.nnet), 256, 0, s>>>(m, P, t);
+    if (learn) k_mb_emit<true><<<dim3((ncol + 7u) / 8u, m.nnet), 256, 0, s>>>(m, P, t);
+    else k_mb_emit<false><<<dim3((ncol + 7u) / 8u, m.nnet), 256, 0, s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
