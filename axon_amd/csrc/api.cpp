@@ -342,7 +342,6 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
                   : (t < clampSteps)      ? 1u
                   : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
         e = axon::launch_drive(n->d, P, t, n->stream);
-        if (e == hipSuccess) e = axon::launch_topk(n->d, P, n->stream);
         if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream);
         if (e == hipSuccess && learn && P.phase != 3u) e = axon::launch_learn(n->d, P, t, stats, n->stream);
     }
@@ -350,7 +349,7 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
 }
 
 // Same launches as enqueue_steps, each carrying start / end timing events,
-// so the four kernels' device durations can be summed after the presentation.
+// so the kernels' device durations can be summed after the presentation.
 static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn,
                                          bool stats) {
     const size_t need = (size_t)8 * std::max<uint32_t>(T, 1);
@@ -374,7 +373,6 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         hipEvent_t* ev = &n->prof_ev[(size_t)8 * t];
         plast[t] = learn && P.phase != 3u;
         if ((e = axon::launch_drive(n->d, P, t, n->stream, ev[0], ev[1])) != hipSuccess) break;
-        if ((e = axon::launch_topk(n->d, P, n->stream, ev[2], ev[3])) != hipSuccess) break;
         if ((e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, ev[4], ev[5])) != hipSuccess) break;
         if (!plast[t]) continue;
         e = axon::launch_learn(n->d, P, t, stats, n->stream, ev[6], ev[7]);
@@ -382,7 +380,8 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
     for (uint32_t t = 0; t < T; ++t) {
-        for (int k = 0; k < (plast[t] ? 4 : 3); ++k) {
+        for (int k = 0; k < 4; ++k) {
+            if (k == 1 || (k == 3 && !plast[t])) continue;  // top-K runs inside emit; no plasticity this step
             float ms = 0.f;
             if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)8 * t + 2 * k], n->prof_ev[(size_t)8 * t + 2 * k + 1])) != hipSuccess)
                 return e;

@@ -121,8 +121,6 @@ hipError_t launch_clear(const NetDev& d, hipStream_t s);
 // e0 / e1 (optional): timing events recorded at the start / end of the kernel
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
-hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s, hipEvent_t e0 = nullptr,
-                       hipEvent_t e1 = nullptr);
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s,

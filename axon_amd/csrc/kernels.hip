@@ -651,10 +651,6 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
     return th;
 }
 
-__global__ __launch_bounds__(1024) void k_topk(NetDev d, Params P) {
-    topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
-}
-
 __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__ drive, uint32_t* theta,
                                                    uint32_t ncol, uint32_t inCols, uint32_t target, uint32_t par) {
     topk_block<1024>(drive + (size_t)blockIdx.x * ncol, theta + 2 * blockIdx.x, ncol, inCols, target, par);
@@ -731,7 +727,7 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
 // timestep is latency-bound; emit_pair is the form for many networks).
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
-                                          uint32_t* __restrict__ spkOut, uint32_t col0) {
+                                          uint32_t* __restrict__ spkOut, uint32_t col0, uint32_t th) {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
@@ -740,7 +736,6 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
     const uint32_t gid = col * NPC + n;
     const bool clampCol = P.clamp != 0u && col < P.inCols;
     const bool rec = col >= P.inCols;
-    const uint32_t th = max(d.theta[0], 1u);
     const uint32_t dv = d.drive[col];
     const uint32_t v = d.acc[gid];
     const uint32_t stimW = clampCol ? d.stim[col] : 0u;
@@ -921,10 +916,17 @@ __global__ __launch_bounds__(256) void k_learn_tiles(NetDev d, Params P, const u
     learn_tiles(d, P, spkIn, blockIdx.x * 4u + wave, gridDim.x * 4u);
 }
 
+// Column top-K and emit in one kernel for a single network: every workgroup
+// (one per CU at 16 columns each) selects theta over all column drives itself
+// -- a few microseconds of LDS work on 16 KiB that the drive kernel left in
+// cache -- instead of waiting for a separate one-workgroup launch; workgroup 0
+// also stores (theta, need).  Then each wave emits one column.
 template <bool LEARN, bool STATS>
-__global__ __launch_bounds__(256) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
-                                              uint32_t* __restrict__ spkOut) {
-    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, blockIdx.x * 4u);
+__global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                               uint32_t* __restrict__ spkOut) {
+    const uint32_t th = topk_block<1024>(d.drive, blockIdx.x == 0u ? d.theta : nullptr, d.ncol, P.inCols,
+                                         P.targetCols, d.topkpar);
+    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, blockIdx.x * 16u, th);
 }
 
 template <bool STATS>
@@ -1511,10 +1513,6 @@ hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_
     return launch(k_drive, dim3(blocks_for(d.ncol, 4)), dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
 }
 
-hipError_t launch_topk(const NetDev& d, const Params& P, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    return launch(k_topk, dim3(1), dim3(1024), 0u, s, e0, e1, d, P);
-}
-
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
@@ -1528,7 +1526,7 @@ hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn,
                        hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
     uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    const dim3 g(blocks_for(d.ncol, 4)), b(256);
+    const dim3 g(blocks_for(d.ncol, 16)), b(1024);  // column top-K included
     if (learn && stats) return launch(k_emit<true, true>, g, b, 0u, s, e0, e1, d, P, in, out);
     if (learn) return launch(k_emit<true, false>, g, b, 0u, s, e0, e1, d, P, in, out);
     if (stats) return launch(k_emit<false, true>, g, b, 0u, s, e0, e1, d, P, in, out);
