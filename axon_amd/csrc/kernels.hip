@@ -722,29 +722,44 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
     }
 }
 
-// Every global value a column's emit can need is loaded up front, so the
-// wave waits for memory once rather than once per use (a single network's
-// timestep is latency-bound; emit_pair is the form for many networks).
+// Every global value a column's emit can need is loaded up front (before
+// the column top-K that k_emit runs first), so the wave waits for memory once
+// rather than once per use (a single network's timestep is latency-bound;
+// emit_pair is the form for many networks).
+struct EmitIn { uint32_t dv, v, stimW, bbv, cn, fwb, band, bia; uint2 lc; };
+template <bool LEARN, bool STATS>
+__device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, uint32_t col) {
+    const uint32_t n = lane_id() & 31u;
+    const uint32_t gid = col * NPC + n;
+    const bool clampCol = P.clamp != 0u && col < P.inCols;
+    const bool rec = col >= P.inCols;
+    EmitIn in;
+    in.dv = d.drive[col];
+    in.v = d.acc[gid];
+    in.stimW = clampCol ? d.stim[col] : 0u;
+    in.bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
+    in.cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
+    in.fwb = LEARN ? d.firedW[col * NSLOT + n] : 0u;
+    in.band = LEARN ? d.tileBand[col * NSLOT + n] : 0u;
+    in.lc = LEARN ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
+    in.bia = (LEARN && rec) ? d.bias[gid] : 0u;
+    return in;
+}
+
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
-                                          uint32_t* __restrict__ spkOut, uint32_t col0, uint32_t th) {
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+                                          uint32_t* __restrict__ spkOut, uint32_t col, const EmitIn& in,
+                                          uint32_t th) {
+    const uint32_t lane = lane_id();
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
-    const uint32_t col = col0 + wave;
     if (col >= d.ncol) return;
     const uint32_t gid = col * NPC + n;
     const bool clampCol = P.clamp != 0u && col < P.inCols;
     const bool rec = col >= P.inCols;
-    const uint32_t dv = d.drive[col];
-    const uint32_t v = d.acc[gid];
-    const uint32_t stimW = clampCol ? d.stim[col] : 0u;
-    const uint32_t bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
-    const uint32_t cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
-    const uint32_t fwb = LEARN ? d.firedW[col * NSLOT + n] : 0u;
-    const uint32_t band = LEARN ? d.tileBand[col * NSLOT + n] : 0u;
-    const uint2 lc = LEARN ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
-    const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
+    const uint32_t dv = in.dv, v = in.v, stimW = in.stimW, bbv = in.bbv, cn = in.cn, fwb = in.fwb, band = in.band;
+    const uint32_t bia = in.bia;
+    const uint2 lc = in.lc;
     const bool gatePass = dv >= th;
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
     if (clampCol || gatePass) {
@@ -924,9 +939,11 @@ __global__ __launch_bounds__(256) void k_learn_tiles(NetDev d, Params P, const u
 template <bool LEARN, bool STATS>
 __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
                                                uint32_t* __restrict__ spkOut) {
+    const uint32_t col = blockIdx.x * 16u + (threadIdx.x >> 6);
+    const EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the top-K
     const uint32_t th = topk_block<1024>(d.drive, blockIdx.x == 0u ? d.theta : nullptr, d.ncol, P.inCols,
                                          P.targetCols, d.topkpar);
-    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, blockIdx.x * 16u, th);
+    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, th);
 }
 
 template <bool STATS>
