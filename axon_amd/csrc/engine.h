@@ -70,7 +70,10 @@ struct NetDev {
     uint4* tileList;     // [2][8][ncol / 8 * 32]
     uint32_t* learnCnt;  // [2][8] list lengths, 32 words apart (reset by the step after next's propagation)
     uint32_t ncol, incols, inbr, topkpar;
-    uint32_t nstats, ndlog, maxt, pad;
+    // active: batched training only -- this round's pattern of the network
+    // (~0 = idle), written by k_mb_begin into the batch's device copy of the
+    // descriptors so that a kernel learns it with the descriptor itself.
+    uint32_t nstats, ndlog, maxt, active;
 };
 
 // Device pointers of a probe batch: `nslots` independent readout

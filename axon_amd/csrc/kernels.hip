@@ -1299,9 +1299,20 @@ __device__ __forceinline__ uint32_t mb_pattern(const MultiDev& m, uint32_t net) 
     return m.sched[(size_t)(*m.round) * m.nnet + net];
 }
 
+// The descriptor of network `net` with its `active` pattern: all fields are
+// loaded in one batch before the idle test (the asm makes the kernel wait for
+// them there), instead of the test's load and then the descriptor's.
+__device__ __forceinline__ NetDev mb_net(const MultiDev& m, uint32_t net) {
+    const NetDev d = m.nets[net];
+    asm volatile("" ::"s"(d.W0), "s"(d.W3), "s"(d.conn), "s"(d.spk[0]), "s"(d.drive), "s"(d.cntProp),
+                 "s"(d.learnCnt), "s"(d.ncol), "s"(d.active));
+    return d;
+}
+
 __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
     const uint32_t net = blockIdx.y;
     const uint32_t p = mb_pattern(m, net);
+    if (blockIdx.x == 0u && threadIdx.x == 0u) const_cast<NetDev*>(m.nets)[net].active = p;
     if (p == 0xFFFFFFFFu) return;
     const NetDev d = m.nets[net];
     const uint32_t* tab = reinterpret_cast<const uint32_t*>(m.stimTabs[net]) + (size_t)p * d.ncol;
@@ -1313,10 +1324,10 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
 
 __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t) {
     const uint32_t net = blockIdx.y;
-    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
+    const NetDev d = mb_net(m, net);
+    if (d.active == 0xFFFFFFFFu) return;
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_COL];
-    const NetDev d = m.nets[net];
     if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
     const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
     const uint32_t* in = (t & 1u) ? d.spk[1] : d.spk[0];  // no dynamic index into the local copy
@@ -1328,8 +1339,8 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
 
 __global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
     const uint32_t net = blockIdx.x;
-    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
-    const NetDev d = m.nets[net];
+    const NetDev d = mb_net(m, net);
+    if (d.active == 0xFFFFFFFFu) return;
     topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
@@ -1405,8 +1416,8 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
 template <bool LEARN>
 __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t t) {
     const uint32_t net = blockIdx.y;
-    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
-    const NetDev d = m.nets[net];
+    const NetDev d = mb_net(m, net);
+    if (d.active == 0xFFFFFFFFu) return;
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
     const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
@@ -1416,8 +1427,8 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
 
 __global__ __launch_bounds__(256) void k_mb_learn(MultiDev m, Params P, uint32_t t) {
     const uint32_t net = blockIdx.y;
-    if (mb_pattern(m, net) == 0xFFFFFFFFu) return;
-    const NetDev d = m.nets[net];
+    const NetDev d = mb_net(m, net);
+    if (d.active == 0xFFFFFFFFu) return;
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

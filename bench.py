@@ -12,8 +12,10 @@ value = synaptic events per second (GSyn/s): 8192 synapses for every
 (column, branch) tile propagation evaluates, over the timed region.
 
 Also reported: plasticity synapse updates/s, the roofline of the dominant
-kernel (per-kernel HIP-event timing on an identical twin network), and the
-CPU oracle's throughput on a bounded sample of the same workload.
+kernel (per-kernel HIP-event timing on an identical twin network), the
+whole-run HBM rate (every kernel's algorithmic bytes over the wall time; also
+for the plain and batched many-network runs), and the CPU oracle's
+throughput on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -69,6 +71,17 @@ def kernel_bytes(ncol, steps, prop_lines, learn_lines):
     return {"propagation": prop, "topk_emit": emit, "plasticity": plast}
 
 
+def whole_run_hbm(ncol, steps, lines, seconds):
+    """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
+    timesteps and device line counters) over the run's wall time: the average
+    rate of the whole job, idle launches and gaps included."""
+    by = kernel_bytes(ncol, steps, lines[0], lines[1])
+    tot = sum(by.values())
+    gbps = tot / seconds / 1e9
+    return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
+            "peak": HBM_PEAK_GBS, "frac": round(gbps / HBM_PEAK_GBS, 4)}
+
+
 def gpu_bench(args):
     import axon_amd
 
@@ -122,6 +135,7 @@ def gpu_bench(args):
         "ms_per_step": 1e3 * dt / args.steps,
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
+        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt),
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dn),
@@ -190,7 +204,8 @@ def c2_networks(args):
     mk = lambda P, sd: axon_amd.Config.parse(CONFIG_C + f"stim={P} epochs={ep} seed={sd}".split())  # noqa: E731
     names = ("W0", "W1", "W2", "W3", "BIAS")
     log(f"networks: plain path, {len(specs)} networks trained one after another")
-    ref, t_plain, tiles_plain = {}, 0.0, 0
+    ref, t_plain, tiles_plain, lines_plain = {}, 0.0, 0, [0, 0]
+    T = mk(*specs[0]).T
     for P, sd in specs:
         net = axon_amd.Net(mk(P, sd))
         t0 = time.perf_counter()
@@ -198,6 +213,7 @@ def c2_networks(args):
         t_plain += time.perf_counter() - t0
         ref[(P, sd)] = [axon_amd.checksum(net, nm) for nm in names]
         tiles_plain += net.work_counters()[0]
+        lines_plain = [a + b for a, b in zip(lines_plain, net.line_counters())]
         net.close()
     log("networks: batched, all networks in shared launches")
     nets = [axon_amd.Net(mk(P, sd)) for P, sd in specs]
@@ -206,6 +222,8 @@ def c2_networks(args):
     exact = all([axon_amd.checksum(n, nm) for nm in names] == ref[spec] for n, spec in zip(nets, specs))
     tiles = sum(n.work_counters()[0] for n in nets)
     learn = sum(n.work_counters()[1] for n in nets)
+    lines = [sum(n.line_counters()[i] for n in nets) for i in (0, 1)]
+    ncol = nets[0].ncol
     mb.close()
     for n in nets:
         n.close()
@@ -214,6 +232,8 @@ def c2_networks(args):
             "plain_gsyn_s": round(SYN_PER_TILE * tiles_plain / t_plain / 1e9, 1),
             "batched_gsyn_s": round(SYN_PER_TILE * tiles / (ms / 1e3) / 1e9, 1),
             "batched_plasticity_gsyn_s": round(SYN_PER_TILE * learn / (ms / 1e3) / 1e9, 1),
+            "plain_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines_plain, t_plain),
+            "batched_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines, ms / 1e3),
             "bit_exact_vs_plain": bool(exact and tiles == tiles_plain)}
 
 
@@ -379,6 +399,7 @@ def main():
         "kernels": {k: {kk: round(vv, 3) if isinstance(vv, float) else vv for kk, vv in v.items()}
                     for k, v in g["kernels"].items()},
         "roofline": g["roofline"],
+        "whole_run_hbm": g["whole_run_hbm"],
         "batched_vs_plain": batched,
         "serving": serving,
         "cpu_baseline": cpu,
