@@ -790,7 +790,7 @@ struct axon_probes {
     axon::ProbeDev d{};
     uint32_t tile = 32;
     std::vector<void*> allocs;
-    std::map<uint32_t, hipGraphExec_t> graphs;  // keyed by steps per launch
+    std::map<uint64_t, hipGraphExec_t> graphs;  // keyed by (steps per launch, slots advanced)
 };
 
 extern "C" void axon_probes_destroy(axon_probes* b) {
@@ -879,21 +879,26 @@ extern "C" axon_status axon_probes_load(axon_probes* b, uint32_t first, uint32_t
     return AXON_OK;
 }
 
-extern "C" axon_status axon_probes_step(axon_probes* b, uint32_t steps) {
-    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
-    if (!steps) return AXON_OK;
+// Advances slots [0, nact) by `steps` timesteps; every slot from nact on must
+// be idle (finished or never admitted), so launches cover only the probe
+// tiles that can hold live work.  nact is rounded up to whole tiles.
+static axon_status probes_advance(axon_probes* b, uint32_t steps, uint32_t nact) {
+    if (!steps || !nact) return AXON_OK;
     axon_net* n = b->net;
+    nact = std::min((nact + b->tile - 1u) / b->tile * b->tile, b->d.nslots);
     HIPCHECK(hipSetDevice(n->device));
     if (!n->use_graphs) {
-        for (uint32_t i = 0; i < steps; ++i) HIPCHECK(axon::launch_probe_step(b->d, b->tile, n->stream));
+        for (uint32_t i = 0; i < steps; ++i) HIPCHECK(axon::launch_probe_step(b->d, b->tile, nact, n->stream));
         return AXON_OK;
     }
-    auto it = b->graphs.find(steps);
+    const uint64_t key = (uint64_t)steps << 32 | nact;
+    auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
-        for (uint32_t i = 0; i < steps && e == hipSuccess; ++i) e = axon::launch_probe_step(b->d, b->tile, n->stream);
+        for (uint32_t i = 0; i < steps && e == hipSuccess; ++i)
+            e = axon::launch_probe_step(b->d, b->tile, nact, n->stream);
         hipError_t e2 = hipStreamEndCapture(n->stream, &g);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "probe capture: %s", hipGetErrorString(e));
         if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -901,10 +906,15 @@ extern "C" axon_status axon_probes_step(axon_probes* b, uint32_t steps) {
         hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e3 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e3));
-        it = b->graphs.emplace(steps, ex).first;
+        it = b->graphs.emplace(key, ex).first;
     }
     HIPCHECK(hipGraphLaunch(it->second, n->stream));
     return AXON_OK;
+}
+
+extern "C" axon_status axon_probes_step(axon_probes* b, uint32_t steps) {
+    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
+    return probes_advance(b, steps, b->d.nslots);
 }
 
 extern "C" axon_status axon_probes_sync(axon_probes* b) {
@@ -1394,10 +1404,11 @@ extern "C" axon_status axon_server_tick(axon_server* s, uint32_t* completed) {
             HIPCHECK(hipMemcpyAsync(v.d_list, v.h_list, (size_t)nadm[i] * (4 + NC) * 4, hipMemcpyHostToDevice, st));
             HIPCHECK(axon::launch_probe_admit(v.pb->d, v.d_list, nadm[i], st));
         }
-        bool any = false;
-        for (auto& sl : v.slots) any |= sl.busy;
-        if (!any) continue;
-        axon_status r = axon_probes_step(v.pb, s->cfg.steps_per_tick);
+        uint32_t hi = 0;  // slots are taken lowest-first: launches cover [0, last busy slot]
+        for (uint32_t k = 0; k < v.slots.size(); ++k)
+            if (v.slots[k].busy) hi = k + 1u;
+        if (!hi) continue;
+        axon_status r = probes_advance(v.pb, s->cfg.steps_per_tick, hi);
         if (r != AXON_OK) return r;
         HIPCHECK(hipMemcpyAsync(v.h_t, v.pb->d.st_t, v.slots.size() * 4, hipMemcpyDeviceToHost, st));
     }

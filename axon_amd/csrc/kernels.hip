@@ -1244,31 +1244,52 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
 // k-of-32 for columns that pass (the others are silent), the input clamp, the
 // spike and history writes and the slot's group-activity bits.  The margin,
 // stats and drive-log side outputs of k_emit are not produced for probes.
-__global__ __launch_bounds__(256) void k_probe_select(ProbeDev d) {
+// 1024 threads per probe: the top-K over the probe's drives, then 16 waves
+// over the columns, 64 at a time.  The k-of-32 selection of the columns that
+// pass the gate is batched: up to 8 of a wave's gated columns per round, two
+// at a time (half-wave h takes every other one), their acc loads all in
+// flight together, results handed to the column's lane through LDS.
+__global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d) {
+    __shared__ uint32_t bres[16][64];
     const uint32_t sl = blockIdx.x;
     if (sl >= d.nslots) return;
     const uint32_t t = d.st_t[sl];
     if (t >= d.st_T[sl]) return;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6, h = lane >> 5, n = lane & 31u;
     const size_t so = (size_t)sl * d.ncol;
-    const uint32_t th = topk_block<256>(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
+    const uint32_t th = topk_block<1024>(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
     const bool clampPhase = t < d.st_clamp[sl];
     const uint32_t* __restrict__ acc = d.acc + so * NPC;
     uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
-    for (uint32_t c0 = wave * 64u; c0 < d.ncol; c0 += 256u) {
+    for (uint32_t c0 = wave * 64u; c0 < d.ncol; c0 += 1024u) {
         const uint32_t c = c0 + lane;
         const bool inb = c < d.ncol;
         const uint32_t dv = inb ? d.drive[so + c] : 0u;
         const bool clampCol = inb && clampPhase && c < d.incols;
         uint32_t b = clampCol ? d.stim[so + c] : 0u;
-        uint64_t m = __ballot(inb && !clampCol && dv >= th);
-        while (m) {
-            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-            m &= m - 1ull;
-            const uint32_t v = acc[(size_t)(c0 + i) * NPC + (lane & 31u)];
-            const Kwta kw = kwta32(v, d.k, lane & 31u);
-            if (lane == i) b = kw.ball;
+        const bool gate = inb && !clampCol && dv >= th;
+        uint64_t m = __ballot(gate);
+        while (m) {  // wave-uniform
+            uint32_t ci[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                ci[j] = m ? (uint32_t)__ffsll((unsigned long long)m) - 1u : 64u;
+                if (m) m &= m - 1ull;
+            }
+            uint32_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t i = h ? ci[2 * j + 1] : ci[2 * j];
+                v[j] = i < 64u ? acc[(size_t)(c0 + i) * NPC + n] : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t i = h ? ci[2 * j + 1] : ci[2 * j];
+                const Kwta kw = kwta32(v[j], d.k, n, h);
+                if (n == 0u && i < 64u) bres[wave][i] = kw.ball;
+            }
         }
+        if (gate) b = bres[wave][lane];  // written above by this wave
         if (inb) { d.spk[so + c] = b; hist[c] = b; }
         const uint64_t nz = __ballot(inb && b != 0u);  // one bit per column of this wave's 64
         uint32_t* ga = d.gact + (size_t)sl * d.gw + (c0 >> 5);
@@ -1471,31 +1492,31 @@ __global__ __launch_bounds__(256) void k_probe_admit(ProbeDev d, const uint32_t*
 // Serving: pack finished slots' results: final spikes, then one 64-bit
 // checksum per history row (the k_checksum formula over the row's words).
 // out + offs[i] receives ncol words followed by 2*T words.
+// Workgroup (j, y): slot slots[j]; y == 0 also copies its spikes; row y of
+// its history (y < T) is reduced to its 64-bit digest.
 __global__ __launch_bounds__(256) void k_probe_gather(ProbeDev d, const uint32_t* __restrict__ slots,
                                                       const uint32_t* __restrict__ offs, uint32_t n,
                                                       uint32_t* __restrict__ out) {
     __shared__ unsigned long long part[4];
     if (blockIdx.x >= n) return;
-    const uint32_t sl = slots[blockIdx.x];
+    const uint32_t sl = slots[blockIdx.x], t = blockIdx.y;
     uint32_t* o = out + offs[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) o[i] = d.spk[(size_t)sl * d.ncol + i];
-    const uint32_t T = d.st_T[sl];
-    for (uint32_t t = 0; t < T; ++t) {
-        const uint32_t* row = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
-        unsigned long long acc = 0;
-        for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) {
-            const uint32_t w = row[i];
-            acc += (unsigned long long)w * (2ull * i + 1ull) + hash32(w ^ i);
-        }
-        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane_id() == 0u) part[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0u) {
-            const unsigned long long v = part[0] + part[1] + part[2] + part[3];
-            o[d.ncol + 2u * t] = (uint32_t)v;
-            o[d.ncol + 2u * t + 1u] = (uint32_t)(v >> 32);
-        }
-        __syncthreads();
+    if (t == 0u)
+        for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) o[i] = d.spk[(size_t)sl * d.ncol + i];
+    if (t >= d.st_T[sl]) return;  // uniform over the workgroup
+    const uint32_t* row = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
+    unsigned long long acc = 0;
+    for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) {
+        const uint32_t w = row[i];
+        acc += (unsigned long long)w * (2ull * i + 1ull) + hash32(w ^ i);
+    }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane_id() == 0u) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        const unsigned long long v = part[0] + part[1] + part[2] + part[3];
+        o[d.ncol + 2u * t] = (uint32_t)v;
+        o[d.ncol + 2u * t + 1u] = (uint32_t)(v >> 32);
     }
 }
 
@@ -1611,9 +1632,11 @@ hipError_t launch_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol
 
 namespace axon {
 
-hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, hipStream_t s) {
+hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, hipStream_t s) {
+    nact = std::min(nact, d.nslots);
+    if (nact == 0u) return hipSuccess;
     const uint32_t ncg = (d.ncol + 3u) / 4u;
-    const uint32_t ntiles = (d.nslots + tile - 1u) / tile;
+    const uint32_t ntiles = (nact + tile - 1u) / tile;
     const uint32_t blocks = ((ncg + 7u) / 8u) * 8u * ntiles;
     switch (tile) {
         case 8: k_probe_drive<8><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
@@ -1624,7 +1647,7 @@ hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, hipStream_t s) {
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_probe_select<<<d.nslots, 256, 0, s>>>(d);
+    k_probe_select<<<nact, 1024, 0, s>>>(d);
     return hipGetLastError();
 }
 
@@ -1685,7 +1708,7 @@ hipError_t launch_probe_admit(const ProbeDev& d, const uint32_t* list, uint32_t 
 hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const uint32_t* offs, uint32_t n,
                                uint32_t* out, hipStream_t s) {
     if (!n) return hipSuccess;
-    k_probe_gather<<<n, 256, 0, s>>>(d, slots, offs, n, out);
+    k_probe_gather<<<dim3(n, std::max(d.maxt, 1u)), 256, 0, s>>>(d, slots, offs, n, out);
     return hipGetLastError();
 }
 }  // namespace axon
