@@ -106,3 +106,35 @@ def test_budget_admission_and_rejection():
     ref = isolated(nets[0], cue, 100)
     for r in done.values():
         assert np.array_equal(r[0], ref[0]) and np.array_equal(r[1], ref[1])
+
+
+def test_launches_cover_up_to_the_highest_busy_slot():
+    """Ticks advance only slots below the highest busy one: a long query in a
+    high slot keeps running while the low slots empty and refill."""
+    nets = trained_nets()[:1]
+    net = nets[0]
+    srv = axon_amd.Server(nets, slots_per_net=32, max_t=200, tile=8, steps_per_tick=5)
+    reqs = [(axon_amd.probe_cue(net, i % 4, 6, 100 + i), 10) for i in range(20)]  # slots 0..19
+    reqs.append((axon_amd.probe_cue(net, 1, 16, 999), 200))  # slot 20: long
+    ids = {srv.submit(0, cue, T): i for i, (cue, T) in enumerate(reqs)}
+    results, ticks, refilled = {}, 0, False
+    while len(results) < len(reqs):
+        srv.tick()
+        ticks += 1
+        for rid in list(ids):
+            r = srv.result(rid, 200)
+            if r is not None:
+                results[ids.pop(rid)] = r
+        if not refilled and len(results) >= 20:  # only the long query is in flight now
+            assert 20 not in results
+            for j in range(5):  # these take the low slots again
+                cue = axon_amd.probe_cue(net, j % 4, 0, 500 + j)
+                reqs.append((cue, 35))
+                ids[srv.submit(0, cue, 35)] = len(reqs) - 1
+            refilled = True
+    assert refilled and ticks >= 40
+    for i, (cue, T) in enumerate(reqs):
+        spk, dig, _, TT = results[i]
+        s_alone, d_alone = isolated(net, cue, T)
+        assert TT == T and np.array_equal(spk, s_alone) and np.array_equal(dig, d_alone), f"request {i}"
+    srv.close()
