@@ -1117,6 +1117,145 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // Blocks are numbered so that the blocks one XCD receives (linear id L,
 // L+8, L+16, ...) walk all probe tiles of the same four columns back to back,
 // so the tile's W3 bytes are reused from that XCD's L2.
+// Probe propagation for tiles of 8 probes, one wave per column, as drive_col
+// does it for one network: work items (branch, spiking word, probe) listed in
+// LDS per half-wave (all of a branch's items, for every probe, go to one
+// half; branches with work are dealt alternately to the halves), so one dword
+// load instruction serves one item of each half and 8 items per half are in
+// flight together.  An item reads the W3 word of (branch, word) for neuron n
+// and the probe's spike word of presynaptic column 8g + w.  Per wave this
+// takes a few round trips instead of one per branch of the tile's union
+// (the probes of a tile share few W3 lines, so that union saves little).
+// Item word: branch | word << 5 | probe << 8 | group << 11 | 1 << 31 on the
+// last item of a (probe, branch) pair.
+constexpr uint32_t PROBE_ITEMS_HALF = 16u * 8u * 8u;  // 16 branches x 8 probes x 8 words
+__global__ __launch_bounds__(256) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles) {
+    constexpr int PT = 8;
+    constexpr uint32_t DB = 8;  // items per half in flight
+    __shared__ __align__(16) uint32_t items[4][2 * PROBE_ITEMS_HALF];
+    const uint32_t L = blockIdx.x;
+    const uint32_t q = L >> 3;
+    const uint32_t tile = q % ntiles;
+    const uint32_t cg = (q / ntiles) * 8u + (L & 7u);
+    if (cg >= ncolgroups) return;
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const bool hi = lane >= 32u;
+    const uint32_t col = cg * 4u + (threadIdx.x >> 6);
+    if (col >= d.ncol) return;
+    const uint32_t base = col * NSLOT;
+    const uint32_t slot0 = tile * PT;
+    const uint32_t gl = d.conn[base + n];
+    uint32_t stt[PT], stT[PT], gaw[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {  // slots past the end read the last slot's state and are masked off
+        const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
+        stt[p] = d.st_t[sl];
+        stT[p] = d.st_T[sl];
+        gaw[p] = d.gact[(size_t)sl * d.gw + (gl >> 2)];
+    }
+    uint32_t fb[PT];  // lane s: which of branch s's 8 presynaptic columns spiked in probe p
+    uint32_t live = 0, ni = 0;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const bool lv = slot0 + (uint32_t)p < d.nslots && stt[p] < stT[p];
+        live |= (lv ? 1u : 0u) << p;
+        fb[p] = lv ? (gaw[p] >> (8u * (gl & 3u))) & 0xFFu : 0u;
+        ni += popc(fb[p]);
+    }
+    if (live == 0u) return;
+    const uint32_t actv = ballot32(ni != 0u);
+    const uint32_t hh = popc(actv & ((1u << n) - 1u)) & 1u;
+    const uint32_t x0 = hh == 0u ? ni : 0u, x1 = hh == 1u ? ni : 0u;
+    uint32_t s0 = x0, s1 = x1;
+#pragma unroll
+    for (uint32_t off = 1; off < 32u; off <<= 1) {
+        const uint32_t t0 = __shfl_up(s0, off, 32), t1 = __shfl_up(s1, off, 32);
+        if (n >= off) { s0 += t0; s1 += t1; }
+    }
+    const uint32_t nA = __builtin_amdgcn_readlane(s0, 31), nB = __builtin_amdgcn_readlane(s1, 31);
+    uint32_t* W = items[threadIdx.x >> 6];
+    if (!hi && ni != 0u) {
+        uint32_t* Lh = W + hh * PROBE_ITEMS_HALF;
+        uint32_t pos = hh ? s1 - x1 : s0 - x0;
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+            uint32_t m = fb[p];
+            while (m) {
+                const uint32_t w = __builtin_ctz(m);
+                m &= m - 1u;
+                Lh[pos++] = n | (w << 5) | ((uint32_t)p << 8) | (gl << 11) | (m == 0u ? 0x80000000u : 0u);
+            }
+        }
+    }
+    const uint32_t myN = hi ? nB : nA;
+    const uint32_t* Li = W + (hi ? PROBE_ITEMS_HALF : 0u);  // read after this wave's writes above
+    const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);
+    const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, NSLOT * TILE_WORDS * 4u);
+    const uint32_t nsl = min((uint32_t)PT, d.nslots - slot0);  // slots of this tile that exist
+    const __amdgpu_buffer_rsrc_t sp = buf_rsrc(d.spk + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
+    uint32_t acc[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) acc[p] = 0u;
+    uint32_t c = 0u;
+    const uint32_t R = max(nA, nB);
+    for (uint32_t r = 0; r < R; r += DB) {
+        const uint32_t kn = min(R - r, DB);  // item places used this round (uniform)
+        const uint4 i0 = *reinterpret_cast<const uint4*>(Li + r);
+        const uint4 i1 = *reinterpret_cast<const uint4*>(Li + r + 4u);
+        const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+        uint32_t wv[DB], sv[DB];
+#pragma unroll
+        for (uint32_t k = 0; k < DB; ++k) {
+            wv[k] = 0u; sv[k] = 0u;
+            if (k < kn) {
+                const bool v = r + k < myN;
+                const uint32_t e = it[k];
+                const uint32_t br = e & 31u, w = (e >> 5) & 7u, p = (e >> 8) & 7u, g = (e >> 11) & 0xFFFFFu;
+                wv[k] = buf_ld(w3, v ? ((br * 8u + w) * 32u + n) * 4u : SKIP);
+                sv[k] = buf_ld(sp, v ? (p * d.ncol + 8u * g + w) * 4u : SKIP);
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < DB; ++k) {
+            if (k < kn && r + k < myN) {
+                const uint32_t e = it[k];
+                c += popc(wv[k] & sv[k]);
+                if (e >> 31) {  // last item of this (probe, branch): its count is complete
+                    const bool nmda = c >= d.dthr;
+                    const uint32_t x = d.mode == 0u ? c : d.mode == 1u ? (nmda ? c + d.boost : c) : (nmda ? 1u : 0u);
+                    const uint32_t p = (e >> 8) & 7u;
+#pragma unroll
+                    for (int qq = 0; qq < PT; ++qq)
+                        if ((uint32_t)qq == p) acc[qq] += x;
+                    c = 0u;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) acc[p] += __shfl_xor(acc[p], 32);  // the two halves' branches
+    const uint32_t gid = base + n;
+    const uint32_t bi = d.bias[gid] >> d.biasShift;
+    uint32_t ub[PT], fs[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
+        ub[p] = d.st_bias[sl];
+        fs[p] = d.facil != 0u ? d.spk[(size_t)sl * d.ncol + col] : 0u;
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        if (((live >> p) & 1u) == 0u) continue;
+        const uint32_t sl = slot0 + (uint32_t)p;
+        uint32_t a = acc[p];
+        if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
+        if (ub[p] != 0u) a = (a > bi) ? (a - bi) : 0u;
+        if (!hi) d.acc[(size_t)sl * d.ncol * NPC + gid] = a;
+        const uint32_t drv = half_sum(a);
+        if (lane == 0u) d.drive[(size_t)sl * d.ncol + col] = drv;
+    }
+}
+
 template <int PT>
 __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles) {
     constexpr int DB = 1;                // branch tiles in flight together
@@ -1639,7 +1778,7 @@ hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, hi
     const uint32_t ntiles = (nact + tile - 1u) / tile;
     const uint32_t blocks = ((ncg + 7u) / 8u) * 8u * ntiles;
     switch (tile) {
-        case 8: k_probe_drive<8><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 8: k_probe_items<<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
         case 16: k_probe_drive<16><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
         case 32: k_probe_drive<32><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
         case 64: k_probe_drive<64><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
