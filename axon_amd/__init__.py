@@ -71,6 +71,7 @@ class PresentOut(ctypes.Structure):
 BUFFERS = ("W0 W1 W2 W3 CONN SPK_A SPK_B DRIVE BBITS ACC BIAS MARG REFBALL DRIVELOG MULT "
            "STIM STATS HIST THETA").split()
 PRESENT_ASYNC = 1
+PROBE_TILES = (1, 2, 4, 8, 16, 32, 64)  # axon_probe_autotune's order (AXON_PROBE_TILES)
 
 _lib = None
 
@@ -290,7 +291,7 @@ class Net:
 class Probes:
     """A probe batch: `slots` readout presentations against one resident network."""
 
-    def __init__(self, net: Net, slots: int, max_t: int, tile: int = 32):
+    def __init__(self, net: Net, slots: int, max_t: int, tile: int = 4):
         self.net = net
         self._h = ctypes.c_void_p()
         _check(lib().axon_probes_create(net._h, ctypes.c_uint32(slots), ctypes.c_uint32(max_t),
@@ -379,7 +380,7 @@ class ServerConfig(ctypes.Structure):
 class Server:
     """Continuous-batching recall server over resident networks."""
 
-    def __init__(self, nets: Sequence[Net], slots_per_net: int = 64, max_t: int = 200, tile: int = 32,
+    def __init__(self, nets: Sequence[Net], slots_per_net: int = 64, max_t: int = 200, tile: int = 4,
                  steps_per_tick: int = 5, hbm_budget: int = 0):
         self.nets = list(nets)
         self.ncol = self.nets[0].ncol
@@ -476,10 +477,10 @@ def probe_autotune(net: Net, stims: np.ndarray, T: int, clamp_steps: int, use_bi
     """Fastest probe tile for this batch: (tile, {tile: ms})."""
     stims = np.ascontiguousarray(stims, np.uint32).reshape(-1, net.ncol)
     best = ctypes.c_uint32()
-    ms = (ctypes.c_double * 4)()
+    ms = (ctypes.c_double * len(PROBE_TILES))()
     _check(lib().axon_probe_autotune(net._h, ctypes.c_uint32(stims.shape[0]), _ptr(stims), ctypes.c_uint32(T),
                                      ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias), ctypes.byref(best), ms))
-    return best.value, {t: ms[i] for i, t in enumerate((8, 16, 32, 64))}
+    return best.value, {t: ms[i] for i, t in enumerate(PROBE_TILES)}
 
 
 def probe_cue(net: Net, pattern: int, nreplace: int, seed: int) -> np.ndarray:

@@ -785,6 +785,9 @@ extern "C" axon_status axon_topk_test(const uint32_t* drive, uint32_t ninst, uin
 }
 
 // ================================================================== probe batches
+// probe tiles (probes sharing one propagation wave), in the autotuner's order
+static const uint32_t kProbeTiles[AXON_PROBE_TILES] = {1, 2, 4, 8, 16, 32, 64};
+
 struct axon_probes {
     axon_net* net = nullptr;
     axon::ProbeDev d{};
@@ -805,8 +808,9 @@ extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t 
                                           axon_probes** out) {
     if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (tile != 8 && tile != 16 && tile != 32 && tile != 64)
-        return fail(AXON_ERR_INVALID_ARG, "probe tile must be 8, 16, 32 or 64 (got %u)", tile);
+    bool known = false;
+    for (uint32_t t : kProbeTiles) known |= t == tile;
+    if (!known) return fail(AXON_ERR_INVALID_ARG, "probe tile must be 1, 2, 4, 8, 16, 32 or 64 (got %u)", tile);
     if (slots == 0 || max_t == 0 || max_t > 4096) return fail(AXON_ERR_INVALID_ARG, "slots >= 1, max_t in 1..4096");
     HIPCHECK(hipSetDevice(n->device));
     axon_probes* b = new axon_probes();
@@ -1485,23 +1489,25 @@ extern "C" axon_status axon_server_status(const axon_server* s, uint32_t* queued
 
 // ================================================================== autotuning
 // Probe-tile autotuner: runs the caller's probe batch once per tile size
-// (8, 16, 32, 64 slots per W3 tile read), timing the device work with events,
-// and reports the fastest.  Larger tiles read W3 fewer times but hold more
-// accumulators per lane (lower occupancy); the balance depends on the cue
-// sparsity and the number of probes, so it is measured, not assumed.
+// (AXON_PROBE_TILES sizes, kProbeTiles), timing the device work with events,
+// and reports the fastest.  Tiles of up to 8 probes list their work per
+// (branch, word, probe) item; larger tiles walk the union of the probes'
+// active branches and read each W3 line once for all of them.  Which wins
+// depends on how much the probes' activity overlaps and on the number of
+// probes, so it is measured, not assumed.
 extern "C" axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
                                            uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
-                                           double ms_per_tile[4]) {
+                                           double ms_per_tile[AXON_PROBE_TILES]) {
     if (!n || !stims || !best_tile) return fail(AXON_ERR_INVALID_ARG, "null argument");
-    const uint32_t tiles[4] = {8, 16, 32, 64};
+    const uint32_t* tiles = kProbeTiles;
     double best = 1e30;
-    *best_tile = 32;
+    *best_tile = 8;
     HIPCHECK(hipSetDevice(n->device));
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     axon_status st = AXON_OK;
-    for (int i = 0; i < 4 && st == AXON_OK; ++i) {
+    for (int i = 0; i < AXON_PROBE_TILES && st == AXON_OK; ++i) {
         axon_probes* b = nullptr;
         if ((st = axon_probes_create(n, count, T, tiles[i], &b)) != AXON_OK) break;
         std::vector<uint32_t> Ts(b->d.nslots, 0);

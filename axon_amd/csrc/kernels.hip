@@ -1127,10 +1127,11 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // takes a few round trips instead of one per branch of the tile's union
 // (the probes of a tile share few W3 lines, so that union saves little).
 // Item word: branch | word << 5 | probe << 8 | group << 11 | 1 << 31 on the
-// last item of a (probe, branch) pair.
-constexpr uint32_t PROBE_ITEMS_HALF = 16u * 8u * 8u;  // 16 branches x 8 probes x 8 words
+// last item of a (probe, branch) pair.  PT = 1..8 probes per tile.
+template <int PT>
 __global__ __launch_bounds__(256) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles) {
-    constexpr int PT = 8;
+    static_assert(PT >= 1 && PT <= 8, "three probe bits in an item");
+    constexpr uint32_t PROBE_ITEMS_HALF = 16u * PT * 8u;  // 16 branches x PT probes x 8 words
     constexpr uint32_t DB = 8;  // items per half in flight
     __shared__ __align__(16) uint32_t items[4][2 * PROBE_ITEMS_HALF];
     const uint32_t L = blockIdx.x;
@@ -1778,7 +1779,10 @@ hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, hi
     const uint32_t ntiles = (nact + tile - 1u) / tile;
     const uint32_t blocks = ((ncg + 7u) / 8u) * 8u * ntiles;
     switch (tile) {
-        case 8: k_probe_items<<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 1: k_probe_items<1><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 2: k_probe_items<2><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 4: k_probe_items<4><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 8: k_probe_items<8><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
         case 16: k_probe_drive<16><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
         case 32: k_probe_drive<32><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
         case 64: k_probe_drive<64><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;

@@ -356,7 +356,7 @@ def main():
     ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-slots", type=int, default=128)
-    ap.add_argument("--serve-tile", type=int, default=8)
+    ap.add_argument("--serve-tile", type=int, default=4)
     ap.add_argument("--serve-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:

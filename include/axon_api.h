@@ -176,10 +176,13 @@ axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t count, uin
  * hist is count*T*NCOL (packed), ms the device wall time of the steps */
 axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
                             uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes, double* ms);
-/* autotuner: runs this probe batch with every tile size (8/16/32/64), times the
- * device work, returns the fastest tile and the per-tile milliseconds */
+/* autotuner: runs this probe batch with every tile size (1/2/4/8/16/32/64
+ * probes per propagation wave), times the device work, returns the fastest
+ * tile and the per-tile milliseconds in that order */
+#define AXON_PROBE_TILES 7
 axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
-                                uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile, double ms_per_tile[4]);
+                                uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
+                                double ms_per_tile[AXON_PROBE_TILES]);
 /* recall cue: pattern's stimulus columns with `nreplace` of them replaced by
  * other input columns (mt19937(seed) draws); pattern 0xFFFFFFFF = a random cue
  * of stimcols input columns.  Extends loadStim (mm:800-804). */
@@ -216,7 +219,7 @@ typedef struct axon_server axon_server;
 typedef struct axon_server_config {
     uint32_t slots_per_net; /* probe slots per network */
     uint32_t max_t;         /* longest query, steps */
-    uint32_t tile;          /* probe tile: 8, 16, 32 or 64 */
+    uint32_t tile;          /* probe tile: 1, 2, 4, 8, 16, 32 or 64 */
     uint32_t steps_per_tick;
     uint64_t hbm_budget;    /* bytes of in-flight query state, 0 = no limit */
     uint32_t reserved[4];

@@ -55,7 +55,7 @@ def test_cue_generator():
     assert np.count_nonzero(rnd) == 64 and np.flatnonzero(rnd).max() < 128
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32, 64])
+@pytest.mark.parametrize("tile", [1, 2, 4, 8, 16, 32, 64])
 def test_probe_batch_matches_single_presentations(tile):
     net, ora = trained(C0, 3)
     stims = cues(net, 40)
