@@ -51,8 +51,10 @@ def train_presentations(net, start, count, T, tc, nstim, outputs=False):
         net.present(p, 1.0, T, tc, True, False, 1000 + p, outputs=outputs)
 
 
-def kernel_bytes(ncol, steps, prop_lines, learn_lines):
-    """Algorithmic HBM bytes per kernel over `steps` learning timesteps.
+def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True):
+    """Algorithmic HBM bytes per kernel over `steps` learning timesteps
+    (bias=False: readout steps without the intrinsic bias, which neither read
+    nor update it).
 
     The synapse planes are word-major tiles, so a kernel touches only the
     128-byte lines of presynaptic columns that spiked (device line counters).
@@ -65,17 +67,18 @@ def kernel_bytes(ncol, steps, prop_lines, learn_lines):
     writes (16 B/column).
     plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read + written)."""
     n = ncol * 32
-    prop = 128 * prop_lines + steps * (128 * ncol + 4 * n + 8 * n + 4 * ncol + 32 * ncol)
-    emit = steps * (4 * n + 8 * n + 20 * ncol)
+    bb = 1 if bias else 0
+    prop = 128 * prop_lines + steps * (128 * ncol + 4 * bb * n + 8 * n + 4 * ncol + 32 * ncol)
+    emit = steps * (4 * n + 8 * bb * n + 20 * ncol)
     plast = 1024 * learn_lines
     return {"propagation": prop, "topk_emit": emit, "plasticity": plast}
 
 
-def whole_run_hbm(ncol, steps, lines, seconds):
+def whole_run_hbm(ncol, steps, lines, seconds, bias=True):
     """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
     timesteps and device line counters) over the run's wall time: the average
     rate of the whole job, idle launches and gaps included."""
-    by = kernel_bytes(ncol, steps, lines[0], lines[1])
+    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias)
     tot = sum(by.values())
     gbps = tot / seconds / 1e9
     return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
@@ -175,10 +178,11 @@ def c3_probes(args):
     stims = probe_set(net, axon_amd)
     T, clamp = 35, 5
     log(f"probes: plain path, {len(stims)} probes one presentation each")
-    net.work_counters(reset=True)
+    net.line_counters(reset=True)
     t0 = time.perf_counter()
     plain = [net.present(0, 1.0, T, clamp, False, True, 0, 0, stim=s).hist for s in stims]
     t_plain = time.perf_counter() - t0
+    lines = net.line_counters()
     tiles_read, _ = net.work_counters(reset=True)
     log("probes: autotuning the probe tile")
     best, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
@@ -187,12 +191,16 @@ def c3_probes(args):
     _, hist, ms = pb.run(stims, T, clamp, 0, hist=True)
     exact = all(np.array_equal(hist[i], plain[i]) for i in range(len(stims)))
     pb.close()
+    net_ncol = net.ncol
     net.close()
     syn = SYN_PER_TILE * tiles_read
     return {"probes": len(stims), "T": T, "plain_gsyn_s": round(syn / t_plain / 1e9, 1),
             "batched_gsyn_s": round(syn / (ms / 1e3) / 1e9, 1), "speedup": round(t_plain / (ms / 1e3), 2),
             "plain_ms": round(1e3 * t_plain, 2), "batched_ms": round(ms, 3), "best_tile": best,
-            "tile_sweep_ms": {str(k): round(v, 3) for k, v in sweep.items()}, "bit_exact_vs_plain": bool(exact)}
+            "tile_sweep_ms": {str(k): round(v, 3) for k, v in sweep.items()},
+            "plain_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, t_plain, bias=False),
+            "batched_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, ms / 1e3, bias=False),
+            "bit_exact_vs_plain": bool(exact)}
 
 
 def c2_networks(args):
