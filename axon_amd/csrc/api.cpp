@@ -829,7 +829,7 @@ extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t 
     struct { uint32_t** p; size_t words; } bufs[] = {
         {&d.spk, ns * NC}, {&d.gact, (size_t)ns * d.gw}, {&d.acc, ns * NN}, {&d.drive, ns * NC},
         {&d.stim, ns * NC}, {&d.hist, (size_t)ns * max_t * NC}, {&d.theta, (size_t)ns * 2},
-        {&d.st_t, ns}, {&d.st_T, ns}, {&d.st_clamp, ns}, {&d.st_bias, ns}};
+        {&d.st_t, ns}, {&d.st_T, ns}, {&d.st_clamp, ns}, {&d.st_bias, ns}, {&d.selDone, ns}};
     for (auto& x : bufs) {
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, x.words * 4);

@@ -93,6 +93,7 @@ struct ProbeDev {
     uint32_t* st_T;      // [slot] steps to run
     uint32_t* st_clamp;  // [slot] clamped steps
     uint32_t* st_bias;   // [slot] use the intrinsic bias (0/1)
+    uint32_t* selDone;   // [slot] workgroups of a split select that finished this step (reset by the last)
     uint32_t nslots, ncol, incols, maxt, gw;
     uint32_t k, facil, dthr, boost, mode, biasShift, target, topkpar;
 };

@@ -1385,24 +1385,31 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
 // k-of-32 for columns that pass (the others are silent), the input clamp, the
 // spike and history writes and the slot's group-activity bits.  The margin,
 // stats and drive-log side outputs of k_emit are not produced for probes.
-// 1024 threads per probe: the top-K over the probe's drives, then 16 waves
-// over the columns, 64 at a time.  The k-of-32 selection of the columns that
-// pass the gate is batched: up to 8 of a wave's gated columns per round, two
-// at a time (half-wave h takes every other one), their acc loads all in
-// flight together, results handed to the column's lane through LDS.
-__global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d) {
+// `parts` 1024-thread workgroups per probe, each over its own range of the
+// columns (a multiple of 64): the top-K over the probe's drives (every part
+// computes it), then 16 waves over the part's columns, 64 at a time.  The
+// k-of-32 selection of the columns that pass the gate is batched: up to 8 of
+// a wave's gated columns per round, two at a time (half-wave h takes every
+// other one), their acc loads all in flight together, results handed to the
+// column's lane through LDS.  Split parts shorten a lightly loaded step; the
+// last part to finish (a per-slot counter) advances the slot's step, after
+// every part has read it.
+__global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d, uint32_t parts) {
     __shared__ uint32_t bres[16][64];
-    const uint32_t sl = blockIdx.x;
+    const uint32_t sl = blockIdx.x / parts, part = blockIdx.x % parts;
     if (sl >= d.nslots) return;
     const uint32_t t = d.st_t[sl];
     if (t >= d.st_T[sl]) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6, h = lane >> 5, n = lane & 31u;
     const size_t so = (size_t)sl * d.ncol;
-    const uint32_t th = topk_block<1024>(d.drive + so, d.theta + 2u * sl, d.ncol, d.incols, d.target, d.topkpar);
+    const uint32_t th = topk_block<1024>(d.drive + so, part == 0u ? d.theta + 2u * sl : nullptr, d.ncol, d.incols,
+                                         d.target, d.topkpar);
     const bool clampPhase = t < d.st_clamp[sl];
     const uint32_t* __restrict__ acc = d.acc + so * NPC;
     uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
-    for (uint32_t c0 = wave * 64u; c0 < d.ncol; c0 += 1024u) {
+    const uint32_t span = ((d.ncol + parts - 1u) / parts + 63u) & ~63u;
+    const uint32_t cEnd = min(d.ncol, (part + 1u) * span);
+    for (uint32_t c0 = part * span + wave * 64u; c0 < cEnd; c0 += 1024u) {
         const uint32_t c = c0 + lane;
         const bool inb = c < d.ncol;
         const uint32_t dv = inb ? d.drive[so + c] : 0u;
@@ -1438,7 +1445,14 @@ __global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d) {
         if (lane == 1u && c0 + 32u < d.ncol) ga[1] = (uint32_t)(nz >> 32);
     }
     __syncthreads();
-    if (tid == 0u) d.st_t[sl] = t + 1u;
+    if (tid == 0u) {
+        if (parts == 1u) {
+            d.st_t[sl] = t + 1u;
+        } else if (atomicAdd(&d.selDone[sl], 1u) == parts - 1u) {
+            d.selDone[sl] = 0u;
+            d.st_t[sl] = t + 1u;
+        }
+    }
 }
 
 // Start of a probe: zero spikes and spiking-column bits, step counter to 0.
@@ -1791,7 +1805,9 @@ hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, hi
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_probe_select<<<nact, 1024, 0, s>>>(d);
+    // up to 64 live slots: each probe's select split over 4 workgroups
+    const uint32_t parts = nact <= 64u ? 4u : 1u;
+    k_probe_select<<<nact * parts, 1024, 0, s>>>(d, parts);
     return hipGetLastError();
 }
 
