@@ -365,7 +365,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-tile", type=int, default=4)
-    ap.add_argument("--serve-steps", type=int, default=10)
+    ap.add_argument("--serve-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (every network, probe batch "
