@@ -151,12 +151,31 @@ class PresentResult:
 class Net:
     """One network resident in HBM (planes, connectivity, bias, stimuli)."""
 
-    def __init__(self, config: Config):
+    def __init__(self, config: Config, _handle: Optional[ctypes.c_void_p] = None):
         self.config = config
-        self._h = ctypes.c_void_p()
-        _check(lib().axon_net_create(ctypes.byref(config.raw), ctypes.byref(self._h)))
+        if _handle is None:
+            self._h = ctypes.c_void_p()
+            _check(lib().axon_net_create(ctypes.byref(config.raw), ctypes.byref(self._h)))
+        else:
+            self._h = _handle
         self.ncol = config.ncol
         self.n = config.ncol * 32
+
+    def save(self, path: str) -> None:
+        """Write a versioned snapshot (planes, conn, bias, stimulus columns; axon_net_save)."""
+        _check(lib().axon_net_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path: str) -> "Net":
+        """A network created from a snapshot file (axon_net_load)."""
+        h = ctypes.c_void_p()
+        _check(lib().axon_net_load(os.fsencode(path), ctypes.byref(h)))
+        c = ConfigStruct()
+        st = lib().axon_net_config(h, ctypes.byref(c))
+        if st != 0:
+            lib().axon_net_destroy(h)
+            _check(st)
+        return cls(Config(c), _handle=h)
 
     def close(self) -> None:
         if self._h:

@@ -15,6 +15,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <cerrno>
 #include <random>
 #include <string>
 #include <tuple>
@@ -138,6 +140,8 @@ extern "C" const char* axon_status_str(axon_status s) {
         case AXON_ERR_INVALID_ARG: return "invalid argument";
         case AXON_ERR_CAPACITY: return "capacity exceeded";
         case AXON_ERR_NOT_READY: return "not ready";
+        case AXON_ERR_IO: return "file error";
+        case AXON_ERR_BAD_SNAPSHOT: return "bad snapshot";
     }
     return "unknown status";
 }
@@ -1544,5 +1548,141 @@ extern "C" axon_status axon_server_completed(const axon_server* s, uint64_t* ids
     const uint32_t k = (uint32_t)std::min<size_t>(max, s->last_done.size());
     if (k) memcpy(ids, s->last_done.data(), (size_t)k * 8);
     *n = (uint32_t)s->last_done.size();
+    return AXON_OK;
+}
+
+// ================================================================== snapshots
+// Trained-network snapshot file, version 1 (layout in axon_api.h).  The planes
+// are stored in the reference layout through axon_read_buffer, so a snapshot
+// does not depend on the engine's internal tile order.
+namespace {
+constexpr char kSnapMagic[8] = {'A', 'X', 'O', 'N', 'S', 'N', 'A', 'P'};
+constexpr uint32_t kSnapVersion = 1;
+constexpr axon_buffer kSnapBufs[6] = {AXON_BUF_W0, AXON_BUF_W1, AXON_BUF_W2, AXON_BUF_W3, AXON_BUF_CONN, AXON_BUF_BIAS};
+
+struct SnapHeader {
+    char magic[8];
+    uint32_t version, cfgBytes;
+    axon_config cfg;
+    uint64_t words[6];  // per stored buffer, kSnapBufs order
+    uint32_t nstim, stimcols;
+};
+
+struct Fnv1a {  // 64-bit FNV-1a
+    uint64_t h = 14695981039346656037ull;
+    void add(const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+};
+
+// buffer sizes (words) a network of this config has, kSnapBufs order
+void snap_words(const axon_config& c, uint64_t w[6]) {
+    const uint64_t N = (uint64_t)c.ncol * axon::NPC;
+    for (int i = 0; i < 4; ++i) w[i] = (uint64_t)c.ncol * axon::NSLOT * axon::TILE_WORDS;
+    w[4] = (uint64_t)c.ncol * axon::NSLOT;
+    w[5] = N;
+}
+}  // namespace
+
+extern "C" axon_status axon_net_config(const axon_net* n, axon_config* out) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    *out = n->cfg;
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_net_save(axon_net* n, const char* path) {
+    if (!n || !path) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    SnapHeader h{};
+    memcpy(h.magic, kSnapMagic, 8);
+    h.version = kSnapVersion;
+    h.cfgBytes = (uint32_t)sizeof(axon_config);
+    h.cfg = n->cfg;
+    snap_words(n->cfg, h.words);
+    h.nstim = (uint32_t)n->stimCols.size();
+    h.stimcols = h.nstim ? (uint32_t)n->stimCols[0].size() : 0u;
+    for (int i = 0; i < 6; ++i) {
+        size_t w = 0;
+        axon_status st = axon_buffer_words(n, kSnapBufs[i], &w);
+        if (st != AXON_OK) return st;
+        if (w < h.words[i]) return fail(AXON_ERR_INVALID_ARG, "buffer %d holds %zu words, expected %llu", i, w,
+                                        (unsigned long long)h.words[i]);
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(AXON_ERR_IO, "cannot create %s: %s", path, strerror(errno));
+    Fnv1a sum;
+    bool ok = true;
+    auto put = [&](const void* p, size_t bytes) {
+        if (ok && fwrite(p, 1, bytes, f) != bytes) ok = false;
+        sum.add(p, bytes);
+    };
+    put(&h, sizeof h);
+    std::vector<uint32_t> buf;
+    axon_status st = AXON_OK;
+    for (int i = 0; i < 6 && st == AXON_OK && ok; ++i) {
+        buf.resize(h.words[i]);
+        st = axon_read_buffer(n, kSnapBufs[i], buf.data(), buf.size() * 4);
+        if (st == AXON_OK) put(buf.data(), buf.size() * 4);
+    }
+    for (const std::vector<uint32_t>& s : n->stimCols) put(s.data(), s.size() * 4);
+    const uint64_t digest = sum.h;
+    if (ok && fwrite(&digest, 1, 8, f) != 8) ok = false;
+    if (fclose(f) != 0) ok = false;
+    if (st != AXON_OK) return st;
+    if (!ok) return fail(AXON_ERR_IO, "writing %s failed", path);
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_net_load(const char* path, axon_net** out) {
+    if (!path || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(AXON_ERR_IO, "cannot open %s: %s", path, strerror(errno));
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, fclose);
+    SnapHeader h{};
+    if (fread(&h, 1, sizeof h, f) != sizeof h) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: shorter than a header", path);
+    if (memcmp(h.magic, kSnapMagic, 8) != 0) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: not an AXON snapshot", path);
+    if (h.version != kSnapVersion)
+        return fail(AXON_ERR_BAD_SNAPSHOT, "%s: snapshot version %u, this engine reads %u", path, h.version,
+                    kSnapVersion);
+    if (h.cfgBytes != sizeof(axon_config)) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: config size differs", path);
+    if (axon_config_validate(&h.cfg) != AXON_OK)
+        return fail(AXON_ERR_BAD_SNAPSHOT, "%s: stored config is invalid: %s", path, g_err.c_str());
+    uint64_t want[6];
+    snap_words(h.cfg, want);
+    for (int i = 0; i < 6; ++i)
+        if (h.words[i] != want[i]) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: buffer %d size differs from its config", path, i);
+    if (h.nstim != h.cfg.stim || h.stimcols != h.cfg.stimcols)
+        return fail(AXON_ERR_BAD_SNAPSHOT, "%s: stimulus table size differs from its config", path);
+    Fnv1a sum;
+    sum.add(&h, sizeof h);
+    std::vector<std::vector<uint32_t>> bufs(6);
+    for (int i = 0; i < 6; ++i) {
+        bufs[i].resize(h.words[i]);
+        if (fread(bufs[i].data(), 4, bufs[i].size(), f) != bufs[i].size())
+            return fail(AXON_ERR_BAD_SNAPSHOT, "%s: truncated", path);
+        sum.add(bufs[i].data(), bufs[i].size() * 4);
+    }
+    std::vector<uint32_t> stim((size_t)h.nstim * h.stimcols);
+    if (fread(stim.data(), 4, stim.size(), f) != stim.size()) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: truncated", path);
+    sum.add(stim.data(), stim.size() * 4);
+    uint64_t digest = 0;
+    if (fread(&digest, 1, 8, f) != 8) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: truncated", path);
+    if (digest != sum.h) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: checksum mismatch", path);
+    if (fgetc(f) != EOF) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: trailing bytes after the checksum", path);
+    axon_net* n = nullptr;
+    axon_status st = axon_net_create(&h.cfg, &n);
+    if (st != AXON_OK) return st;
+    for (size_t q = 0; q < n->stimCols.size(); ++q)
+        if (!std::equal(n->stimCols[q].begin(), n->stimCols[q].end(), stim.begin() + q * h.stimcols)) {
+            axon_net_destroy(n);
+            return fail(AXON_ERR_BAD_SNAPSHOT, "%s: stimulus columns of pattern %zu differ from its config's", path, q);
+        }
+    for (int i = 0; i < 6 && st == AXON_OK; ++i) st = axon_write_buffer(n, kSnapBufs[i], bufs[i].data(), bufs[i].size() * 4);
+    if (st != AXON_OK) {
+        axon_net_destroy(n);
+        return st;
+    }
+    *out = n;
     return AXON_OK;
 }

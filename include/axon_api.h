@@ -34,7 +34,9 @@ typedef enum axon_status {
     AXON_ERR_DEVICE = 4,          /* HIP runtime error or no usable gfx950 device */
     AXON_ERR_INVALID_ARG = 5,
     AXON_ERR_CAPACITY = 6, /* e.g. T beyond the history capacity max_t */
-    AXON_ERR_NOT_READY = 7 /* serving: no result available yet */
+    AXON_ERR_NOT_READY = 7, /* serving: no result available yet */
+    AXON_ERR_IO = 8,        /* a file could not be opened, read or written */
+    AXON_ERR_BAD_SNAPSHOT = 9 /* not a snapshot of this version, wrong sizes, or a checksum mismatch */
 } axon_status;
 
 /* Every key=value tunable of the reference CLI, mm:687-735 (defaults mm:688-728,
@@ -66,6 +68,21 @@ axon_status axon_config_validate(const axon_config* c);
  * dispatch mm:770-782 and the stimulus draw mm:793-799 ---- */
 axon_status axon_net_create(const axon_config* c, axon_net** out);
 void axon_net_destroy(axon_net* n);
+/* the network's configuration (as created or loaded) */
+axon_status axon_net_config(const axon_net* n, axon_config* out);
+
+/* Trained-network snapshots.  No counterpart in the reference, which retrains
+ * from its seed on every run; for serving residency and sweeps.  Version-1
+ * file, little-endian: a header (magic "AXONSNAP", version, the network's
+ * axon_config, the buffer sizes), the four planes in the reference layout
+ * (mm:170), conn, bias, the stimulus columns of every pattern (mm:793-799),
+ * then a 64-bit FNV-1a checksum of every byte before it.
+ * axon_net_load checks magic, version, sizes, the config and the checksum
+ * before any device work, then creates the network from the stored config
+ * and writes its buffers; the stimulus columns it regenerates from the config
+ * must equal the stored ones. */
+axon_status axon_net_save(axon_net* n, const char* path);
+axon_status axon_net_load(const char* path, axon_net** out);
 
 /* ---- one presentation: replaces present() mm:815-888 ----
  * use_bias: 0, 1 or 2 (= auto: on while learning, off at readout, mm:822).
