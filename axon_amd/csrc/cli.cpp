@@ -77,6 +77,16 @@ struct Engine {
     }
 };
 
+// String-valued keys of this CLI (no reference counterpart): prefix + '=',
+// the last occurrence wins, as argu() does for the numeric keys (mm:660-667).
+const char* arg_str(int argc, char** argv, const char* key) {
+    const char* v = nullptr;
+    const size_t kl = strlen(key);
+    for (int i = 1; i < argc; ++i)
+        if (!strncmp(argv[i], key, kl) && argv[i][kl] == '=') v = argv[i] + kl + 1;
+    return v;
+}
+
 double overlap(const Vec& a, const Vec& b) {  // mm:896-901
     if (a.empty() || b.empty()) return 0.0;
     Vec in;
@@ -98,6 +108,12 @@ int main(int argc, char** argv) {
     axon_config& c = E.c;
     CHECK(axon_config_parse(argc, (const char* const*)argv, &c));
     CHECK(axon_config_validate(&c));
+    // save=<path>: snapshot of the network after its training epochs (before
+    // the extra presentation).  load=<path>: start from such a snapshot (made
+    // with the same keys) instead of running the epochs; everything after
+    // them -- the extra presentation, references and readouts -- is unchanged.
+    const char* savePath = arg_str(argc, argv, "save");
+    const char* loadPath = arg_str(argc, argv, "load");
     const uint32_t NCOL = c.ncol, N = NCOL * NPC, IN = c.incols, NSTIM = c.stim, T = c.T, CLAMP = c.clampsteps;
     const uint32_t K = c.k, targetC = c.target, biasUp = c.biasup, inBranches = c.inbr, EPOCHS = c.epochs;
     const uint32_t trainClamp = c.trainclamp, quiet = c.quiet;
@@ -219,8 +235,31 @@ int main(int argc, char** argv) {
     // counters are read at the end of each epoch that prints a line.
     double msAcc = 0;
     int msN = 0;
+    if (loadPath) {
+        // the step time comes from learning presentations on this untrained
+        // network, which the snapshot then replaces
+        const uint32_t np = std::min(NSTIM, 12u);
+        CHECK(axon_sync(E.n));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t p = 0; p < np; ++p) E.present(p, 1.0, T, trainClamp, true, false, 1000 + p, 2u, false);
+        CHECK(axon_sync(E.n));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        axon_net_destroy(E.n);
+        E.n = nullptr;
+        CHECK(axon_net_load(loadPath, &E.n));
+        axon_config lc{};
+        CHECK(axon_net_config(E.n, &lc));
+        if (memcmp(&lc, &c, sizeof c) != 0) {
+            fprintf(stderr, "axon: %s was saved with other keys than this run's\n", loadPath);
+            axon_net_destroy(E.n);
+            return 2;
+        }
+        if (!quiet) printf("  loaded trained network %s (epochs not rerun)\n", loadPath);
+        msAcc = ms / T;
+        msN = (int)np;
+    }
     const auto tTrain = std::chrono::steady_clock::now();
-    for (uint32_t ep = 0; ep < EPOCHS; ++ep) {
+    for (uint32_t ep = 0; ep < (loadPath ? 0u : EPOCHS); ++ep) {
         const bool pr = !quiet && (ep % 5 == 4 || ep == 0);
         for (uint32_t p = 0; p < NSTIM; ++p) {
             E.present(p, 1.0, T, trainClamp, true, false, 1000 + p, 2u, pr && p + 1 == NSTIM);
@@ -235,9 +274,14 @@ int main(int argc, char** argv) {
         }
     }
     CHECK(axon_sync(E.n));
-    if (msN) msAcc = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tTrain).count() / T;
+    if (msN && !loadPath)
+        msAcc = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tTrain).count() / T;
     if (msN == 0) { msAcc = E.present(0, 1.0, T, T, false, true, 777); msN = 1; }
     const double msStep = msAcc / msN;
+    if (savePath) {
+        CHECK(axon_net_save(E.n, savePath));
+        if (!quiet) printf("  saved trained network %s\n", savePath);
+    }
 
     {  // firing steps per neuron, one extra training presentation (mm:1028-1072)
         E.present(0, 1.0, T, trainClamp, true, false, 31337);
