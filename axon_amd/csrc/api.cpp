@@ -1608,8 +1608,11 @@ extern "C" axon_status axon_net_save(axon_net* n, const char* path) {
         if (w < h.words[i]) return fail(AXON_ERR_INVALID_ARG, "buffer %d holds %zu words, expected %llu", i, w,
                                         (unsigned long long)h.words[i]);
     }
-    FILE* f = fopen(path, "wb");
-    if (!f) return fail(AXON_ERR_IO, "cannot create %s: %s", path, strerror(errno));
+    // written under a temporary name and renamed when complete, so a reader
+    // never sees a partial snapshot at `path`
+    const std::string tmp = std::string(path) + ".partial";
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return fail(AXON_ERR_IO, "cannot create %s: %s", tmp.c_str(), strerror(errno));
     Fnv1a sum;
     bool ok = true;
     auto put = [&](const void* p, size_t bytes) {
@@ -1628,8 +1631,15 @@ extern "C" axon_status axon_net_save(axon_net* n, const char* path) {
     const uint64_t digest = sum.h;
     if (ok && fwrite(&digest, 1, 8, f) != 8) ok = false;
     if (fclose(f) != 0) ok = false;
-    if (st != AXON_OK) return st;
-    if (!ok) return fail(AXON_ERR_IO, "writing %s failed", path);
+    if (st != AXON_OK || !ok) {
+        remove(tmp.c_str());
+        return st != AXON_OK ? st : fail(AXON_ERR_IO, "writing %s failed", tmp.c_str());
+    }
+    if (rename(tmp.c_str(), path) != 0) {
+        const int err = errno;
+        remove(tmp.c_str());
+        return fail(AXON_ERR_IO, "cannot rename %s to %s: %s", tmp.c_str(), path, strerror(err));
+    }
     return AXON_OK;
 }
 
