@@ -502,6 +502,14 @@ def probe_autotune(net: Net, stims: np.ndarray, T: int, clamp_steps: int, use_bi
     return best.value, {t: ms[i] for i, t in enumerate(PROBE_TILES)}
 
 
+def mem_probe(kind: int, nbytes: int, reps: int = 5) -> float:
+    """Read GB/s of a streaming pass (0), 128-byte line gathers (1) or 1 KiB tile gathers (2)."""
+    g = ctypes.c_double()
+    _check(lib().axon_mem_probe(ctypes.c_uint32(kind), ctypes.c_uint64(nbytes), ctypes.c_uint32(reps),
+                                ctypes.byref(g)))
+    return g.value
+
+
 def probe_cue(net: Net, pattern: int, nreplace: int, seed: int) -> np.ndarray:
     """Recall cue: pattern's stimulus columns with nreplace of them swapped for
     other input columns; pattern = -1 draws a random cue."""

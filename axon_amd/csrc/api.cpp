@@ -1696,3 +1696,50 @@ extern "C" axon_status axon_net_load(const char* path, axon_net** out) {
     *out = n;
     return AXON_OK;
 }
+
+// ================================================================== memory probes
+extern "C" axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t reps, double* gbps) {
+    if (!gbps || kind > 2u || reps == 0u) return fail(AXON_ERR_INVALID_ARG, "kind 0..2, reps >= 1, gbps not null");
+    if (bytes < 1024u || bytes % 1024u || bytes > (64ull << 30))
+        return fail(AXON_ERR_INVALID_ARG, "bytes must be a multiple of 1 KiB in 1 KiB..64 GiB");
+    int dev = 0;
+    axon_status st = select_device(&dev);
+    if (st != AXON_OK) return st;
+    void* p = nullptr;
+    uint32_t* out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (s) (void)hipStreamSynchronize(s);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        if (out) (void)hipFree(out);
+        if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&p, bytes) != hipSuccess || hipMalloc((void**)&out, 64) != hipSuccess) {
+        cleanup();
+        return fail(AXON_ERR_OUT_OF_MEMORY, "memory probe: hipMalloc(%llu)", (unsigned long long)bytes);
+    }
+    const uint32_t blocks = kind == 0 ? 4096u : 8192u, iters = 64u;
+    double best = 0.0;
+    hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(p, 0, bytes, s);
+    if (e == hipSuccess) e = axon::launch_mem_probe(kind, (const uint32_t*)p, bytes, blocks, iters, out, s);  // warm-up
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
+        e = hipEventRecord(e0, s);
+        if (e == hipSuccess) e = axon::launch_mem_probe(kind, (const uint32_t*)p, bytes, blocks, iters, out, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0.f)
+            best = std::max(best, (double)axon::mem_probe_bytes(kind, bytes, blocks, iters) / (ms * 1e-3) / 1e9);
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "memory probe: %s", hipGetErrorString(e));
+    *gbps = best;
+    return AXON_OK;
+}

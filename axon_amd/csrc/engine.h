@@ -139,6 +139,10 @@ hipError_t launch_untouched(const NetDev& d, uint32_t* out, hipStream_t s);
 hipError_t launch_wdiag(const NetDev& d, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
                         uint32_t sel, uint32_t* hist, hipStream_t s);
 hipError_t launch_fill(uint32_t* p, uint32_t value, size_t nwords, hipStream_t s);
+// memory probes: kind 0 streaming read, 1 128-byte line gathers, 2 1 KiB tile gathers
+hipError_t launch_mem_probe(uint32_t kind, const uint32_t* p, size_t bytes, uint32_t blocks, uint32_t iters,
+                            uint32_t* out, hipStream_t s);
+size_t mem_probe_bytes(uint32_t kind, size_t bytes, uint32_t blocks, uint32_t iters);
 hipError_t launch_checksum(const uint32_t* p, size_t n, unsigned long long* out, hipStream_t s);
 
 hipError_t launch_bstest(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1, uint32_t* a2,

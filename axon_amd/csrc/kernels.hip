@@ -1852,6 +1852,59 @@ hipError_t launch_multi_end(const MultiDev& m, hipStream_t s) {
 
 }  // namespace axon
 
+// ============================================================== memory probes
+// Read bandwidth of the access shapes the engine uses, to calibrate its
+// percent-of-peak figures: a streaming pass (16 B per lane), 128-byte line
+// gathers (a half-wave per line, the propagation's W3 reads) and 1 KiB tile
+// gathers (a wave per tile), at pseudo-random unit-aligned places.  Each
+// result is folded into a value that is stored only if it matches a constant
+// that a fill of zeros never produces, so the loads stay and nothing is
+// written.
+namespace axon {
+namespace {
+__global__ __launch_bounds__(256) void k_mem_stream(const uint4* __restrict__ p, size_t n16, uint32_t* out) {
+    uint32_t x = 0;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256ull) {
+        const uint4 v = p[i];
+        x ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (x == 0x9E3779B9u) out[0] = x;
+}
+// unit = 128: lane (h, n) reads dword n of line (wave, it, h); unit = 1024: lane l
+// reads 16 B at 16 l of tile (wave, it).  nunits = buffer bytes / unit.
+template <uint32_t UNIT>
+__global__ __launch_bounds__(256) void k_mem_gather(const uint32_t* __restrict__ p, uint32_t nunits, uint32_t iters,
+                                                    uint32_t* out) {
+    const uint32_t lane = lane_id(), wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+    uint32_t x = 0;
+#pragma unroll 8
+    for (uint32_t it = 0; it < iters; ++it) {
+        if (UNIT == 128u) {
+            const uint32_t u = hash32(wave * 2654435761u + it * 40503u + (lane >> 5)) % nunits;
+            x ^= p[(size_t)u * 32u + (lane & 31u)];
+        } else {
+            const uint32_t u = hash32(wave * 2654435761u + it * 40503u) % nunits;
+            const uint4 v = reinterpret_cast<const uint4*>(p + (size_t)u * 256u)[lane];
+            x ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    if (x == 0x9E3779B9u) out[0] = x;
+}
+}  // namespace
+// bytes read by one launch
+size_t mem_probe_bytes(uint32_t kind, size_t bytes, uint32_t blocks, uint32_t iters) {
+    if (kind == 0) return bytes;
+    return (size_t)blocks * 4u * iters * (kind == 1 ? 2u * 128u : 1024u);
+}
+hipError_t launch_mem_probe(uint32_t kind, const uint32_t* p, size_t bytes, uint32_t blocks, uint32_t iters,
+                            uint32_t* out, hipStream_t s) {
+    if (kind == 0) k_mem_stream<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint4*>(p), bytes / 16u, out);
+    else if (kind == 1) k_mem_gather<128><<<blocks, 256, 0, s>>>(p, (uint32_t)(bytes / 128u), iters, out);
+    else k_mem_gather<1024><<<blocks, 256, 0, s>>>(p, (uint32_t)(bytes / 1024u), iters, out);
+    return hipGetLastError();
+}
+}  // namespace axon
+
 namespace axon {
 hipError_t launch_checksum(const uint32_t* p, size_t n, unsigned long long* out, hipStream_t s) {
     k_checksum<<<2048, 256, 0, s>>>(p, n, out);

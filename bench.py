@@ -85,6 +85,19 @@ def whole_run_hbm(ncol, steps, lines, seconds, bias=True):
             "peak": HBM_PEAK_GBS, "frac": round(gbps / HBM_PEAK_GBS, 4)}
 
 
+def hbm_calibration():
+    """Achieved read bandwidth of the engine's access shapes, beyond the 256 MiB
+    Infinity Cache (4 GiB) and within it (one plane, 128 MiB)."""
+    import axon_amd
+
+    out = {}
+    for name, kind in (("stream", 0), ("line_128B_gather", 1), ("tile_1KiB_gather", 2)):
+        for label, nbytes in (("4GiB", 4 << 30), ("128MiB", 128 << 20)):
+            g = axon_amd.mem_probe(kind, nbytes, 5)
+            out[f"{name}_{label}"] = {"GBps": round(g, 1), "frac_of_peak": round(g / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def gpu_bench(args):
     import axon_amd
 
@@ -359,7 +372,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="c1,probes,networks,serving",
-                    help="comma list of c1, probes, networks, serving (copy: counter calibration only)")
+                    help="comma list of c1, probes, networks, serving, calib (copy: counter calibration only)")
     ap.add_argument("--net-epochs", type=int, default=40)
     ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
@@ -378,6 +391,7 @@ def main():
         copy_calibration(args)
         return
     g = gpu_bench(args) if "c1" in sections else None
+    calib = hbm_calibration() if ("c1" in sections or "calib" in sections) else None
     batched = {}
     if "probes" in sections:
         batched["probes"] = c3_probes(args)
@@ -387,7 +401,7 @@ def main():
     cpu = None if args.no_cpu else cpu_bench(args)
     if g is None:  # a run of the batched sections alone (profiling): no headline figure
         print(json.dumps({"sections": sorted(sections), "batched_vs_plain": batched, "serving": serving,
-                          "cpu_baseline": cpu}), flush=True)
+                          "hbm_calibration": calib, "cpu_baseline": cpu}), flush=True)
         return
     out = {
         "metric": METRIC,
@@ -412,6 +426,7 @@ def main():
                     for k, v in g["kernels"].items()},
         "roofline": g["roofline"],
         "whole_run_hbm": g["whole_run_hbm"],
+        "hbm_calibration": calib,
         "batched_vs_plain": batched,
         "serving": serving,
         "cpu_baseline": cpu,

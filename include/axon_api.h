@@ -221,6 +221,14 @@ axon_status axon_multi_train(axon_multi* b, const uint32_t* epochs /* nnet */, u
 /* order-independent 64-bit checksum of an engine buffer (state comparison) */
 axon_status axon_checksum(axon_net* n, axon_buffer which, uint64_t* sum);
 
+/* Read bandwidth of the engine's access shapes over a `bytes` buffer (a
+ * multiple of 1 KiB, at most 64 GiB): kind 0 a streaming pass, 1 128-byte
+ * line gathers, 2 1 KiB tile gathers (pseudo-random places).  Best of `reps`
+ * timed launches after one warm-up, in GB/s (1e9 bytes/s).  The counterpart
+ * of the reference's Apple probes (bench/probe04_roofline.mm): calibration
+ * of the percent-of-peak figures, not part of the hot path. */
+axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t reps, double* gbps);
+
 /* ---- serving: continuous batching of recall queries ----
  * Extension with no reference counterpart.  Trained networks stay resident;
  * each query is one readout presentation (a probe) of T steps on one network.

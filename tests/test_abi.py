@@ -82,3 +82,12 @@ def test_net_create_without_a_gpu_fails_loudly():
     with pytest.raises(axon_amd.AxonError) as e:
         axon_amd.Net(axon_amd.Config.parse(C0))
     assert e.value.status == 4  # AXON_ERR_DEVICE, no fallback path
+
+
+def test_mem_probe_rejects_bad_arguments():
+    """Argument checks come before any device work (no GPU needed)."""
+    import axon_amd
+    for kind, nbytes, reps in ((3, 1 << 20, 1), (0, 1000, 1), (0, 1 << 20, 0), (1, 65 << 30, 1)):
+        with pytest.raises(axon_amd.AxonError) as e:
+            axon_amd.mem_probe(kind, nbytes, reps)
+        assert e.value.status == 5

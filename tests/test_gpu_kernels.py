@@ -72,3 +72,13 @@ def test_topk_batched_instances():
     th = axon_amd.topk_test(d, 512, 64, True)
     for i in range(d.shape[0]):
         assert tuple(int(x) for x in th[i]) == oracle.topk(d[i], 512, 64, True)
+
+
+def test_mem_probe_runs_and_reports_plausible_bandwidth():
+    """The calibration probes run (each shape, a small and a cache-exceeding
+    size) and report a positive rate below the 8 TB/s peak."""
+    for kind in (0, 1, 2):
+        for nbytes in (64 << 20, 1 << 30):
+            g = axon_amd.mem_probe(kind, nbytes, 2)
+            print(f"mem probe kind {kind} {nbytes >> 20} MiB: {g:.0f} GB/s", flush=True)
+            assert 100.0 < g < 8000.0
