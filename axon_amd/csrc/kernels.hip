@@ -1601,14 +1601,13 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
     emit_pair<LEARN>(d, P, (t & 1u) ? s0 : s1, colA);
 }
 
-__global__ __launch_bounds__(256) void k_mb_learn(MultiDev m, Params P, uint32_t t) {
+__global__ __launch_bounds__(64) void k_mb_learn(MultiDev m, Params P, uint32_t t) {  // single-wave workgroups
     const uint32_t net = blockIdx.y;
     const NetDev d = mb_net(m, net);
     if (d.active == 0xFFFFFFFFu) return;
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    learn_tiles(d, P, (t & 1u) ? s1 : s0, blockIdx.x * 4u + wave, gridDim.x * 4u);
+    learn_tiles(d, P, (t & 1u) ? s1 : s0, blockIdx.x, gridDim.x);
 }
 
 __global__ void k_mb_end(MultiDev m) {
@@ -1842,7 +1841,7 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
     return hipGetLastError();
 }
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
-    k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 256u), m.nnet), 256, 0, s>>>(m, P, t);
+    k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 256u) * 4u, m.nnet), 64, 0, s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s) {
