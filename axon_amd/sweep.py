@@ -21,14 +21,11 @@ import subprocess
 import sys
 import time
 
-from . import Config, MultiNet, Net
+from . import CONFIG_C, Config, MultiNet, Net
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 AXON_BIN = os.path.join(HERE, "bin", "axon")
 
-# Configuration C of the reference scripts (bench/seeds.sh:6, bench/cap10.sh:7)
-CONFIG_C = ("nand=4 target=64 k=24 dthr=1 mode=1 inbr=4 stimcols=64 contrast=1 margingate=0 "
-            "ltdrep=1 ltdevery=2 biasup=73 biasdown=1 T=10 clampsteps=5").split()
 READOUT = "wave=1 wavet=35 deep=1".split()
 
 
