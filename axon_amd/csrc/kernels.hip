@@ -1513,11 +1513,14 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     drive_pair(d, P, in, colA, di, f, items[threadIdx.x >> 6]);
 }
 
-__global__ __launch_bounds__(1024) void k_mb_topk(MultiDev m, Params P) {
+// 256 threads per network: this launch is a latency chain of a few barriers,
+// cheaper with 4 waves than with 16 (measured), and it leaves the device to
+// the other stream groups' kernels
+__global__ __launch_bounds__(256) void k_mb_topk(MultiDev m, Params P) {
     const uint32_t net = blockIdx.x;
     const NetDev d = mb_net(m, net);
     if (d.active == 0xFFFFFFFFu) return;
-    topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+    topk_block<256>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
 // k_emit + k_bias + the plasticity listing (as emit_cols) for two columns
@@ -1831,7 +1834,7 @@ hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, ui
     return hipGetLastError();
 }
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s) {
-    k_mb_topk<<<m.nnet, 1024, 0, s>>>(m, P);
+    k_mb_topk<<<m.nnet, 256, 0, s>>>(m, P);
     return hipGetLastError();
 }
 hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
