@@ -72,6 +72,7 @@ BUFFERS = ("W0 W1 W2 W3 CONN SPK_A SPK_B DRIVE BBITS ACC BIAS MARG REFBALL DRIVE
            "STIM STATS HIST THETA").split()
 PRESENT_ASYNC = 1
 PROBE_TILES = (1, 2, 4, 8, 16, 32, 64)  # axon_probe_autotune's order (AXON_PROBE_TILES)
+PROBE_WAVES = (0, 32, 16)  # waves per CU, its order of occupancies (AXON_PROBE_OCCUPANCIES)
 
 _lib = None
 
@@ -310,11 +311,12 @@ class Net:
 class Probes:
     """A probe batch: `slots` readout presentations against one resident network."""
 
-    def __init__(self, net: Net, slots: int, max_t: int, tile: int = 4):
+    def __init__(self, net: Net, slots: int, max_t: int, tile: int = 4, waves_per_cu: int = 0):
         self.net = net
         self._h = ctypes.c_void_p()
         _check(lib().axon_probes_create(net._h, ctypes.c_uint32(slots), ctypes.c_uint32(max_t),
                                         ctypes.c_uint32(tile), ctypes.byref(self._h)))
+        _check(lib().axon_probes_set_occupancy(self._h, ctypes.c_uint32(waves_per_cu)))
         s, m, t = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().axon_probes_info(self._h, ctypes.byref(s), ctypes.byref(m), ctypes.byref(t)))
         self.slots, self.max_t, self.tile = s.value, m.value, t.value
@@ -393,18 +395,18 @@ class MultiNet:
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots_per_net", ctypes.c_uint32), ("max_t", ctypes.c_uint32), ("tile", ctypes.c_uint32),
                 ("steps_per_tick", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("waves_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class Server:
     """Continuous-batching recall server over resident networks."""
 
     def __init__(self, nets: Sequence[Net], slots_per_net: int = 64, max_t: int = 200, tile: int = 4,
-                 steps_per_tick: int = 5, hbm_budget: int = 0):
+                 steps_per_tick: int = 5, hbm_budget: int = 0, waves_per_cu: int = 0):
         self.nets = list(nets)
         self.ncol = self.nets[0].ncol
         arr = (ctypes.c_void_p * len(self.nets))(*[n._h.value for n in self.nets])
-        cfg = ServerConfig(slots_per_net, max_t, tile, steps_per_tick, hbm_budget)
+        cfg = ServerConfig(slots_per_net, max_t, tile, steps_per_tick, hbm_budget, waves_per_cu)
         self._h = ctypes.c_void_p()
         L = lib()
         L.axon_server_clock.restype = ctypes.c_double
@@ -493,13 +495,16 @@ def checksum(net: Net, which: str) -> int:
 
 
 def probe_autotune(net: Net, stims: np.ndarray, T: int, clamp_steps: int, use_bias: int = 0):
-    """Fastest probe tile for this batch: (tile, {tile: ms})."""
+    """Fastest (tile, waves per CU) for this batch and the time of every pair:
+    (tile, waves, {(tile, waves): ms})."""
     stims = np.ascontiguousarray(stims, np.uint32).reshape(-1, net.ncol)
-    best = ctypes.c_uint32()
-    ms = (ctypes.c_double * len(PROBE_TILES))()
+    best, waves = ctypes.c_uint32(), ctypes.c_uint32()
+    ms = (ctypes.c_double * (len(PROBE_TILES) * len(PROBE_WAVES)))()
     _check(lib().axon_probe_autotune(net._h, ctypes.c_uint32(stims.shape[0]), _ptr(stims), ctypes.c_uint32(T),
-                                     ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias), ctypes.byref(best), ms))
-    return best.value, {t: ms[i] for i, t in enumerate(PROBE_TILES)}
+                                     ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias), ctypes.byref(best),
+                                     ctypes.byref(waves), ms))
+    table = {(t, w): ms[i * len(PROBE_WAVES) + j] for i, t in enumerate(PROBE_TILES) for j, w in enumerate(PROBE_WAVES)}
+    return best.value, waves.value, table
 
 
 def mem_probe(kind: int, nbytes: int, reps: int = 5) -> float:

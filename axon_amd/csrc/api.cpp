@@ -792,12 +792,16 @@ extern "C" axon_status axon_topk_test(const uint32_t* drive, uint32_t ninst, uin
 // probe tiles (probes sharing one propagation wave), in the autotuner's order
 static const uint32_t kProbeTiles[AXON_PROBE_TILES] = {1, 2, 4, 8, 16, 32, 64};
 
+// waves per CU the probe propagation may keep resident, in the autotuner's order (0 = as many as fit)
+static const uint32_t kProbeWaves[AXON_PROBE_OCCUPANCIES] = {0, 32, 16};
+
 struct axon_probes {
     axon_net* net = nullptr;
     axon::ProbeDev d{};
     uint32_t tile = 32;
+    uint32_t waves = 0;  // waves per CU of the propagation, 0 = no limit
     std::vector<void*> allocs;
-    std::map<uint64_t, hipGraphExec_t> graphs;  // keyed by (steps per launch, slots advanced)
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, hipGraphExec_t> graphs;  // (steps, slots advanced, waves)
 };
 
 extern "C" void axon_probes_destroy(axon_probes* b) {
@@ -856,6 +860,15 @@ extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t 
     return AXON_OK;
 }
 
+extern "C" axon_status axon_probes_set_occupancy(axon_probes* b, uint32_t waves_per_cu) {
+    if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
+    bool known = false;
+    for (uint32_t w : kProbeWaves) known |= w == waves_per_cu;
+    if (!known) return fail(AXON_ERR_INVALID_ARG, "waves per CU must be 0 (no limit), 32 or 16 (got %u)", waves_per_cu);
+    b->waves = waves_per_cu;
+    return AXON_OK;
+}
+
 extern "C" axon_status axon_probes_info(const axon_probes* b, uint32_t* slots, uint32_t* max_t, uint32_t* tile) {
     if (!b) return fail(AXON_ERR_INVALID_ARG, "null batch");
     if (slots) *slots = b->d.nslots;
@@ -896,17 +909,17 @@ static axon_status probes_advance(axon_probes* b, uint32_t steps, uint32_t nact)
     nact = std::min((nact + b->tile - 1u) / b->tile * b->tile, b->d.nslots);
     HIPCHECK(hipSetDevice(n->device));
     if (!n->use_graphs) {
-        for (uint32_t i = 0; i < steps; ++i) HIPCHECK(axon::launch_probe_step(b->d, b->tile, nact, n->stream));
+        for (uint32_t i = 0; i < steps; ++i) HIPCHECK(axon::launch_probe_step(b->d, b->tile, nact, b->waves, n->stream));
         return AXON_OK;
     }
-    const uint64_t key = (uint64_t)steps << 32 | nact;
+    const auto key = std::make_tuple(steps, nact, b->waves);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         for (uint32_t i = 0; i < steps && e == hipSuccess; ++i)
-            e = axon::launch_probe_step(b->d, b->tile, nact, n->stream);
+            e = axon::launch_probe_step(b->d, b->tile, nact, b->waves, n->stream);
         hipError_t e2 = hipStreamEndCapture(n->stream, &g);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "probe capture: %s", hipGetErrorString(e));
         if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -1341,6 +1354,7 @@ extern "C" axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, 
         if (!nets[i] || nets[i]->NCOL != s->ncol) return bail(fail(AXON_ERR_INVALID_ARG, "network %u missing or of another size", i));
         SrvNet& v = s->sn[i];
         axon_status st = axon_probes_create(nets[i], cfg->slots_per_net, cfg->max_t, cfg->tile, &v.pb);
+        if (st == AXON_OK) st = axon_probes_set_occupancy(v.pb, cfg->waves_per_cu);
         if (st != AXON_OK) return bail(st);
         const uint32_t ns = v.pb->d.nslots;
         v.slots.resize(ns);
@@ -1493,31 +1507,39 @@ extern "C" axon_status axon_server_status(const axon_server* s, uint32_t* queued
 
 // ================================================================== autotuning
 // Probe-tile autotuner: runs the caller's probe batch once per tile size
-// (AXON_PROBE_TILES sizes, kProbeTiles), timing the device work with events,
-// and reports the fastest.  Tiles of up to 8 probes list their work per
+// (AXON_PROBE_TILES sizes, kProbeTiles) and per occupancy of the propagation
+// (kProbeWaves: waves per CU left resident), timing the device work with
+// events, and reports the fastest pair.  Tiles of up to 8 probes list their work per
 // (branch, word, probe) item; larger tiles walk the union of the probes'
 // active branches and read each W3 line once for all of them.  Which wins
 // depends on how much the probes' activity overlaps and on the number of
 // probes, so it is measured, not assumed.
 extern "C" axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
                                            uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
-                                           double ms_per_tile[AXON_PROBE_TILES]) {
-    if (!n || !stims || !best_tile) return fail(AXON_ERR_INVALID_ARG, "null argument");
+                                           uint32_t* best_waves_per_cu,
+                                           double ms[AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES]) {
+    if (!n || !stims || !best_tile || !best_waves_per_cu) return fail(AXON_ERR_INVALID_ARG, "null argument");
     const uint32_t* tiles = kProbeTiles;
     double best = 1e30;
-    *best_tile = 8;
+    *best_tile = 4;
+    *best_waves_per_cu = 0;
     HIPCHECK(hipSetDevice(n->device));
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     axon_status st = AXON_OK;
-    for (int i = 0; i < AXON_PROBE_TILES && st == AXON_OK; ++i) {
+    for (int c = 0; c < AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES && st == AXON_OK; ++c) {
+        const int i = c / AXON_PROBE_OCCUPANCIES, o = c % AXON_PROBE_OCCUPANCIES;
         axon_probes* b = nullptr;
         if ((st = axon_probes_create(n, count, T, tiles[i], &b)) != AXON_OK) break;
+        if ((st = axon_probes_set_occupancy(b, kProbeWaves[o])) != AXON_OK) {
+            axon_probes_destroy(b);
+            break;
+        }
         std::vector<uint32_t> Ts(b->d.nslots, 0);
         std::fill(Ts.begin(), Ts.begin() + count, T);
         std::vector<uint32_t> pad((size_t)(b->d.nslots - count) * n->NCOL, 0);
-        double ms = 0;
+        double t_ms = 0;
         for (int rep = 0; rep < 2 && st == AXON_OK; ++rep) {  // first pass captures the graph
             st = axon_probes_load(b, 0, count, stims, Ts.data(), clamp_steps, use_bias);
             if (st == AXON_OK && count < b->d.nslots)
@@ -1531,12 +1553,12 @@ extern "C" axon_status axon_probe_autotune(axon_net* n, uint32_t count, const ui
             }
             float f = 0;
             (void)hipEventElapsedTime(&f, e0, e1);
-            ms = f;
+            t_ms = f;
         }
         axon_probes_destroy(b);
         if (st != AXON_OK) break;
-        if (ms_per_tile) ms_per_tile[i] = ms;
-        if (ms < best) { best = ms; *best_tile = tiles[i]; }
+        if (ms) ms[c] = t_ms;
+        if (t_ms < best) { best = t_ms; *best_tile = tiles[i]; *best_waves_per_cu = kProbeWaves[o]; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -115,8 +115,9 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
                              hipStream_t s);
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s);
-// advances slots [0, nact) (nact <= nslots; the others must be idle)
-hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, hipStream_t s);
+// advances slots [0, nact) (nact <= nslots; the others must be idle);
+// wavesPerCU (0: no limit) caps the propagation's resident waves per CU
+hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, uint32_t wavesPerCU, hipStream_t s);
 hipError_t launch_probe_reset(const ProbeDev& d, uint32_t first, uint32_t count, hipStream_t s);
 hipError_t launch_probe_admit(const ProbeDev& d, const uint32_t* list, uint32_t n, hipStream_t s);
 hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const uint32_t* offs, uint32_t n,

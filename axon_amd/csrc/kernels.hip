@@ -1789,20 +1789,38 @@ hipError_t launch_topk_test(const uint32_t* drive, uint32_t ninst, uint32_t ncol
 
 namespace axon {
 
-hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, hipStream_t s) {
+// Dynamic LDS that limits a kernel to wgPerCU workgroups per CU (0: no limit):
+// each workgroup then holds 160 KiB / wgPerCU of the CU's LDS.
+static uint32_t occupancy_pad(const void* kernel, uint32_t wgPerCU) {
+    if (!wgPerCU) return 0u;
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, kernel) != hipSuccess) return 0u;
+    const size_t want = (160u * 1024u) / wgPerCU;
+    return want > a.sharedSizeBytes ? (uint32_t)(want - a.sharedSizeBytes) : 0u;
+}
+
+template <typename K>
+static void probe_launch(K kernel, uint32_t blocks, uint32_t wgPerCU, hipStream_t s, const ProbeDev& d, uint32_t ncg,
+                         uint32_t ntiles) {
+    const uint32_t pad = occupancy_pad(reinterpret_cast<const void*>(kernel), wgPerCU);
+    kernel<<<blocks, 256, pad, s>>>(d, ncg, ntiles);
+}
+
+hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, uint32_t wavesPerCU, hipStream_t s) {
     nact = std::min(nact, d.nslots);
     if (nact == 0u) return hipSuccess;
     const uint32_t ncg = (d.ncol + 3u) / 4u;
     const uint32_t ntiles = (nact + tile - 1u) / tile;
     const uint32_t blocks = ((ncg + 7u) / 8u) * 8u * ntiles;
+    const uint32_t wg = wavesPerCU / 4u;  // 4 waves per workgroup
     switch (tile) {
-        case 1: k_probe_items<1><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
-        case 2: k_probe_items<2><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
-        case 4: k_probe_items<4><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
-        case 8: k_probe_items<8><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
-        case 16: k_probe_drive<16><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
-        case 32: k_probe_drive<32><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
-        case 64: k_probe_drive<64><<<blocks, 256, 0, s>>>(d, ncg, ntiles); break;
+        case 1: probe_launch(k_probe_items<1>, blocks, wg, s, d, ncg, ntiles); break;
+        case 2: probe_launch(k_probe_items<2>, blocks, wg, s, d, ncg, ntiles); break;
+        case 4: probe_launch(k_probe_items<4>, blocks, wg, s, d, ncg, ntiles); break;
+        case 8: probe_launch(k_probe_items<8>, blocks, wg, s, d, ncg, ntiles); break;
+        case 16: probe_launch(k_probe_drive<16>, blocks, wg, s, d, ncg, ntiles); break;
+        case 32: probe_launch(k_probe_drive<32>, blocks, wg, s, d, ncg, ntiles); break;
+        case 64: probe_launch(k_probe_drive<64>, blocks, wg, s, d, ncg, ntiles); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

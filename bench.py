@@ -197,9 +197,9 @@ def c3_probes(args):
     t_plain = time.perf_counter() - t0
     lines = net.line_counters()
     tiles_read, _ = net.work_counters(reset=True)
-    log("probes: autotuning the probe tile")
-    best, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
-    pb = axon_amd.Probes(net, len(stims), T, best)
+    log("probes: autotuning the probe tile and the propagation's occupancy")
+    best, waves, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
+    pb = axon_amd.Probes(net, len(stims), T, best, waves)
     pb.run(stims, T, clamp, 0, hist=False)  # warm (graph capture)
     _, hist, ms = pb.run(stims, T, clamp, 0, hist=True)
     exact = all(np.array_equal(hist[i], plain[i]) for i in range(len(stims)))
@@ -210,7 +210,8 @@ def c3_probes(args):
     return {"probes": len(stims), "T": T, "plain_gsyn_s": round(syn / t_plain / 1e9, 1),
             "batched_gsyn_s": round(syn / (ms / 1e3) / 1e9, 1), "speedup": round(t_plain / (ms / 1e3), 2),
             "plain_ms": round(1e3 * t_plain, 2), "batched_ms": round(ms, 3), "best_tile": best,
-            "tile_sweep_ms": {str(k): round(v, 3) for k, v in sweep.items()},
+            "best_waves_per_cu": waves,
+            "tile_sweep_ms": {f"{t}/{w}": round(v, 3) for (t, w), v in sweep.items()},
             "plain_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, t_plain, bias=False),
             "batched_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, ms / 1e3, bias=False),
             "bit_exact_vs_plain": bool(exact)}

@@ -193,13 +193,18 @@ axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t count, uin
  * hist is count*T*NCOL (packed), ms the device wall time of the steps */
 axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
                             uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes, double* ms);
+/* waves per CU the batch's propagation may keep resident: 0 (as many as
+ * fit), 32 or 16; fewer leaves each wave more of the CU's caches */
+axon_status axon_probes_set_occupancy(axon_probes* b, uint32_t waves_per_cu);
 /* autotuner: runs this probe batch with every tile size (1/2/4/8/16/32/64
- * probes per propagation wave), times the device work, returns the fastest
- * tile and the per-tile milliseconds in that order */
+ * probes per propagation wave) and every occupancy (0/32/16 waves per CU),
+ * times the device work, returns the fastest pair; ms[tile_index *
+ * AXON_PROBE_OCCUPANCIES + occupancy_index] in those orders */
 #define AXON_PROBE_TILES 7
+#define AXON_PROBE_OCCUPANCIES 3
 axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
                                 uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
-                                double ms_per_tile[AXON_PROBE_TILES]);
+                                uint32_t* best_waves_per_cu, double ms[AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES]);
 /* recall cue: pattern's stimulus columns with `nreplace` of them replaced by
  * other input columns (mt19937(seed) draws); pattern 0xFFFFFFFF = a random cue
  * of stimcols input columns.  Extends loadStim (mm:800-804). */
@@ -247,7 +252,8 @@ typedef struct axon_server_config {
     uint32_t tile;          /* probe tile: 1, 2, 4, 8, 16, 32 or 64 */
     uint32_t steps_per_tick;
     uint64_t hbm_budget;    /* bytes of in-flight query state, 0 = no limit */
-    uint32_t reserved[4];
+    uint32_t waves_per_cu;  /* probe propagation occupancy: 0 (no limit), 32 or 16 */
+    uint32_t reserved[3];
 } axon_server_config;
 axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
                                axon_server** out);

@@ -125,3 +125,31 @@ def test_probe_errors():
     assert e.value.status == 6
     with pytest.raises(axon_amd.AxonError):
         pb.load(7, np.zeros((2, 512), np.uint32), [5, 5], 5)
+
+
+@pytest.mark.parametrize("tile", [1, 4, 16])
+def test_occupancy_limits_change_no_result(tile):
+    """Capping the propagation's waves per CU (LDS padding) changes only timing."""
+    net, _ = trained(C0, 2)
+    stims = cues(net, 24)
+    T, clamp = 30, 5
+    ref = None
+    for waves in axon_amd.PROBE_WAVES:
+        pb = axon_amd.Probes(net, slots=24, max_t=64, tile=tile, waves_per_cu=waves)
+        spk, hist, _ = pb.run(stims, T, clamp, use_bias=0)
+        pb.close()
+        if ref is None:
+            ref = (spk, hist)
+        else:
+            assert np.array_equal(spk, ref[0]) and np.array_equal(hist, ref[1]), f"tile {tile} waves {waves}"
+
+
+def test_autotuner_sweeps_tiles_and_occupancies():
+    net, _ = trained(C0, 2)
+    stims = cues(net, 16)
+    best, waves, table = axon_amd.probe_autotune(net, stims, 20, 5, 0)
+    assert best in axon_amd.PROBE_TILES and waves in axon_amd.PROBE_WAVES
+    assert len(table) == len(axon_amd.PROBE_TILES) * len(axon_amd.PROBE_WAVES)
+    assert all(v > 0 for v in table.values()) and table[(best, waves)] == min(table.values())
+    with pytest.raises(axon_amd.AxonError):
+        axon_amd.Probes(net, slots=8, max_t=32, tile=4, waves_per_cu=24)
