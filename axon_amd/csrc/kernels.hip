@@ -13,6 +13,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "engine.h"
 
@@ -37,6 +38,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     x += __shfl_xor(x, 4);
     x += __shfl_xor(x, 2);
     x += __shfl_xor(x, 1);
+    return x;
+}
+// Inclusive prefix sum over the 32 lanes of each half (u32 wrap-around):
+// row_shr 1, 2, 4, 8 within the 16-lane rows, then row 0's (row 2's) total
+// broadcast from lane 15 (47) into row 1 (row 3) only.  Lane 31 (63) holds
+// the half's total.
+__device__ __forceinline__ uint32_t half_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
     return x;
 }
 // Sum over the 32 lanes of each half.
@@ -1118,143 +1131,156 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // Blocks are numbered so that the blocks one XCD receives (linear id L,
 // L+8, L+16, ...) walk all probe tiles of the same four columns back to back,
 // so the tile's W3 bytes are reused from that XCD's L2.
-// Probe propagation for tiles of 8 probes, one wave per column, as drive_col
-// does it for one network: work items (branch, spiking word, probe) listed in
-// LDS per half-wave (all of a branch's items, for every probe, go to one
-// half; branches with work are dealt alternately to the halves), so one dword
-// load instruction serves one item of each half and 8 items per half are in
+// Probe propagation for tiles of up to 8 probes, as drive_col does it for one
+// network: work items (branch, spiking word, probe) listed in LDS per
+// half-wave (all of a branch's items, for every probe, go to one half;
+// branches with work are dealt alternately to the halves), so one dword load
+// instruction serves one item of each half and 8 items per half are in
 // flight together.  An item reads the W3 word of (branch, word) for neuron n
 // and the probe's spike word of presynaptic column 8g + w.  Per wave this
 // takes a few round trips instead of one per branch of the tile's union
 // (the probes of a tile share few W3 lines, so that union saves little).
-// Item word: branch | word << 5 | probe << 8 | group << 11 | 1 << 31 on the
-// last item of a (probe, branch) pair.  PT = 1..8 probes per tile.
-template <int PT>
-__global__ __launch_bounds__(256) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles) {
-    static_assert(PT >= 1 && PT <= 8, "three probe bits in an item");
+//
+// The kernel is bound by instruction issue, so an item carries its load
+// offsets ready-made:
+//   bits 0-7   branch * 8 + word     (W3 word index within the column tile)
+//   bits 8-10  probe within the tile
+//   bit  11    last item of its (probe, branch) pair
+//   bits 12-31 probe * ncol + 8g + word  (spike word index; needs PT*ncol <= 2^20)
+// Each half's list is padded to a whole round with PAD items (spike index out
+// of range: a zero word; never last), so a round has no per-item bounds test.
+// A (probe, branch) count is thresholded when its last item is consumed and
+// added to the neuron's accumulator of that probe in LDS (one per probe and
+// lane, shared by both halves).
+//
+// A wave walks `tpw` consecutive tiles of its column: conn and bias are read
+// once, the next tile's group-activity words are requested before this
+// tile's items, and each tile's slot state is requested with its activity
+// words.
+constexpr uint32_t PROBE_PAD_ITEM = 0xFFFFF000u;
+template <int PT, int MODE>
+__global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles, uint32_t tpw) {
+    static_assert(PT == 1 || PT == 2 || PT == 4 || PT == 8, "three probe bits in an item");
+    using Mask = typename std::conditional<(PT > 4), uint64_t, uint32_t>::type;
     constexpr uint32_t PROBE_ITEMS_HALF = 16u * PT * 8u;  // 16 branches x PT probes x 8 words
     constexpr uint32_t DB = 8;  // items per half in flight
     __shared__ __align__(16) uint32_t items[4][2 * PROBE_ITEMS_HALF];
+    __shared__ uint32_t accs[4][PT * 32];
+    const uint32_t nchunks = (ntiles + tpw - 1u) / tpw;
     const uint32_t L = blockIdx.x;
     const uint32_t q = L >> 3;
-    const uint32_t tile = q % ntiles;
-    const uint32_t cg = (q / ntiles) * 8u + (L & 7u);
+    const uint32_t chunk = q % nchunks;
+    const uint32_t cg = (q / nchunks) * 8u + (L & 7u);
     if (cg >= ncolgroups) return;
     const uint32_t lane = lane_id(), n = lane & 31u;
     const bool hi = lane >= 32u;
     const uint32_t col = cg * 4u + (threadIdx.x >> 6);
     if (col >= d.ncol) return;
     const uint32_t base = col * NSLOT;
-    const uint32_t slot0 = tile * PT;
-    const uint32_t gl = d.conn[base + n];
-    uint32_t stt[PT], stT[PT], gaw[PT];
-#pragma unroll
-    for (int p = 0; p < PT; ++p) {  // slots past the end read the last slot's state and are masked off
-        const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
-        stt[p] = d.st_t[sl];
-        stT[p] = d.st_T[sl];
-        gaw[p] = d.gact[(size_t)sl * d.gw + (gl >> 2)];
-    }
-    uint32_t fb[PT];  // lane s: which of branch s's 8 presynaptic columns spiked in probe p
-    uint32_t live = 0, ni = 0;
-#pragma unroll
-    for (int p = 0; p < PT; ++p) {
-        const bool lv = slot0 + (uint32_t)p < d.nslots && stt[p] < stT[p];
-        live |= (lv ? 1u : 0u) << p;
-        fb[p] = lv ? (gaw[p] >> (8u * (gl & 3u))) & 0xFFu : 0u;
-        ni += popc(fb[p]);
-    }
-    if (live == 0u) return;
-    const uint32_t actv = ballot32(ni != 0u);
-    const uint32_t hh = popc(actv & ((1u << n) - 1u)) & 1u;
-    const uint32_t x0 = hh == 0u ? ni : 0u, x1 = hh == 1u ? ni : 0u;
-    uint32_t s0 = x0, s1 = x1;
-#pragma unroll
-    for (uint32_t off = 1; off < 32u; off <<= 1) {
-        const uint32_t t0 = __shfl_up(s0, off, 32), t1 = __shfl_up(s1, off, 32);
-        if (n >= off) { s0 += t0; s1 += t1; }
-    }
-    const uint32_t nA = __builtin_amdgcn_readlane(s0, 31), nB = __builtin_amdgcn_readlane(s1, 31);
-    uint32_t* W = items[threadIdx.x >> 6];
-    if (!hi && ni != 0u) {
-        uint32_t* Lh = W + hh * PROBE_ITEMS_HALF;
-        uint32_t pos = hh ? s1 - x1 : s0 - x0;
-#pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            uint32_t m = fb[p];
-            while (m) {
-                const uint32_t w = __builtin_ctz(m);
-                m &= m - 1u;
-                Lh[pos++] = n | (w << 5) | ((uint32_t)p << 8) | (gl << 11) | (m == 0u ? 0x80000000u : 0u);
-            }
-        }
-    }
-    const uint32_t myN = hi ? nB : nA;
-    const uint32_t* Li = W + (hi ? PROBE_ITEMS_HALF : 0u);  // read after this wave's writes above
+    const uint32_t gid = base + n;
+    const uint32_t gl = d.conn[gid];
+    const uint32_t bi = d.bias[gid] >> d.biasShift;
+    const uint32_t t0 = chunk * tpw, t1 = min(t0 + tpw, ntiles);
     const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);
     const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, NSLOT * TILE_WORDS * 4u);
-    const uint32_t nsl = min((uint32_t)PT, d.nslots - slot0);  // slots of this tile that exist
-    const __amdgpu_buffer_rsrc_t sp = buf_rsrc(d.spk + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
-    uint32_t acc[PT];
+    uint32_t* W = items[threadIdx.x >> 6];
+    uint32_t* A = accs[threadIdx.x >> 6];
+    uint32_t* Li = W + (hi ? PROBE_ITEMS_HALF : 0u);
+    uint32_t gaw[PT];
 #pragma unroll
-    for (int p = 0; p < PT; ++p) acc[p] = 0u;
-    uint32_t c = 0u;
-    const uint32_t R = max(nA, nB);
-    for (uint32_t r = 0; r < R; r += DB) {
-        const uint32_t kn = min(R - r, DB);  // item places used this round (uniform)
-        const uint4 i0 = *reinterpret_cast<const uint4*>(Li + r);
-        const uint4 i1 = *reinterpret_cast<const uint4*>(Li + r + 4u);
-        const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-        uint32_t wv[DB], sv[DB];
+    for (int p = 0; p < PT; ++p)  // slots past the end read the last slot's words and are masked off
+        gaw[p] = d.gact[(size_t)min(t0 * PT + (uint32_t)p, d.nslots - 1u) * d.gw + (gl >> 2)];
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        const uint32_t slot0 = tile * PT;
+        // slot state of the tile's probes, one probe per lane, as bit masks
+        const uint32_t slp = slot0 + n;
+        const bool slv = n < (uint32_t)PT && slp < d.nslots;
+        const uint32_t slc = min(slp, d.nslots - 1u);
+        const uint32_t st_t = d.st_t[slc], st_T = d.st_T[slc], st_b = d.st_bias[slc];
+        uint32_t fs[PT];
 #pragma unroll
-        for (uint32_t k = 0; k < DB; ++k) {
-            wv[k] = 0u; sv[k] = 0u;
-            if (k < kn) {
-                const bool v = r + k < myN;
-                const uint32_t e = it[k];
-                const uint32_t br = e & 31u, w = (e >> 5) & 7u, p = (e >> 8) & 7u, g = (e >> 11) & 0xFFFFFu;
-                wv[k] = buf_ld(w3, v ? ((br * 8u + w) * 32u + n) * 4u : SKIP);
-                sv[k] = buf_ld(sp, v ? (p * d.ncol + 8u * g + w) * 4u : SKIP);
+        for (int p = 0; p < PT; ++p)
+            fs[p] = d.facil != 0u ? d.spk[(size_t)min(slot0 + (uint32_t)p, d.nslots - 1u) * d.ncol + col] : 0u;
+        const uint32_t live = ballot32(slv && st_t < st_T);
+        const uint32_t ubm = ballot32(slv && st_b != 0u);
+        Mask fm = 0;  // bit 8p + w: presynaptic column 8g + w of this lane's branch spiked in probe p
+#pragma unroll
+        for (int p = 0; p < PT; ++p)
+            if ((live >> p) & 1u) fm |= (Mask)((gaw[p] >> (8u * (gl & 3u))) & 0xFFu) << (8 * p);
+        if (tile + 1u < t1) {
+#pragma unroll
+            for (int p = 0; p < PT; ++p)
+                gaw[p] = d.gact[(size_t)min(slot0 + PT + (uint32_t)p, d.nslots - 1u) * d.gw + (gl >> 2)];
+        }
+        if (live == 0u) continue;
+#pragma unroll
+        for (uint32_t i = lane; i < PT * 32u; i += 64u) A[i] = 0u;
+        const uint32_t ni = sizeof(Mask) == 8 ? (uint32_t)__popcll((unsigned long long)fm) : popc((uint32_t)fm);
+        const uint32_t actv = ballot32(ni != 0u);
+        const uint32_t hh = popc(actv & ((1u << n) - 1u)) & 1u;
+        // both halves' item positions in one scan: half 0 in bits 0-15, half 1 in bits 16-31 (each <= 16 * 8 * PT)
+        const uint32_t x01 = ni << (16u * hh);
+        const uint32_t s01 = half_scan(x01);
+        const uint32_t tot = __builtin_amdgcn_readlane(s01, 31);
+        const uint32_t nA = tot & 0xFFFFu, nB = tot >> 16;
+        if (!hi) {
+            uint32_t* Lh = W + hh * PROBE_ITEMS_HALF;
+            uint32_t pos = ((s01 - x01) >> (16u * hh)) & 0xFFFFu;
+            const uint32_t e0 = (n << 3) | ((8u * gl) << 12);  // branch and group parts
+            Mask m = fm;
+            while (m) {
+                const uint32_t bit = sizeof(Mask) == 8 ? (uint32_t)__builtin_ctzll((unsigned long long)m)
+                                                       : (uint32_t)__builtin_ctz((uint32_t)m);
+                m &= m - 1u;
+                const uint32_t pp = bit >> 3, w = bit & 7u;
+                const bool last = ((m >> (bit & ~7u)) & (Mask)0xFFu) == 0u;
+                Lh[pos++] = e0 + w + (pp << 8) + (last ? 0x800u : 0u) + ((pp * d.ncol + w) << 12);
             }
         }
+        const uint32_t myN = hi ? nB : nA;
+        const uint32_t R = (max(nA, nB) + DB - 1u) & ~(DB - 1u);  // rounds of DB items
+        for (uint32_t i = myN + n; i < R; i += 32u) Li[i] = PROBE_PAD_ITEM;
+        const uint32_t nsl = min((uint32_t)PT, d.nslots - slot0);  // slots of this tile that exist
+        const __amdgpu_buffer_rsrc_t sp = buf_rsrc(d.spk + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
+        const uint32_t n4 = n * 4u;
+        uint32_t c = 0u;
+        for (uint32_t r = 0; r < R; r += DB) {
+            const uint4 i0 = *reinterpret_cast<const uint4*>(Li + r);
+            const uint4 i1 = *reinterpret_cast<const uint4*>(Li + r + 4u);
+            const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+            uint32_t wv[DB], sv[DB];
 #pragma unroll
-        for (uint32_t k = 0; k < DB; ++k) {
-            if (k < kn && r + k < myN) {
+            for (uint32_t k = 0; k < DB; ++k) {
+                wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4);
+                sv[k] = buf_ld(sp, (it[k] >> 12) << 2);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < DB; ++k) {
                 const uint32_t e = it[k];
                 c += popc(wv[k] & sv[k]);
-                if (e >> 31) {  // last item of this (probe, branch): its count is complete
-                    const bool nmda = c >= d.dthr;
-                    const uint32_t x = d.mode == 0u ? c : d.mode == 1u ? (nmda ? c + d.boost : c) : (nmda ? 1u : 0u);
-                    const uint32_t p = (e >> 8) & 7u;
-#pragma unroll
-                    for (int qq = 0; qq < PT; ++qq)
-                        if ((uint32_t)qq == p) acc[qq] += x;
+                if (e & 0x800u) {  // last item of this (probe, branch): its count is complete
+                    // mode 0: the count; mode 1: + boost at the threshold; otherwise 1 at the threshold
+                    const uint32_t x = MODE == 0 ? c : MODE == 1 ? (c >= d.dthr ? c + d.boost : c) : (c >= d.dthr ? 1u : 0u);
+                    atomicAdd(&A[((e >> 8) & 7u) * 32u + n], x);
                     c = 0u;
                 }
             }
         }
-    }
+        uint32_t av[PT];
 #pragma unroll
-    for (int p = 0; p < PT; ++p) acc[p] += __shfl_xor(acc[p], 32);  // the two halves' branches
-    const uint32_t gid = base + n;
-    const uint32_t bi = d.bias[gid] >> d.biasShift;
-    uint32_t ub[PT], fs[PT];
+        for (int p = 0; p < PT; ++p) av[p] = A[p * 32 + n];
+        const __amdgpu_buffer_rsrc_t ar = buf_rsrc(d.acc + (size_t)slot0 * d.ncol * NPC, nsl * d.ncol * NPC * 4u);
+        const __amdgpu_buffer_rsrc_t dr = buf_rsrc(d.drive + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
 #pragma unroll
-    for (int p = 0; p < PT; ++p) {
-        const uint32_t sl = min(slot0 + (uint32_t)p, d.nslots - 1u);
-        ub[p] = d.st_bias[sl];
-        fs[p] = d.facil != 0u ? d.spk[(size_t)sl * d.ncol + col] : 0u;
-    }
-#pragma unroll
-    for (int p = 0; p < PT; ++p) {
-        if (((live >> p) & 1u) == 0u) continue;
-        const uint32_t sl = slot0 + (uint32_t)p;
-        uint32_t a = acc[p];
-        if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
-        if (ub[p] != 0u) a = (a > bi) ? (a - bi) : 0u;
-        if (!hi) d.acc[(size_t)sl * d.ncol * NPC + gid] = a;
-        const uint32_t drv = half_sum(a);
-        if (lane == 0u) d.drive[(size_t)sl * d.ncol + col] = drv;
+        for (int p = 0; p < PT; ++p) {
+            if (((live >> p) & 1u) == 0u) continue;
+            uint32_t a = av[p];
+            if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
+            if (((ubm >> p) & 1u) != 0u) a = (a > bi) ? (a - bi) : 0u;
+            buf_st(ar, hi ? SKIP : ((uint32_t)p * d.ncol * NPC + gid) * 4u, a);
+            const uint32_t drv = half_scan(a);
+            buf_st(dr, lane == 31u ? ((uint32_t)p * d.ncol + col) * 4u : SKIP, drv);
+        }
     }
 }
 
@@ -1799,12 +1825,23 @@ static uint32_t occupancy_pad(const void* kernel, uint32_t wgPerCU) {
     return want > a.sharedSizeBytes ? (uint32_t)(want - a.sharedSizeBytes) : 0u;
 }
 
-template <typename K>
-static void probe_launch(K kernel, uint32_t blocks, uint32_t wgPerCU, hipStream_t s, const ProbeDev& d, uint32_t ncg,
-                         uint32_t ntiles) {
+template <typename K, typename... A>
+static void probe_launch(K kernel, uint32_t blocks, uint32_t wgPerCU, hipStream_t s, const ProbeDev& d, A... args) {
     const uint32_t pad = occupancy_pad(reinterpret_cast<const void*>(kernel), wgPerCU);
-    kernel<<<blocks, 256, pad, s>>>(d, ncg, ntiles);
+    kernel<<<blocks, 256, pad, s>>>(d, args...);
 }
+
+template <int PT>
+static void probe_items_launch(uint32_t blocks, uint32_t wgPerCU, hipStream_t s, const ProbeDev& d, uint32_t ncg,
+                               uint32_t ntiles, uint32_t tpw) {
+    if (d.mode == 0u) probe_launch(k_probe_items<PT, 0>, blocks, wgPerCU, s, d, ncg, ntiles, tpw);
+    else if (d.mode == 1u) probe_launch(k_probe_items<PT, 1>, blocks, wgPerCU, s, d, ncg, ntiles, tpw);
+    else probe_launch(k_probe_items<PT, 2>, blocks, wgPerCU, s, d, ncg, ntiles, tpw);
+}
+
+#ifndef AXON_PROBE_TPW
+#define AXON_PROBE_TPW 4  // most probe tiles one k_probe_items wave walks
+#endif
 
 hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, uint32_t wavesPerCU, hipStream_t s) {
     nact = std::min(nact, d.nslots);
@@ -1813,11 +1850,21 @@ hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, ui
     const uint32_t ntiles = (nact + tile - 1u) / tile;
     const uint32_t blocks = ((ncg + 7u) / 8u) * 8u * ntiles;
     const uint32_t wg = wavesPerCU / 4u;  // 4 waves per workgroup
+    // tiles per wave: up to AXON_PROBE_TPW while the launch keeps >= 16384 waves (64 per CU)
+    const uint32_t tpw = std::max<uint32_t>(1u, std::min<uint32_t>(AXON_PROBE_TPW, (uint32_t)((uint64_t)ntiles * d.ncol / 16384u)));
+    const uint32_t iblocks = ((ncg + 7u) / 8u) * 8u * ((ntiles + tpw - 1u) / tpw);
+    if ((tile == 1u || tile == 2u || tile == 4u || tile == 8u) && (uint64_t)tile * d.ncol > (1u << 20)) {
+        // item words hold spike indices below 2^20: wider batches take the per-branch kernel (same results)
+        const uint32_t nt16 = (nact + 15u) / 16u;
+        probe_launch(k_probe_drive<16>, ((ncg + 7u) / 8u) * 8u * nt16, wg, s, d, ncg, nt16);
+        tile = 0u;
+    }
     switch (tile) {
-        case 1: probe_launch(k_probe_items<1>, blocks, wg, s, d, ncg, ntiles); break;
-        case 2: probe_launch(k_probe_items<2>, blocks, wg, s, d, ncg, ntiles); break;
-        case 4: probe_launch(k_probe_items<4>, blocks, wg, s, d, ncg, ntiles); break;
-        case 8: probe_launch(k_probe_items<8>, blocks, wg, s, d, ncg, ntiles); break;
+        case 0: break;
+        case 1: probe_items_launch<1>(iblocks, wg, s, d, ncg, ntiles, tpw); break;
+        case 2: probe_items_launch<2>(iblocks, wg, s, d, ncg, ntiles, tpw); break;
+        case 4: probe_items_launch<4>(iblocks, wg, s, d, ncg, ntiles, tpw); break;
+        case 8: probe_items_launch<8>(iblocks, wg, s, d, ncg, ntiles, tpw); break;
         case 16: probe_launch(k_probe_drive<16>, blocks, wg, s, d, ncg, ntiles); break;
         case 32: probe_launch(k_probe_drive<32>, blocks, wg, s, d, ncg, ntiles); break;
         case 64: probe_launch(k_probe_drive<64>, blocks, wg, s, d, ncg, ntiles); break;
