@@ -308,10 +308,32 @@ __device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t*
 // halves, so a half holds at most 16 branches of at most 8 items.
 constexpr uint32_t ITEMS_PER_HALF = (NSLOT / 2u) * 8u;
 
+// Work items are (branch, spiking presynaptic column) pairs, one 128-byte W3
+// line each, listed in LDS with their load offsets ready-made (the drive
+// kernels are bound by instruction issue and latency, not bytes):
+//   bits 0-7   branch * 8 + word  (W3 word index within the column tile)
+//   bit  8     last item of its branch
+//   bits 9-28  presynaptic column 8g + word (ncol <= 2^20)
+// A half's list is padded to a whole round of DB items with DRIVE_PAD_ITEM
+// (never last; its spike index is past any column count but 2^20, and a pad
+// read inside the range only adds to a count that is never used).
+constexpr uint32_t DRIVE_PAD_ITEM = 0xFFFFFu << 9;
+__device__ __forceinline__ uint32_t drive_item(uint32_t br, uint32_t w, uint32_t g, bool last) {
+    return (br << 3) + w + (last ? 0x100u : 0u) + ((8u * g + w) << 9);
+}
+// The contribution of a branch whose count is c (mm:175-183): mode 0 the
+// count, mode 1 the count plus boost at the NMDA threshold, otherwise 1 at
+// the threshold.
+template <int MODE>
+__device__ __forceinline__ uint32_t branch_out(uint32_t c, bool nmda, uint32_t boost) {
+    return MODE == 0 ? c : MODE == 1 ? (nmda ? c + boost : c) : (nmda ? 1u : 0u);
+}
+
 // Propagation (k_drive, mm:150-190) for one column per wave, the form for a
 // single network, where a timestep is latency-bound: the column's active
 // branches are dealt alternately to the two half-waves, halving the items a
 // half works through.
+template <int MODE>
 __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t col, const DriveIn& in, const uint8_t* fired,
                                           uint32_t* items) {
@@ -338,73 +360,59 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     if (!hi) d.firedW[base + n] = (uint8_t)fwl;
     const uint32_t nActive = popc(mask);
 
-    // Work items are (branch, spiking presynaptic column) pairs, one 128-byte
-    // W3 line each.  The active branches are dealt alternately to the two
-    // half-waves; each half lists all items of its branches, in order, in LDS
-    // (lane s writes branch s's items at a scanned offset), so a single dword
-    // load instruction serves one item of each half and a branch's count c
-    // (summed over its items) completes inside one half.  Item word: branch |
-    // word << 5 | group << 8 | 1 << 31 on the branch's last item.
+    // The active branches are dealt alternately to the two half-waves; each
+    // half lists all items of its branches, in order, in LDS (lane s writes
+    // branch s's items at a scanned offset), so a single dword load
+    // instruction serves one item of each half and a branch's count c
+    // (summed over its items) completes inside one half.
     const uint32_t rank = popc(mask & ((1u << n) - 1u));
     const uint32_t hh = rank & 1u;
     const uint32_t ni = popc(fwl);
-    const uint32_t x0 = hh == 0u ? ni : 0u, x1 = hh == 1u ? ni : 0u;
-    uint32_t s0 = x0, s1 = x1;  // inclusive scans over the 32 branches (each half computes the same)
-#pragma unroll
-    for (uint32_t off = 1; off < 32u; off <<= 1) {
-        const uint32_t t0 = __shfl_up(s0, off, 32), t1 = __shfl_up(s1, off, 32);
-        if (n >= off) { s0 += t0; s1 += t1; }
-    }
-    const uint32_t nA = __builtin_amdgcn_readlane(s0, 31), nB = __builtin_amdgcn_readlane(s1, 31);
+    // both halves' offsets in one scan: half 0 in bits 0-15, half 1 in bits 16-31
+    const uint32_t x01 = ni << (16u * hh);
+    const uint32_t s01 = half_scan(x01);
+    const uint32_t tot = __builtin_amdgcn_readlane(s01, 31);
+    const uint32_t nA = tot & 0xFFFFu, nB = tot >> 16;
     if (!hi && fwl != 0u) {
         uint32_t* L = items + hh * ITEMS_PER_HALF;
-        uint32_t pos = hh ? s1 - x1 : s0 - x0;
+        uint32_t pos = ((s01 - x01) >> (16u * hh)) & 0xFFFFu;
         uint32_t m = fwl;
         while (m) {
             const uint32_t w = __builtin_ctz(m);
             m &= m - 1u;
-            L[pos++] = n | (w << 5) | (gl << 8) | (m == 0u ? 0x80000000u : 0u);
+            L[pos++] = drive_item(n, w, gl, m == 0u);
         }
     }
+    constexpr uint32_t DB = 8;  // items per half fetched per round trip
     const uint32_t nLines = nA + nB;
     const uint32_t myN = hi ? nB : nA;
-    const uint32_t* L = items + (hi ? ITEMS_PER_HALF : 0u);  // read after this wave's writes above
+    uint32_t* L = items + (hi ? ITEMS_PER_HALF : 0u);  // read after this wave's writes above
+    const uint32_t R = (max(nA, nB) + DB - 1u) & ~(DB - 1u);
+    for (uint32_t i = myN + n; i < R; i += 32u) L[i] = DRIVE_PAD_ITEM;
     const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);  // wave-uniform: the resources live in SGPRs
     const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, NSLOT * TILE_WORDS * 4u);
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(spkIn, d.ncol * 4u);
+    const uint32_t n4 = n * 4u;
     uint32_t acc = 0u, bb = 0u, c = 0u;
-    constexpr uint32_t DB = 8;  // items per half fetched per round trip
-    const uint32_t R = max(nA, nB);
     for (uint32_t r = 0; r < R; r += DB) {
-        const uint32_t kn = min(R - r, DB);  // item places used this round (uniform)
         const uint4 i0 = *reinterpret_cast<const uint4*>(L + r);
         const uint4 i1 = *reinterpret_cast<const uint4*>(L + r + 4u);
         const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
         uint32_t wv[DB], sv[DB];
 #pragma unroll
         for (uint32_t k = 0; k < DB; ++k) {
-            wv[k] = 0u; sv[k] = 0u;
-            if (k < kn) {  // no load instructions for unused item places
-                const bool v = r + k < myN;
-                const uint32_t e = it[k];
-                const uint32_t br = e & 31u, w = (e >> 5) & 7u, g = (e >> 8) & 0x7FFFFFu;
-                wv[k] = buf_ld(w3, v ? ((br * 8u + w) * 32u + n) * 4u : SKIP);
-                sv[k] = buf_ld(sr, v ? (8u * g + w) * 4u : SKIP);
-            }
+            wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4);
+            sv[k] = buf_ld(sr, ((it[k] >> 9) & 0xFFFFFu) << 2);
         }
 #pragma unroll
         for (uint32_t k = 0; k < DB; ++k) {
-            if (k < kn && r + k < myN) {
-                const uint32_t e = it[k];
-                c += popc(wv[k] & sv[k]);
-                if (e >> 31) {  // last item of this half's branch: its count is complete
-                    const bool nmda = c >= P.dthr;
-                    if (nmda) bb |= 1u << (e & 31u);
-                    if (P.mode == 0u) acc += c;
-                    else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
-                    else acc += nmda ? 1u : 0u;
-                    c = 0u;
-                }
+            const uint32_t e = it[k];
+            c += popc(wv[k] & sv[k]);
+            if (e & 0x100u) {  // last item of this half's branch: its count is complete
+                const bool nmda = c >= P.dthr;
+                if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
+                acc += branch_out<MODE>(c, nmda, P.boost);
+                c = 0u;
             }
         }
     }
@@ -414,8 +422,8 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     const uint32_t bi = biasw >> P.biasShift;
     acc = (acc > bi) ? (acc - bi) : 0u;
     if (!hi) { d.acc[gid] = acc; d.bbits[gid] = bb; }
-    const uint32_t drv = half_sum(acc);
-    if (lane == 0u) {
+    const uint32_t drv = half_scan(acc);
+    if (lane == 31u) {
         d.drive[col] = drv;
         reinterpret_cast<uint2*>(d.cntProp)[col] = make_uint2(in.cnt.x + nActive, in.cnt.y + nLines);
     }
@@ -426,14 +434,12 @@ constexpr uint32_t ITEMS_PER_COL = NSLOT * 8u;  // LDS item list capacity of a c
 // Propagation (k_drive, mm:150-190) for two columns per wave, the form for
 // many networks at once, where throughput counts: half h of the wave is
 // column colA + h, and lane n of a half is neuron n (and, for the per-branch
-// lookups, branch n); no work is duplicated between the halves.
-//
-// Work items are (branch, spiking presynaptic column) pairs, one 128-byte W3
-// line each.  Each half lists its column's items in branch order in LDS (lane
-// n writes branch n's items at a scanned offset); one dword load instruction
-// then serves one item of each half, and a branch's count c, summed over its
-// items, completes inside the half.  Item word: branch | word << 5 |
-// group << 8 | 1 << 31 on the branch's last item.
+// lookups, branch n); no work is duplicated between the halves.  Each half
+// lists its column's items in branch order in LDS (lane n writes branch n's
+// items at a scanned offset); one dword load instruction then serves one
+// item of each half, and a branch's count c, summed over its items,
+// completes inside the half.
+template <int MODE>
 __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                            uint32_t colA, const DriveIn& in, const uint8_t* fired,
                                            uint32_t* items) {
@@ -461,12 +467,7 @@ __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, con
     const uint32_t nActive = popc(mask);
 
     const uint32_t ni = popc(fwl);
-    uint32_t sc = ni;  // inclusive scan of the item counts over the half's branches
-#pragma unroll
-    for (uint32_t off = 1; off < 32u; off <<= 1) {
-        const uint32_t t = __shfl_up(sc, off, 32);
-        if (n >= off) sc += t;
-    }
+    const uint32_t sc = half_scan(ni);  // inclusive scan of the item counts over the half's branches
     const uint32_t nA = __builtin_amdgcn_readlane(sc, 31), nB = __builtin_amdgcn_readlane(sc, 63);
     uint32_t* L = items + h * ITEMS_PER_COL;  // written here, read back below by the same wave
     if (fwl != 0u) {
@@ -475,47 +476,38 @@ __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, con
         while (m) {
             const uint32_t w = __builtin_ctz(m);
             m &= m - 1u;
-            L[pos++] = n | (w << 5) | (gl << 8) | (m == 0u ? 0x80000000u : 0u);
+            L[pos++] = drive_item(n, w, gl, m == 0u);
         }
     }
+    constexpr uint32_t DB = 8;  // items per half fetched per round trip
     const uint32_t myN = hi ? nB : nA;
+    const uint32_t R = (max(nA, nB) + DB - 1u) & ~(DB - 1u);
+    for (uint32_t i = myN + n; i < R; i += 32u) L[i] = DRIVE_PAD_ITEM;
     // the two columns' 64 tiles of W3 (64 KiB) and the spike words, as range-checked resources
     const uint32_t ubase = __builtin_amdgcn_readfirstlane(colA * NSLOT);
     const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, 2u * NSLOT * TILE_WORDS * 4u);
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(spkIn, d.ncol * 4u);
+    const uint32_t n4h = h * (NSLOT * TILE_WORDS * 4u) + n * 4u;
     uint32_t acc = 0u, bb = 0u, c = 0u;
-    constexpr uint32_t DB = 8;  // items per half fetched per round trip
-    const uint32_t R = max(nA, nB);
     for (uint32_t r = 0; r < R; r += DB) {
-        const uint32_t kn = min(R - r, DB);  // item places used this round (uniform)
         const uint4 i0 = *reinterpret_cast<const uint4*>(L + r);
         const uint4 i1 = *reinterpret_cast<const uint4*>(L + r + 4u);
         const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
         uint32_t wv[DB], sv[DB];
 #pragma unroll
         for (uint32_t k = 0; k < DB; ++k) {
-            wv[k] = 0u; sv[k] = 0u;
-            if (k < kn) {  // no load instructions for unused item places
-                const bool v = r + k < myN;
-                const uint32_t e = it[k];
-                const uint32_t br = e & 31u, w = (e >> 5) & 7u, g = (e >> 8) & 0x7FFFFFu;
-                wv[k] = buf_ld(w3, v ? (((h * NSLOT + br) * 8u + w) * 32u + n) * 4u : SKIP);
-                sv[k] = buf_ld(sr, v ? (8u * g + w) * 4u : SKIP);
-            }
+            wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4h);
+            sv[k] = buf_ld(sr, ((it[k] >> 9) & 0xFFFFFu) << 2);
         }
 #pragma unroll
         for (uint32_t k = 0; k < DB; ++k) {
-            if (k < kn && r + k < myN) {
-                const uint32_t e = it[k];
-                c += popc(wv[k] & sv[k]);
-                if (e >> 31) {  // last item of the branch: its count is complete
-                    const bool nmda = c >= P.dthr;
-                    if (nmda) bb |= 1u << (e & 31u);
-                    if (P.mode == 0u) acc += c;
-                    else if (P.mode == 1u) acc += nmda ? (c + P.boost) : c;
-                    else acc += nmda ? 1u : 0u;
-                    c = 0u;
-                }
+            const uint32_t e = it[k];
+            c += popc(wv[k] & sv[k]);
+            if (e & 0x100u) {  // last item of the branch: its count is complete
+                const bool nmda = c >= P.dthr;
+                if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
+                acc += branch_out<MODE>(c, nmda, P.boost);
+                c = 0u;
             }
         }
     }
@@ -524,8 +516,8 @@ __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, con
     acc = (acc > bi) ? (acc - bi) : 0u;
     d.acc[gid] = acc;
     d.bbits[gid] = bb;
-    const uint32_t drv = half_sum(acc);
-    if (n == 0u) {
+    const uint32_t drv = half_scan(acc);
+    if (n == 31u) {
         d.drive[col] = drv;
         // only this column's half writes its counters: a plain read-modify-write
         reinterpret_cast<uint2*>(d.cntProp)[col] = make_uint2(in.cnt.x + nActive, in.cnt.y + myN);
@@ -545,6 +537,7 @@ __device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* ld
     return reinterpret_cast<const uint8_t*>(lds);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
@@ -554,7 +547,7 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
     const DriveIn in = drive_inputs(d, spkIn, min(col, d.ncol - 1u));
     const uint8_t* f = stage_fired(d, fired);
     if (col >= d.ncol) return;
-    drive_col(d, P, spkIn, col, in, f, items[threadIdx.x >> 6]);
+    drive_col<MODE>(d, P, spkIn, col, in, f, items[threadIdx.x >> 6]);
 }
 
 // ============================================================== column top-K
@@ -1524,6 +1517,7 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
     if (i < d.nstats) d.stats[i] = 0u;
 }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t) {
     const uint32_t net = blockIdx.y;
     const NetDev d = mb_net(m, net);
@@ -1536,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     const DriveIn di = drive_inputs(d, in, min(colA + (lane_id() >> 5), d.ncol - 1u));
     const uint8_t* f = stage_fired(d, fired);
     if (colA >= d.ncol) return;
-    drive_pair(d, P, in, colA, di, f, items[threadIdx.x >> 6]);
+    drive_pair<MODE>(d, P, in, colA, di, f, items[threadIdx.x >> 6]);
 }
 
 // 256 threads per network: this launch is a latency chain of a few barriers,
@@ -1742,7 +1736,10 @@ static hipError_t launch(F k, dim3 g, dim3 b, uint32_t lds, hipStream_t s, hipEv
 
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
-    return launch(k_drive, dim3(blocks_for(d.ncol, 4)), dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
+    const dim3 g(blocks_for(d.ncol, 4));
+    if (P.mode == 0u) return launch(k_drive<0>, g, dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
+    if (P.mode == 1u) return launch(k_drive<1>, g, dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
+    return launch(k_drive<2>, g, dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
 }
 
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s, hipEvent_t e0,
@@ -1895,7 +1892,9 @@ hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats,
 }
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
     dim3 g((ncol + 7u) / 8u, m.nnet);
-    k_mb_drive<<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
+    if (P.mode == 0u) k_mb_drive<0><<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
+    else if (P.mode == 1u) k_mb_drive<1><<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
+    else k_mb_drive<2><<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s) {
