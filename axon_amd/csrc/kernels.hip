@@ -304,9 +304,9 @@ __device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t*
                    reinterpret_cast<const uint2*>(d.cntProp)[col]};
 }
 
-// LDS item list capacity of a half-wave: branches are dealt alternately to the
-// halves, so a half holds at most 16 branches of at most 8 items.
-constexpr uint32_t ITEMS_PER_HALF = (NSLOT / 2u) * 8u;
+// LDS item list capacity of a half-wave: the branches are split where the
+// column's items halve, so a half holds at most (256 + 8) / 2 items.
+constexpr uint32_t ITEMS_PER_HALF = (NSLOT * 8u + 8u) / 2u;
 
 // Work items are (branch, spiking presynaptic column) pairs, one 128-byte W3
 // line each, listed in LDS with their load offsets ready-made (the drive
@@ -327,6 +327,39 @@ __device__ __forceinline__ uint32_t drive_item(uint32_t br, uint32_t w, uint32_t
 template <int MODE>
 __device__ __forceinline__ uint32_t branch_out(uint32_t c, bool nmda, uint32_t boost) {
     return MODE == 0 ? c : MODE == 1 ? (nmda ? c + boost : c) : (nmda ? 1u : 0u);
+}
+// K items of a half's list (K = 8 or 4): their W3 words (at byte offset
+// woff within the resource, plus the item's line) and spike words are
+// requested together; then each branch count is added up and, on the
+// branch's last item, thresholded into acc and the NMDA bits bb.
+template <uint32_t K, int MODE>
+__device__ __forceinline__ void drive_round(const uint32_t* L, __amdgpu_buffer_rsrc_t w3, __amdgpu_buffer_rsrc_t sr,
+                                            uint32_t woff, const Params& P, uint32_t& c, uint32_t& acc,
+                                            uint32_t& bb) {
+    uint32_t it[K];
+    const uint4 i0 = *reinterpret_cast<const uint4*>(L);
+    it[0] = i0.x; it[1] = i0.y; it[2] = i0.z; it[3] = i0.w;
+    if constexpr (K == 8) {
+        const uint4 i1 = *reinterpret_cast<const uint4*>(L + 4u);
+        it[4] = i1.x; it[5] = i1.y; it[6] = i1.z; it[7] = i1.w;
+    }
+    uint32_t wv[K], sv[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + woff);
+        sv[k] = buf_ld(sr, ((it[k] >> 9) & 0xFFFFFu) << 2);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t e = it[k];
+        c += popc(wv[k] & sv[k]);
+        if (e & 0x100u) {  // last item of the branch: its count is complete
+            const bool nmda = c >= P.dthr;
+            if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
+            acc += branch_out<MODE>(c, nmda, P.boost);
+            c = 0u;
+        }
+    }
 }
 
 // Propagation (k_drive, mm:150-190) for one column per wave, the form for a
@@ -365,17 +398,19 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     // branch s's items at a scanned offset), so a single dword load
     // instruction serves one item of each half and a branch's count c
     // (summed over its items) completes inside one half.
-    const uint32_t rank = popc(mask & ((1u << n) - 1u));
-    const uint32_t hh = rank & 1u;
+    // Branches go to the halves in lane order, split where the column's items
+    // halve (both halves compute the same scan).
     const uint32_t ni = popc(fwl);
-    // both halves' offsets in one scan: half 0 in bits 0-15, half 1 in bits 16-31
-    const uint32_t x01 = ni << (16u * hh);
-    const uint32_t s01 = half_scan(x01);
-    const uint32_t tot = __builtin_amdgcn_readlane(s01, 31);
-    const uint32_t nA = tot & 0xFFFFu, nB = tot >> 16;
+    const uint32_t sc = half_scan(ni);
+    const uint32_t total = __builtin_amdgcn_readlane(sc, 31);
+    const uint32_t ex = sc - ni;
+    const uint32_t hh = (2u * ex + ni > total) ? 1u : 0u;  // non-decreasing in lane order
+    const uint32_t b1 = ballot32(hh != 0u);
+    const uint32_t nA = b1 != 0u ? __builtin_amdgcn_readlane(ex, __builtin_ctz(b1)) : total;
+    const uint32_t nB = total - nA;
     if (!hi && fwl != 0u) {
         uint32_t* L = items + hh * ITEMS_PER_HALF;
-        uint32_t pos = ((s01 - x01) >> (16u * hh)) & 0xFFFFu;
+        uint32_t pos = hh ? ex - nA : ex;
         uint32_t m = fwl;
         while (m) {
             const uint32_t w = __builtin_ctz(m);
@@ -387,35 +422,15 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     const uint32_t nLines = nA + nB;
     const uint32_t myN = hi ? nB : nA;
     uint32_t* L = items + (hi ? ITEMS_PER_HALF : 0u);  // read after this wave's writes above
-    const uint32_t R = (max(nA, nB) + DB - 1u) & ~(DB - 1u);
+    const uint32_t R = (max(nA, nB) + 3u) & ~3u;  // rounds of DB items, then one of 4 if needed
     for (uint32_t i = myN + n; i < R; i += 32u) L[i] = DRIVE_PAD_ITEM;
     const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);  // wave-uniform: the resources live in SGPRs
     const __amdgpu_buffer_rsrc_t w3 = buf_rsrc(d.W3 + (size_t)ubase * TILE_WORDS, NSLOT * TILE_WORDS * 4u);
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(spkIn, d.ncol * 4u);
-    const uint32_t n4 = n * 4u;
     uint32_t acc = 0u, bb = 0u, c = 0u;
-    for (uint32_t r = 0; r < R; r += DB) {
-        const uint4 i0 = *reinterpret_cast<const uint4*>(L + r);
-        const uint4 i1 = *reinterpret_cast<const uint4*>(L + r + 4u);
-        const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-        uint32_t wv[DB], sv[DB];
-#pragma unroll
-        for (uint32_t k = 0; k < DB; ++k) {
-            wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4);
-            sv[k] = buf_ld(sr, ((it[k] >> 9) & 0xFFFFFu) << 2);
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < DB; ++k) {
-            const uint32_t e = it[k];
-            c += popc(wv[k] & sv[k]);
-            if (e & 0x100u) {  // last item of this half's branch: its count is complete
-                const bool nmda = c >= P.dthr;
-                if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
-                acc += branch_out<MODE>(c, nmda, P.boost);
-                c = 0u;
-            }
-        }
-    }
+    uint32_t r = 0;
+    for (; r + DB <= R; r += DB) drive_round<DB, MODE>(L + r, w3, sr, n * 4u, P, c, acc, bb);
+    if (r < R) drive_round<4, MODE>(L + r, w3, sr, n * 4u, P, c, acc, bb);
     acc += __shfl_xor(acc, 32);  // the two halves' branches
     bb |= __shfl_xor(bb, 32);
     if (P.facil != 0u && ((prev >> n) & 1u) != 0u) acc += P.facil;
@@ -481,7 +496,7 @@ __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, con
     }
     constexpr uint32_t DB = 8;  // items per half fetched per round trip
     const uint32_t myN = hi ? nB : nA;
-    const uint32_t R = (max(nA, nB) + DB - 1u) & ~(DB - 1u);
+    const uint32_t R = (max(nA, nB) + 3u) & ~3u;  // rounds of DB items, then one of 4 if needed
     for (uint32_t i = myN + n; i < R; i += 32u) L[i] = DRIVE_PAD_ITEM;
     // the two columns' 64 tiles of W3 (64 KiB) and the spike words, as range-checked resources
     const uint32_t ubase = __builtin_amdgcn_readfirstlane(colA * NSLOT);
@@ -489,28 +504,9 @@ __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, con
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(spkIn, d.ncol * 4u);
     const uint32_t n4h = h * (NSLOT * TILE_WORDS * 4u) + n * 4u;
     uint32_t acc = 0u, bb = 0u, c = 0u;
-    for (uint32_t r = 0; r < R; r += DB) {
-        const uint4 i0 = *reinterpret_cast<const uint4*>(L + r);
-        const uint4 i1 = *reinterpret_cast<const uint4*>(L + r + 4u);
-        const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-        uint32_t wv[DB], sv[DB];
-#pragma unroll
-        for (uint32_t k = 0; k < DB; ++k) {
-            wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4h);
-            sv[k] = buf_ld(sr, ((it[k] >> 9) & 0xFFFFFu) << 2);
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < DB; ++k) {
-            const uint32_t e = it[k];
-            c += popc(wv[k] & sv[k]);
-            if (e & 0x100u) {  // last item of the branch: its count is complete
-                const bool nmda = c >= P.dthr;
-                if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
-                acc += branch_out<MODE>(c, nmda, P.boost);
-                c = 0u;
-            }
-        }
-    }
+    uint32_t r = 0;
+    for (; r + DB <= R; r += DB) drive_round<DB, MODE>(L + r, w3, sr, n4h, P, c, acc, bb);
+    if (r < R) drive_round<4, MODE>(L + r, w3, sr, n4h, P, c, acc, bb);
     if (P.facil != 0u && ((prev >> n) & 1u) != 0u) acc += P.facil;
     const uint32_t bi = biasw >> P.biasShift;
     acc = (acc > bi) ? (acc - bi) : 0u;
@@ -1151,11 +1147,43 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // tile's items, and each tile's slot state is requested with its activity
 // words.
 constexpr uint32_t PROBE_PAD_ITEM = 0xFFFFF000u;
+// K items of a half's list (K = 8 or 4): their W3 and spike words are
+// requested together, then each count is added up and, on a pair's last
+// item, thresholded into the neuron's accumulator of that probe (An = A + n).
+template <uint32_t K, int MODE>
+__device__ __forceinline__ void probe_round(const uint32_t* Li, __amdgpu_buffer_rsrc_t w3, __amdgpu_buffer_rsrc_t sp,
+                                            uint32_t n4, uint32_t* An, uint32_t& c, uint32_t dthr, uint32_t boost) {
+    uint32_t it[K];
+    const uint4 i0 = *reinterpret_cast<const uint4*>(Li);
+    it[0] = i0.x; it[1] = i0.y; it[2] = i0.z; it[3] = i0.w;
+    if constexpr (K == 8) {
+        const uint4 i1 = *reinterpret_cast<const uint4*>(Li + 4u);
+        it[4] = i1.x; it[5] = i1.y; it[6] = i1.z; it[7] = i1.w;
+    }
+    uint32_t wv[K], sv[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4);
+        sv[k] = buf_ld(sp, (it[k] >> 12) << 2);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t e = it[k];
+        c += popc(wv[k] & sv[k]);
+        if (e & 0x800u) {  // last item of this (probe, branch): its count is complete
+            // mode 0: the count; mode 1: + boost at the threshold; otherwise 1 at the threshold
+            const uint32_t x = MODE == 0 ? c : MODE == 1 ? (c >= dthr ? c + boost : c) : (c >= dthr ? 1u : 0u);
+            atomicAdd(&An[((e >> 8) & 7u) * 32u], x);
+            c = 0u;
+        }
+    }
+}
 template <int PT, int MODE>
 __global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles, uint32_t tpw) {
     static_assert(PT == 1 || PT == 2 || PT == 4 || PT == 8, "three probe bits in an item");
     using Mask = typename std::conditional<(PT > 4), uint64_t, uint32_t>::type;
-    constexpr uint32_t PROBE_ITEMS_HALF = 16u * PT * 8u;  // 16 branches x PT probes x 8 words
+    // a half's list: at most (total + one branch's items) / 2 of the 32 x 8 x PT items, padded to 4
+    constexpr uint32_t PROBE_ITEMS_HALF = 132u * PT;
     constexpr uint32_t DB = 8;  // items per half in flight
     __shared__ __align__(16) uint32_t items[4][2 * PROBE_ITEMS_HALF];
     __shared__ uint32_t accs[4][PT * 32];
@@ -1209,18 +1237,26 @@ __global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t nco
 #pragma unroll
         for (uint32_t i = lane; i < PT * 32u; i += 64u) A[i] = 0u;
         const uint32_t ni = sizeof(Mask) == 8 ? (uint32_t)__popcll((unsigned long long)fm) : popc((uint32_t)fm);
-        const uint32_t actv = ballot32(ni != 0u);
-        const uint32_t hh = popc(actv & ((1u << n) - 1u)) & 1u;
-        // both halves' item positions in one scan: half 0 in bits 0-15, half 1 in bits 16-31 (each <= 16 * 8 * PT)
-        const uint32_t x01 = ni << (16u * hh);
-        const uint32_t s01 = half_scan(x01);
-        const uint32_t tot = __builtin_amdgcn_readlane(s01, 31);
-        const uint32_t nA = tot & 0xFFFFu, nB = tot >> 16;
-        if (!hi) {
+        // Branches go to the halves in lane order, split where the tile's items halve, so the
+        // halves' lists differ by less than one branch's items (both halves compute the same scan).
+        const uint32_t sc = half_scan(ni);
+        const uint32_t total = __builtin_amdgcn_readlane(sc, 31);
+        const uint32_t ex = sc - ni;
+        const uint32_t hh = (2u * ex + ni > total) ? 1u : 0u;  // non-decreasing in lane order
+        const uint32_t b1 = ballot32(hh != 0u);
+        const uint32_t nA = b1 != 0u ? __builtin_amdgcn_readlane(ex, __builtin_ctz(b1)) : total;
+        const uint32_t nB = total - nA;
+        {
+            // lane n lists branch n's items of the first PH probes, lane n + 32 those of the others
+            constexpr int PH = PT >= 2 ? PT / 2 : PT;
+            const Mask lowm = PT >= 2 ? (Mask)(((Mask)1 << (8 * PH)) - 1u) : (Mask)~(Mask)0;
+            const Mask mine = hi ? (PT >= 2 ? (Mask)(fm & ~lowm) : (Mask)0) : (Mask)(fm & lowm);
             uint32_t* Lh = W + hh * PROBE_ITEMS_HALF;
-            uint32_t pos = ((s01 - x01) >> (16u * hh)) & 0xFFFFu;
+            const uint32_t nlow = sizeof(Mask) == 8 ? (uint32_t)__popcll((unsigned long long)(fm & lowm))
+                                                    : popc((uint32_t)(fm & lowm));
+            uint32_t pos = (hh ? ex - nA : ex) + (hi ? nlow : 0u);
             const uint32_t e0 = (n << 3) | ((8u * gl) << 12);  // branch and group parts
-            Mask m = fm;
+            Mask m = mine;
             while (m) {
                 const uint32_t bit = sizeof(Mask) == 8 ? (uint32_t)__builtin_ctzll((unsigned long long)m)
                                                        : (uint32_t)__builtin_ctz((uint32_t)m);
@@ -1231,34 +1267,15 @@ __global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t nco
             }
         }
         const uint32_t myN = hi ? nB : nA;
-        const uint32_t R = (max(nA, nB) + DB - 1u) & ~(DB - 1u);  // rounds of DB items
+        const uint32_t R = (max(nA, nB) + 3u) & ~3u;  // rounds of DB items, then one of 4 if needed
         for (uint32_t i = myN + n; i < R; i += 32u) Li[i] = PROBE_PAD_ITEM;
         const uint32_t nsl = min((uint32_t)PT, d.nslots - slot0);  // slots of this tile that exist
         const __amdgpu_buffer_rsrc_t sp = buf_rsrc(d.spk + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
         const uint32_t n4 = n * 4u;
         uint32_t c = 0u;
-        for (uint32_t r = 0; r < R; r += DB) {
-            const uint4 i0 = *reinterpret_cast<const uint4*>(Li + r);
-            const uint4 i1 = *reinterpret_cast<const uint4*>(Li + r + 4u);
-            const uint32_t it[DB] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
-            uint32_t wv[DB], sv[DB];
-#pragma unroll
-            for (uint32_t k = 0; k < DB; ++k) {
-                wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4);
-                sv[k] = buf_ld(sp, (it[k] >> 12) << 2);
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < DB; ++k) {
-                const uint32_t e = it[k];
-                c += popc(wv[k] & sv[k]);
-                if (e & 0x800u) {  // last item of this (probe, branch): its count is complete
-                    // mode 0: the count; mode 1: + boost at the threshold; otherwise 1 at the threshold
-                    const uint32_t x = MODE == 0 ? c : MODE == 1 ? (c >= d.dthr ? c + d.boost : c) : (c >= d.dthr ? 1u : 0u);
-                    atomicAdd(&A[((e >> 8) & 7u) * 32u + n], x);
-                    c = 0u;
-                }
-            }
-        }
+        uint32_t r = 0;
+        for (; r + DB <= R; r += DB) probe_round<DB, MODE>(Li + r, w3, sp, n4, A + n, c, d.dthr, d.boost);
+        if (r < R) probe_round<4, MODE>(Li + r, w3, sp, n4, A + n, c, d.dthr, d.boost);
         uint32_t av[PT];
 #pragma unroll
         for (int p = 0; p < PT; ++p) av[p] = A[p * 32 + n];
