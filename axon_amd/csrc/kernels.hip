@@ -1207,17 +1207,22 @@ __global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t nco
     uint32_t* W = items[threadIdx.x >> 6];
     uint32_t* A = accs[threadIdx.x >> 6];
     uint32_t* Li = W + (hi ? PROBE_ITEMS_HALF : 0u);
+    // per-slot state and activity words through range-checked resources (32-bit offsets)
+    const __amdgpu_buffer_rsrc_t ga = buf_rsrc(d.gact, d.nslots * d.gw * 4u);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(d.st_t, d.nslots * 4u), rT = buf_rsrc(d.st_T, d.nslots * 4u);
+    const __amdgpu_buffer_rsrc_t rb = buf_rsrc(d.st_bias, d.nslots * 4u);
+    const uint32_t gaOff = (gl >> 2) * 4u;
     uint32_t gaw[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p)  // slots past the end read the last slot's words and are masked off
-        gaw[p] = d.gact[(size_t)min(t0 * PT + (uint32_t)p, d.nslots - 1u) * d.gw + (gl >> 2)];
+        gaw[p] = buf_ld(ga, min(t0 * PT + (uint32_t)p, d.nslots - 1u) * d.gw * 4u + gaOff);
     for (uint32_t tile = t0; tile < t1; ++tile) {
         const uint32_t slot0 = tile * PT;
         // slot state of the tile's probes, one probe per lane, as bit masks
         const uint32_t slp = slot0 + n;
         const bool slv = n < (uint32_t)PT && slp < d.nslots;
         const uint32_t slc = min(slp, d.nslots - 1u);
-        const uint32_t st_t = d.st_t[slc], st_T = d.st_T[slc], st_b = d.st_bias[slc];
+        const uint32_t st_t = buf_ld(rt, slc * 4u), st_T = buf_ld(rT, slc * 4u), st_b = buf_ld(rb, slc * 4u);
         uint32_t fs[PT];
 #pragma unroll
         for (int p = 0; p < PT; ++p)
@@ -1231,7 +1236,7 @@ __global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t nco
         if (tile + 1u < t1) {
 #pragma unroll
             for (int p = 0; p < PT; ++p)
-                gaw[p] = d.gact[(size_t)min(slot0 + PT + (uint32_t)p, d.nslots - 1u) * d.gw + (gl >> 2)];
+                gaw[p] = buf_ld(ga, min(slot0 + PT + (uint32_t)p, d.nslots - 1u) * d.gw * 4u + gaOff);
         }
         if (live == 0u) continue;
 #pragma unroll
