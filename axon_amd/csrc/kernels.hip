@@ -312,14 +312,15 @@ constexpr uint32_t ITEMS_PER_HALF = (NSLOT * 8u + 8u) / 2u;
 // line each, listed in LDS with their load offsets ready-made (the drive
 // kernels are bound by instruction issue and latency, not bytes):
 //   bits 0-7   branch * 8 + word  (W3 word index within the column tile)
-//   bit  8     last item of its branch
-//   bits 9-28  presynaptic column 8g + word (ncol <= 2^20)
-// A half's list is padded to a whole round of DB items with DRIVE_PAD_ITEM
-// (never last; its spike index is past any column count but 2^20, and a pad
-// read inside the range only adds to a count that is never used).
-constexpr uint32_t DRIVE_PAD_ITEM = 0xFFFFFu << 9;
+//   bit  9     pad: its W3 offset (64 KiB on) is past both drive forms' resources
+//   bit  10    last item of its branch
+//   bits 11-30 presynaptic column 8g + word (ncol <= 2^20)
+// A half's list is padded to a whole round with DRIVE_PAD_ITEM (never last;
+// its W3 word reads as 0 without a request, and so does its spike word for
+// any column count but 2^20).
+constexpr uint32_t DRIVE_PAD_ITEM = 0x200u | (0xFFFFFu << 11);
 __device__ __forceinline__ uint32_t drive_item(uint32_t br, uint32_t w, uint32_t g, bool last) {
-    return (br << 3) + w + (last ? 0x100u : 0u) + ((8u * g + w) << 9);
+    return (br << 3) + w + (last ? 0x400u : 0u) + ((8u * g + w) << 11);
 }
 // The contribution of a branch whose count is c (mm:175-183): mode 0 the
 // count, mode 1 the count plus boost at the NMDA threshold, otherwise 1 at
@@ -346,14 +347,14 @@ __device__ __forceinline__ void drive_round(const uint32_t* L, __amdgpu_buffer_r
     uint32_t wv[K], sv[K];
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
-        wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + woff);
-        sv[k] = buf_ld(sr, ((it[k] >> 9) & 0xFFFFFu) << 2);
+        wv[k] = buf_ld(w3, ((it[k] & 0x3FFu) << 7) + woff);
+        sv[k] = buf_ld(sr, ((it[k] >> 11) & 0xFFFFFu) << 2);
     }
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
         const uint32_t e = it[k];
         c += popc(wv[k] & sv[k]);
-        if (e & 0x100u) {  // last item of the branch: its count is complete
+        if (e & 0x400u) {  // last item of the branch: its count is complete
             const bool nmda = c >= P.dthr;
             if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
             acc += branch_out<MODE>(c, nmda, P.boost);
