@@ -153,3 +153,51 @@ def test_autotuner_sweeps_tiles_and_occupancies():
     assert all(v > 0 for v in table.values()) and table[(best, waves)] == min(table.values())
     with pytest.raises(axon_amd.AxonError):
         axon_amd.Probes(net, slots=8, max_t=32, tile=4, waves_per_cu=24)
+
+
+@pytest.mark.parametrize("mode_args", [["mode=0"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=2"]],
+                         ids=["mode0", "mode1-dthr2", "mode2"])
+def test_tile_walks_and_dendritic_modes(mode_args):
+    """130 probes at ncol=512: tiles 1 and 2 walk several tiles per wave (the last run ragged), tiles 4 and 8
+    one; every mode of the branch output; all equal to the per-branch kernel (tile 16), to single
+    presentations and, for a sample, to the oracle."""
+    args = C0 + mode_args
+    net, ora = trained(args, 1)
+    stims = cues(net, 130)
+    T, clamp = 24, 5
+    res = {}
+    for tile in (16, 1, 2, 4, 8):
+        pb = axon_amd.Probes(net, slots=130, max_t=32, tile=tile)
+        print(f"{mode_args} tile {tile}", flush=True)
+        res[tile] = pb.run(stims, T, clamp, use_bias=1)[:2]
+        pb.close()
+    for tile in (1, 2, 4, 8):
+        assert np.array_equal(res[tile][0], res[16][0]), f"{mode_args} tile {tile}: final spikes"
+        assert np.array_equal(res[tile][1], res[16][1]), f"{mode_args} tile {tile}: histories"
+    for i in range(0, 130, 13):
+        r = net.present(0, 1.0, T, clamp, False, True, 0, 1, stim=stims[i])
+        assert np.array_equal(res[1][1][i], r.hist), f"{mode_args} probe {i}: single presentation"
+        if i % 39 == 0:
+            ora.present(0, 1.0, T, clamp, False, 0, 1, stim=stims[i])
+            assert np.array_equal(res[1][1][i].ravel(), ora.buf("HIST")[: T * 512]), f"{mode_args} probe {i}: oracle"
+
+
+def test_wide_network_takes_the_per_branch_kernel():
+    """Item words hold spike indices below 2^20: at ncol = 131080 a tile of 8 probes runs the per-branch
+    kernel instead, with the same results as tile 1 (items) and as single presentations."""
+    args = ["ncol=131080", "incols=512", "stim=2", "seed=5"] + CONFIG_C
+    net = axon_amd.Net(axon_amd.Config.parse(args))
+    stims = np.stack([axon_amd.probe_cue(net, i % 2, 8 * i, 40 + i) for i in range(8)])
+    T, clamp = 8, 4
+    out = {}
+    for tile in (1, 8):
+        pb = axon_amd.Probes(net, slots=8, max_t=16, tile=tile)
+        print(f"ncol 131080 tile {tile}", flush=True)
+        out[tile] = pb.run(stims, T, clamp, use_bias=1)[:2]
+        pb.close()
+    assert np.array_equal(out[1][0], out[8][0]) and np.array_equal(out[1][1], out[8][1])
+    for i in (0, 5):
+        r = net.present(0, 1.0, T, clamp, False, True, 0, 1, stim=stims[i])
+        assert np.array_equal(out[8][1][i], r.hist), f"probe {i}"
+    assert np.count_nonzero(out[8][1]) > 0
+    net.close()
