@@ -62,3 +62,22 @@ def test_multi_rejects_mismatched_dynamics():
         axon_amd.MultiNet([a, b])
     with pytest.raises(axon_amd.AxonError):
         axon_amd.MultiNet([a, a])
+
+
+@pytest.mark.parametrize("mode_args", [["mode=0", "hetero=1"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=3"]],
+                         ids=["mode0-hetero", "mode1-dthr2", "mode2"])
+def test_multi_dendritic_modes(mode_args):
+    """Every branch-output mode of the two-column propagation: batched training equals training alone."""
+    specs = [(4, 0, 2), (6, 1, 1), (5, 2, 3)]
+    batched = [axon_amd.Net(axon_amd.Config.parse(cfg_args(s, sd) + mode_args)) for s, sd, _ in specs]
+    mb = axon_amd.MultiNet(batched)
+    mb.train([e for _, _, e in specs], extra=True)
+    print(f"multi {mode_args}: trained", flush=True)
+    for (stim, seed, ep), bn in zip(specs, batched):
+        alone = axon_amd.Net(axon_amd.Config.parse(cfg_args(stim, seed) + mode_args))
+        train_alone(alone, ep)
+        for name in ("W0", "W1", "W2", "W3", "BIAS"):
+            assert np.array_equal(bn.read(name), alone.read(name)), f"{mode_args} stim={stim} seed={seed}: {name}"
+        assert bn.work_counters() == alone.work_counters()
+        alone.close()
+    mb.close()
