@@ -1180,7 +1180,7 @@ __device__ __forceinline__ void probe_round(const uint32_t* Li, __amdgpu_buffer_
     }
 }
 template <int PT, int MODE>
-__global__ __launch_bounds__(256, 8) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles, uint32_t tpw) {
+__global__ __launch_bounds__(256, PT > 4 ? 4 : 8) void k_probe_items(ProbeDev d, uint32_t ncolgroups, uint32_t ntiles, uint32_t tpw) {
     static_assert(PT == 1 || PT == 2 || PT == 4 || PT == 8, "three probe bits in an item");
     using Mask = typename std::conditional<(PT > 4), uint64_t, uint32_t>::type;
     // a half's list: at most (total + one branch's items) / 2 of the 32 x 8 x PT items, padded to 4
@@ -1429,62 +1429,99 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
 // stats and drive-log side outputs of k_emit are not produced for probes.
 // `parts` 1024-thread workgroups per probe, each over its own range of the
 // columns (a multiple of 64): the top-K over the probe's drives (every part
-// computes it), then 16 waves over the part's columns, 64 at a time.  The
-// k-of-32 selection of the columns that pass the gate is batched: up to 8 of
-// a wave's gated columns per round, two at a time (half-wave h takes every
-// other one), their acc loads all in flight together, results handed to the
-// column's lane through LDS.  Split parts shorten a lightly loaded step; the
-// last part to finish (a per-slot counter) advances the slot's step, after
-// every part has read it.
+// computes it), then 16 waves over the part's columns, in groups of up to 4
+// chunks of 64 (wave w takes chunks w, w + 16, ...).  A group's drive and
+// clamp words are requested before they are needed (the first group's before
+// the top-K), and the k-of-32 selection of the group's gated columns is
+// batched over all its chunks: up to 8 gated columns per round, two at a time
+// (half-wave h takes every other one), their acc loads all in flight
+// together, results handed to the column's lane through LDS.  Split parts
+// shorten a lightly loaded step; the last part to finish (a per-slot counter)
+// advances the slot's step, after every part has read it.
+constexpr uint32_t SEL_CHUNKS = 4;
+struct SelGroup { uint32_t dv[SEL_CHUNKS], st[SEL_CHUNKS]; };
+__device__ __forceinline__ SelGroup select_inputs(const ProbeDev& d, size_t so, uint32_t g0, uint32_t cEnd,
+                                                  bool clampPhase) {
+    SelGroup in;
+#pragma unroll
+    for (uint32_t j = 0; j < SEL_CHUNKS; ++j) {
+        const uint32_t c = g0 + j * 1024u + lane_id();
+        const bool inb = c < cEnd;
+        in.dv[j] = inb ? d.drive[so + c] : 0u;
+        in.st[j] = (inb && clampPhase && c < d.incols) ? d.stim[so + c] : 0u;
+    }
+    return in;
+}
 __global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d, uint32_t parts) {
-    __shared__ uint32_t bres[16][64];
+    __shared__ uint32_t bres[16][SEL_CHUNKS * 64];
     const uint32_t sl = blockIdx.x / parts, part = blockIdx.x % parts;
     if (sl >= d.nslots) return;
     const uint32_t t = d.st_t[sl];
     if (t >= d.st_T[sl]) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = threadIdx.x >> 6, h = lane >> 5, n = lane & 31u;
     const size_t so = (size_t)sl * d.ncol;
-    const uint32_t th = topk_block<1024>(d.drive + so, part == 0u ? d.theta + 2u * sl : nullptr, d.ncol, d.incols,
-                                         d.target, d.topkpar);
     const bool clampPhase = t < d.st_clamp[sl];
-    const uint32_t* __restrict__ acc = d.acc + so * NPC;
-    uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
     const uint32_t span = ((d.ncol + parts - 1u) / parts + 63u) & ~63u;
     const uint32_t cEnd = min(d.ncol, (part + 1u) * span);
-    for (uint32_t c0 = part * span + wave * 64u; c0 < cEnd; c0 += 1024u) {
-        const uint32_t c = c0 + lane;
-        const bool inb = c < d.ncol;
-        const uint32_t dv = inb ? d.drive[so + c] : 0u;
-        const bool clampCol = inb && clampPhase && c < d.incols;
-        uint32_t b = clampCol ? d.stim[so + c] : 0u;
-        const bool gate = inb && !clampCol && dv >= th;
-        uint64_t m = __ballot(gate);
-        while (m) {  // wave-uniform
-            uint32_t ci[8];
+    const uint32_t gFirst = part * span + wave * 64u;
+    SelGroup in = select_inputs(d, so, gFirst, cEnd, clampPhase);  // in flight during the top-K
+    const uint32_t th = topk_block<1024>(d.drive + so, part == 0u ? d.theta + 2u * sl : nullptr, d.ncol, d.incols,
+                                         d.target, d.topkpar);
+    const uint32_t* __restrict__ acc = d.acc + so * NPC;
+    uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
+    uint32_t* __restrict__ bw = bres[wave];
+    for (uint32_t g0 = gFirst; g0 < cEnd; g0 += SEL_CHUNKS * 1024u) {
+        if (g0 != gFirst) in = select_inputs(d, so, g0, cEnd, clampPhase);
+        uint64_t m[SEL_CHUNKS];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                ci[j] = m ? (uint32_t)__ffsll((unsigned long long)m) - 1u : 64u;
-                if (m) m &= m - 1ull;
+        for (uint32_t j = 0; j < SEL_CHUNKS; ++j) {
+            const uint32_t c = g0 + j * 1024u + lane;
+            const bool clampCol = c < cEnd && clampPhase && c < d.incols;
+            m[j] = __ballot(c < cEnd && !clampCol && in.dv[j] >= th);
+        }
+        uint32_t jj = 0;  // first chunk with gated columns left (wave-uniform)
+        while (true) {
+            while (jj < SEL_CHUNKS && m[jj] == 0ull) ++jj;
+            if (jj == SEL_CHUNKS) break;
+            uint32_t ci[8];  // chunk * 64 + column of up to 8 gated columns, 256 = none
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                while (jj < SEL_CHUNKS && m[jj] == 0ull) ++jj;
+                if (jj < SEL_CHUNKS) {
+                    ci[q] = jj * 64u + (uint32_t)__ffsll((unsigned long long)m[jj]) - 1u;
+                    m[jj] &= m[jj] - 1ull;
+                } else {
+                    ci[q] = 256u;
+                }
             }
             uint32_t v[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t i = h ? ci[2 * j + 1] : ci[2 * j];
-                v[j] = i < 64u ? acc[(size_t)(c0 + i) * NPC + n] : 0u;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = h ? ci[2 * q + 1] : ci[2 * q];
+                v[q] = i < 256u ? acc[(size_t)(g0 + (i >> 6) * 1024u + (i & 63u)) * NPC + n] : 0u;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t i = h ? ci[2 * j + 1] : ci[2 * j];
-                const Kwta kw = kwta32(v[j], d.k, n, h);
-                if (n == 0u && i < 64u) bres[wave][i] = kw.ball;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = h ? ci[2 * q + 1] : ci[2 * q];
+                const Kwta kw = kwta32(v[q], d.k, n, h);
+                if (n == 0u && i < 256u) bw[i] = kw.ball;
             }
         }
-        if (gate) b = bres[wave][lane];  // written above by this wave
-        if (inb) { d.spk[so + c] = b; hist[c] = b; }
-        const uint64_t nz = __ballot(inb && b != 0u);  // one bit per column of this wave's 64
-        uint32_t* ga = d.gact + (size_t)sl * d.gw + (c0 >> 5);
-        if (lane == 0u) ga[0] = (uint32_t)nz;
-        if (lane == 1u && c0 + 32u < d.ncol) ga[1] = (uint32_t)(nz >> 32);
+        uint32_t* ga = d.gact + (size_t)sl * d.gw;
+#pragma unroll
+        for (uint32_t j = 0; j < SEL_CHUNKS; ++j) {
+            const uint32_t c0 = g0 + j * 1024u;
+            if (c0 >= cEnd) break;  // wave-uniform
+            const uint32_t c = c0 + lane;
+            const bool inb = c < cEnd;
+            const bool clampCol = inb && clampPhase && c < d.incols;
+            uint32_t b = in.st[j];
+            if (inb && !clampCol && in.dv[j] >= th) b = bw[j * 64u + lane];  // written above by this wave
+            if (inb) { d.spk[so + c] = b; hist[c] = b; }
+            const uint64_t nz = __ballot(inb && b != 0u);  // one bit per column of these 64
+            if (lane == 0u) ga[c0 >> 5] = (uint32_t)nz;
+            if (lane == 1u && c0 + 32u < d.ncol) ga[(c0 >> 5) + 1u] = (uint32_t)(nz >> 32);
+        }
     }
     __syncthreads();
     if (tid == 0u) {
