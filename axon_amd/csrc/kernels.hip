@@ -52,6 +52,13 @@ __device__ __forceinline__ uint32_t half_scan(uint32_t x) {
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
     return x;
 }
+// Inclusive prefix sum over all 64 lanes: half_scan, then the first half's
+// total (lane 31) broadcast into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
+    x = half_scan(x);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+    return x;
+}
 // Sum over the 32 lanes of each half.
 __device__ __forceinline__ uint32_t half_sum(uint32_t x) {
     x += __shfl_xor(x, 16);
@@ -563,7 +570,7 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
                                uint32_t inCols, uint32_t target, uint32_t par) {
     constexpr uint32_t VPT = 4096u / NT;
     __shared__ uint32_t hist[2][256];
-    __shared__ uint32_t sh_need, sh_prefix, sh_or;
+    __shared__ uint32_t sh_need, sh_prefix, sh_or[NT / 64u];
     const uint32_t tid = threadIdx.x, lane = lane_id();
     uint32_t rv[VPT];
 #pragma unroll
@@ -573,7 +580,7 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
     }
     const uint32_t cRest = inCols + VPT * NT;  // values beyond the register cache
     for (uint32_t i = tid; i < 256u; i += NT) hist[0][i] = 0u;
-    if (tid == 0) { sh_need = target; sh_prefix = 0u; sh_or = 0u; }
+    if (tid == 0) { sh_need = target; sh_prefix = 0u; }
     // A leading pass whose byte is zero for every value puts them all in bin 0
     // and leaves (prefix, need) unchanged under both selection rules (except
     // the serial rule with need == 0, which picks bin 255), so it is skipped.
@@ -582,11 +589,13 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
     for (uint32_t i = 0; i < VPT; ++i) orv |= rv[i];
     for (uint32_t c = cRest + tid; c < ncol; c += NT) orv |= drive[c];
     orv = __reduce_or_sync(~0ull, orv);
+    if (lane == 0u) sh_or[tid >> 6] = orv;
     __syncthreads();
-    if (lane == 0u) atomicOr(&sh_or, orv);
-    __syncthreads();
+    uint32_t allOr = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < NT / 64u; ++w) allOr |= sh_or[w];
     int byStart = 2;
-    if (par || target > 0u) byStart = (sh_or >> 16) ? 2 : (sh_or >> 8) ? 1 : 0;
+    if (par || target > 0u) byStart = (allOr >> 16) ? 2 : (allOr >> 8) ? 1 : 0;
     uint32_t hb = 0;
     for (int by = byStart; by >= 0; --by, hb ^= 1u) {
         const uint32_t need = sh_need, prefix = sh_prefix;
@@ -607,25 +616,24 @@ __device__ uint32_t topk_block(const uint32_t* __restrict__ drive, uint32_t* __r
         if (tid < 64u) {
             const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2], h3 = h[4 * lane + 3];
             const uint32_t loc = h0 + h1 + h2 + h3;
-            uint32_t S = loc;  // inclusive suffix sum over lanes
-#pragma unroll
-            for (uint32_t off = 1; off < 64u; off <<= 1) {
-                const uint32_t y = __shfl_down(S, off);
-                if (lane + off < 64u) S += y;
-            }
+            const uint32_t P = wave_scan(loc);  // inclusive prefix over lanes
+            const uint32_t S = __builtin_amdgcn_readlane(P, 63) - P + loc;  // inclusive suffix sum over lanes
             const uint32_t above4 = S - loc;
             const uint32_t sc3 = h3 + above4, sc2 = h2 + sc3, sc1 = h1 + sc2, sc0 = S;
-            // largest bin i with scan[i] >= need
+            // largest bin i with scan[i] >= need: the suffix sums do not increase with i,
+            // so it lies in the last lane whose first bin qualifies
             int best = -1;
-            if (sc0 >= need) best = 4 * (int)lane;
-            if (sc1 >= need) best = 4 * (int)lane + 1;
-            if (sc2 >= need) best = 4 * (int)lane + 2;
-            if (sc3 >= need) best = 4 * (int)lane + 3;
-            best = __reduce_max_sync(~0ull, best);
+            const uint64_t ge = __ballot(sc0 >= need);
+            if (ge != 0ull) {
+                const uint32_t L = 63u - (uint32_t)__builtin_clzll((unsigned long long)ge);
+                best = (int)(4u * L) + (__builtin_amdgcn_readlane(sc1, L) >= need ? 1 : 0) +
+                       (__builtin_amdgcn_readlane(sc2, L) >= need ? 1 : 0) +
+                       (__builtin_amdgcn_readlane(sc3, L) >= need ? 1 : 0);
+            }
             auto scan_at = [&](uint32_t i) -> uint32_t {  // i < 256, wave-uniform
                 const uint32_t q = i & 3u, src = i >> 2;
                 const uint32_t v = q == 0 ? sc0 : q == 1 ? sc1 : q == 2 ? sc2 : sc3;
-                return __shfl(v, (int)src);
+                return __builtin_amdgcn_readlane(v, src);
             };
             uint32_t b, above;
             if (par) {
@@ -1479,32 +1487,37 @@ __global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d, uint32_t part
             const bool clampCol = c < cEnd && clampPhase && c < d.incols;
             m[j] = __ballot(c < cEnd && !clampCol && in.dv[j] >= th);
         }
-        uint32_t jj = 0;  // first chunk with gated columns left (wave-uniform)
-        while (true) {
-            while (jj < SEL_CHUNKS && m[jj] == 0ull) ++jj;
-            if (jj == SEL_CHUNKS) break;
-            uint32_t ci[8];  // chunk * 64 + column of up to 8 gated columns, 256 = none
+        while ((m[0] | m[1] | m[2] | m[3]) != 0ull) {  // wave-uniform
+            // up to 8 gated columns (chunk * 64 + column, 256 = none), half-wave h taking every other
+            // one; the picks are taken with constant indices only, so masks and picks stay in registers
+            uint32_t ci[4];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                while (jj < SEL_CHUNKS && m[jj] == 0ull) ++jj;
-                if (jj < SEL_CHUNKS) {
-                    ci[q] = jj * 64u + (uint32_t)__ffsll((unsigned long long)m[jj]) - 1u;
-                    m[jj] &= m[jj] - 1ull;
-                } else {
-                    ci[q] = 256u;
+            for (int q = 0; q < 4; ++q) {
+                uint32_t pa = 256u, pb = 256u;
+#pragma unroll
+                for (uint32_t j = 0; j < SEL_CHUNKS; ++j) {
+                    if (pa == 256u && m[j] != 0ull) {
+                        pa = j * 64u + (uint32_t)__ffsll((unsigned long long)m[j]) - 1u;
+                        m[j] &= m[j] - 1ull;
+                    }
                 }
+#pragma unroll
+                for (uint32_t j = 0; j < SEL_CHUNKS; ++j) {
+                    if (pb == 256u && m[j] != 0ull) {
+                        pb = j * 64u + (uint32_t)__ffsll((unsigned long long)m[j]) - 1u;
+                        m[j] &= m[j] - 1ull;
+                    }
+                }
+                ci[q] = h ? pb : pa;
             }
             uint32_t v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t i = h ? ci[2 * q + 1] : ci[2 * q];
-                v[q] = i < 256u ? acc[(size_t)(g0 + (i >> 6) * 1024u + (i & 63u)) * NPC + n] : 0u;
-            }
+            for (int q = 0; q < 4; ++q)
+                v[q] = ci[q] < 256u ? acc[(size_t)(g0 + (ci[q] >> 6) * 1024u + (ci[q] & 63u)) * NPC + n] : 0u;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t i = h ? ci[2 * q + 1] : ci[2 * q];
                 const Kwta kw = kwta32(v[q], d.k, n, h);
-                if (n == 0u && i < 256u) bw[i] = kw.ball;
+                if (n == 0u && ci[q] < 256u) bw[ci[q]] = kw.ball;
             }
         }
         uint32_t* ga = d.gact + (size_t)sl * d.gw;
