@@ -320,14 +320,14 @@ constexpr uint32_t ITEMS_PER_HALF = (NSLOT * 8u + 8u) / 2u;
 // kernels are bound by instruction issue and latency, not bytes):
 //   bits 0-7   branch * 8 + word  (W3 word index within the column tile)
 //   bit  9     pad: its W3 offset (64 KiB on) is past both drive forms' resources
-//   bit  10    last item of its branch
 //   bits 11-30 presynaptic column 8g + word (ncol <= 2^20)
+//   bit  31    last item of its branch (a sign test)
 // A half's list is padded to a whole round with DRIVE_PAD_ITEM (never last;
 // its W3 word reads as 0 without a request, and so does its spike word for
 // any column count but 2^20).
 constexpr uint32_t DRIVE_PAD_ITEM = 0x200u | (0xFFFFFu << 11);
 __device__ __forceinline__ uint32_t drive_item(uint32_t br, uint32_t w, uint32_t g, bool last) {
-    return (br << 3) + w + (last ? 0x400u : 0u) + ((8u * g + w) << 11);
+    return (br << 3) + w + (last ? 0x80000000u : 0u) + ((8u * g + w) << 11);
 }
 // The contribution of a branch whose count is c (mm:175-183): mode 0 the
 // count, mode 1 the count plus boost at the NMDA threshold, otherwise 1 at
@@ -361,7 +361,7 @@ __device__ __forceinline__ void drive_round(const uint32_t* L, __amdgpu_buffer_r
     for (uint32_t k = 0; k < K; ++k) {
         const uint32_t e = it[k];
         c += popc(wv[k] & sv[k]);
-        if (e & 0x400u) {  // last item of the branch: its count is complete
+        if ((int32_t)e < 0) {  // last item of the branch: its count is complete
             const bool nmda = c >= P.dthr;
             if (nmda) bb |= 1u << ((e & 0xFFu) >> 3);
             acc += branch_out<MODE>(c, nmda, P.boost);
@@ -1143,8 +1143,8 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // offsets ready-made:
 //   bits 0-7   branch * 8 + word     (W3 word index within the column tile)
 //   bits 8-10  probe within the tile
-//   bit  11    last item of its (probe, branch) pair
-//   bits 12-31 probe * ncol + 8g + word  (spike word index; needs PT*ncol <= 2^20)
+//   bits 11-30 probe * ncol + 8g + word  (spike word index; needs PT*ncol <= 2^20)
+//   bit  31    last item of its (probe, branch) pair (a sign test)
 // Each half's list is padded to a whole round with PAD items (spike index out
 // of range: a zero word; never last), so a round has no per-item bounds test.
 // A (probe, branch) count is thresholded when its last item is consumed and
@@ -1155,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_kwta_test(const uint32_t* __restrict__ 
 // once, the next tile's group-activity words are requested before this
 // tile's items, and each tile's slot state is requested with its activity
 // words.
-constexpr uint32_t PROBE_PAD_ITEM = 0xFFFFF000u;
+constexpr uint32_t PROBE_PAD_ITEM = 0xFFFFFu << 11;
 // K items of a half's list (K = 8 or 4): their W3 and spike words are
 // requested together, then each count is added up and, on a pair's last
 // item, thresholded into the neuron's accumulator of that probe (An = A + n).
@@ -1173,13 +1173,13 @@ __device__ __forceinline__ void probe_round(const uint32_t* Li, __amdgpu_buffer_
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
         wv[k] = buf_ld(w3, ((it[k] & 0xFFu) << 7) + n4);
-        sv[k] = buf_ld(sp, (it[k] >> 12) << 2);
+        sv[k] = buf_ld(sp, ((it[k] >> 11) & 0xFFFFFu) << 2);
     }
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
         const uint32_t e = it[k];
         c += popc(wv[k] & sv[k]);
-        if (e & 0x800u) {  // last item of this (probe, branch): its count is complete
+        if ((int32_t)e < 0) {  // last item of this (probe, branch): its count is complete
             // mode 0: the count; mode 1: + boost at the threshold; otherwise 1 at the threshold
             const uint32_t x = MODE == 0 ? c : MODE == 1 ? (c >= dthr ? c + boost : c) : (c >= dthr ? 1u : 0u);
             atomicAdd(&An[((e >> 8) & 7u) * 32u], x);
@@ -1269,7 +1269,7 @@ __global__ __launch_bounds__(256, PT > 4 ? 4 : 8) void k_probe_items(ProbeDev d,
             const uint32_t nlow = sizeof(Mask) == 8 ? (uint32_t)__popcll((unsigned long long)(fm & lowm))
                                                     : popc((uint32_t)(fm & lowm));
             uint32_t pos = (hh ? ex - nA : ex) + (hi ? nlow : 0u);
-            const uint32_t e0 = (n << 3) | ((8u * gl) << 12);  // branch and group parts
+            const uint32_t e0 = (n << 3) | ((8u * gl) << 11);  // branch and group parts
             Mask m = mine;
             while (m) {
                 const uint32_t bit = sizeof(Mask) == 8 ? (uint32_t)__builtin_ctzll((unsigned long long)m)
@@ -1277,7 +1277,7 @@ __global__ __launch_bounds__(256, PT > 4 ? 4 : 8) void k_probe_items(ProbeDev d,
                 m &= m - 1u;
                 const uint32_t pp = bit >> 3, w = bit & 7u;
                 const bool last = ((m >> (bit & ~7u)) & (Mask)0xFFu) == 0u;
-                Lh[pos++] = e0 + w + (pp << 8) + (last ? 0x800u : 0u) + ((pp * d.ncol + w) << 12);
+                Lh[pos++] = e0 + w + (pp << 8) + (last ? 0x80000000u : 0u) + ((pp * d.ncol + w) << 11);
             }
         }
         const uint32_t myN = hi ? nB : nA;
