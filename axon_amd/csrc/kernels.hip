@@ -52,6 +52,24 @@ __device__ __forceinline__ uint32_t half_scan(uint32_t x) {
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
     return x;
 }
+// 32x32 bit transpose within each half-wave: lane s of a half returns, as bit
+// n, bit s of lane n's x (the same half).  Five butterfly stages, each
+// exchanging blocks with the lane j apart (ds_swizzle, xor mask j).
+__device__ __forceinline__ uint32_t half_transpose32(uint32_t x) {
+    const uint32_t n = lane_id() & 31u;
+    uint32_t y;
+    y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);
+    x = (n & 16u) ? ((x & 0xFFFF0000u) | (y >> 16)) : ((x & 0x0000FFFFu) | (y << 16));
+    y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x201F);
+    x = (n & 8u) ? ((x & 0xFF00FF00u) | ((y >> 8) & 0x00FF00FFu)) : ((x & 0x00FF00FFu) | ((y & 0x00FF00FFu) << 8));
+    y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x101F);
+    x = (n & 4u) ? ((x & 0xF0F0F0F0u) | ((y >> 4) & 0x0F0F0F0Fu)) : ((x & 0x0F0F0F0Fu) | ((y & 0x0F0F0F0Fu) << 4));
+    y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x081F);
+    x = (n & 2u) ? ((x & 0xCCCCCCCCu) | ((y >> 2) & 0x33333333u)) : ((x & 0x33333333u) | ((y & 0x33333333u) << 2));
+    y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x041F);
+    x = (n & 1u) ? ((x & 0xAAAAAAAAu) | ((y >> 1) & 0x55555555u)) : ((x & 0x55555555u) | ((y & 0x55555555u) << 1));
+    return x;
+}
 // Inclusive prefix sum over all 64 lanes: half_scan, then the first half's
 // total (lane 31) broadcast into rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
@@ -709,12 +727,9 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
         }
         return;
     }
-    uint32_t want = 0u;  // lane s < 32: the neurons whose slot-s branch learns
-#pragma unroll
-    for (uint32_t s = 0; s < 32u; ++s) {
-        const uint32_t m = ballot32(((bbv >> s) & 1u) != 0u);
-        if (lane == s) want = m;
-    }
+    // lane s < 32: the neurons whose slot-s branch learns (the NMDA bits transposed)
+    const uint32_t tr = half_transpose32(bbv);
+    const uint32_t want = lane < 32u ? tr : 0u;
     const uint32_t voted = ballot32(want != 0u);
     const bool full = full_tile_update(P, band);
     const uint32_t lines = wave_sum(want != 0u ? (full ? 8u : popc(fwb)) : 0u);
@@ -876,12 +891,9 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
         const uint32_t cn = d.conn[col * NSLOT + n];
         const uint32_t fwb = d.firedW[col * NSLOT + n];
         const uint32_t band = d.tileBand[col * NSLOT + n];
-        uint32_t want = 0u;  // lane s < 32: the neurons whose slot-s branch learns
-#pragma unroll
-        for (uint32_t s = 0; s < 32u; ++s) {
-            const uint32_t m = ballot32(((bb >> s) & 1u) != 0u);
-            if (lane == s) want = m;
-        }
+        // lane s < 32: the neurons whose slot-s branch learns (the NMDA bits transposed)
+        const uint32_t tr = half_transpose32(bb);
+        const uint32_t want = lane < 32u ? tr : 0u;
         const uint32_t fl = fwb | (full_tile_update(P, band) ? 0x100u : 0u);
         uint32_t voted = ballot32(want != 0u);
         uint32_t nLTP = 0, nLTD = 0;
@@ -1660,12 +1672,8 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
     const uint32_t fwb = d.firedW[col * NSLOT + n];
     const uint32_t band = d.tileBand[col * NSLOT + n];
     const uint2 lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
-    uint32_t want = 0u;  // lane n: the neurons whose branch-n dendrite learns
-#pragma unroll
-    for (uint32_t s = 0; s < 32u; ++s) {
-        const uint32_t m = hball(((bbv >> s) & 1u) != 0u, h);
-        if (n == s) want = m;
-    }
+    // lane n: the neurons whose branch-n dendrite learns (this half's NMDA bits transposed)
+    const uint32_t want = half_transpose32(bbv);
     const uint32_t voted = hball(want != 0u, h);
     const bool full = full_tile_update(P, band);
     uint32_t lines = want != 0u ? (full ? 8u : popc(fwb)) : 0u;
