@@ -162,13 +162,26 @@ __device__ __forceinline__ uint32_t kth_largest(uint32_t v, uint32_t m, int hb, 
     }
     return x;
 }
+// Maximum over the 32 lanes of each half, in every lane of that half.
+__device__ __forceinline__ uint32_t half_max(uint32_t x) {
+    x = max(x, (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F));
+    x = max(x, (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x201F));
+    x = max(x, (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x101F));
+    x = max(x, (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x081F));
+    x = max(x, (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x041F));
+    return x;
+}
 struct Kwta { uint32_t ball, vk, vk1; };
 __device__ __forceinline__ Kwta kwta32(uint32_t v, uint32_t k, uint32_t n, uint32_t h = 0u) {
     const uint32_t ov = half_or(v);
     const int hb = ov ? 31 - __clz(ov) : -1;
     Kwta r;
     r.vk = (k >= 1u && k <= 32u) ? kth_largest(v, k, hb, h) : 0u;
-    r.vk1 = (k <= 31u) ? kth_largest(v, k + 1u, hb, h) : 0u;
+    // the (k+1)-th largest: vk itself when more than k values reach it, else the
+    // largest value below vk (k = 0: the largest value)
+    if (k == 0u) r.vk1 = half_max(v);
+    else if (k <= 31u) r.vk1 = popc(hball(v >= r.vk, h)) > k ? r.vk : half_max(v < r.vk ? v : 0u);
+    else r.vk1 = 0u;
     if (k == 0u) r.ball = 0u;
     else if (k >= 32u) r.ball = 0xFFFFFFFFu;
     else {
