@@ -201,3 +201,34 @@ def test_wide_network_takes_the_per_branch_kernel():
         assert np.array_equal(out[8][1][i], r.hist), f"probe {i}"
     assert np.count_nonzero(out[8][1]) > 0
     net.close()
+
+
+def test_dense_cues_match_across_tiles():
+    """Dense activity (256 clamped input columns, half the recurrent columns winning): long item lists in
+    every tile; tiles 1, 4 and 8 equal the per-branch kernel (tile 16) and single presentations."""
+    args = ["ncol=512", "incols=320", "stim=3", "seed=9"] + CONFIG_C + ["stimcols=256", "target=96"]
+    net = axon_amd.Net(axon_amd.Config.parse(args))
+    stims = np.stack([axon_amd.probe_cue(net, i % 3, 16 * (i % 4), 60 + i) for i in range(24)])
+    T, clamp = 12, 6
+    out = {}
+    for tile in (16, 1, 4, 8):
+        pb = axon_amd.Probes(net, slots=24, max_t=16, tile=tile)
+        print(f"dense cues tile {tile}", flush=True)
+        out[tile] = pb.run(stims, T, clamp, use_bias=1)[:2]
+        pb.close()
+    for tile in (1, 4, 8):
+        assert np.array_equal(out[tile][0], out[16][0]) and np.array_equal(out[tile][1], out[16][1]), tile
+    for i in (0, 7, 13):
+        r = net.present(0, 1.0, T, clamp, False, True, 0, 1, stim=stims[i])
+        assert np.array_equal(out[8][1][i], r.hist), f"probe {i}"
+    # items of the densest (column, 8-probe tile, step): (probe, branch, spiking presynaptic column) triples
+    conn = net.read("CONN").reshape(512, 32)
+    hist = out[8][1]
+    most = 0
+    for t in range(1, T):
+        act = (hist[:, t - 1, :] != 0).reshape(24, 64, 8)
+        items = act.sum(axis=2)
+        for j in range(3):
+            most = max(most, int(items[8 * j:8 * j + 8].sum(axis=0)[conn].sum(axis=1).max()))
+    print(f"densest 8-probe tile: {most} items", flush=True)
+    assert most > 256
