@@ -365,6 +365,18 @@ class Probes:
         return spk, h, ms.value
 
 
+NOALLOC_SCRATCH_BYTES = 32 << 20  # AXON_NOALLOC_SCRATCH_BYTES
+
+
+def probes_run_noalloc(net: Net, count: int, d_stims: int, T: int, clamp_steps: int, use_bias: int, tile: int,
+                       d_hist: int = 0, d_spikes: int = 0, stream: int = 0) -> None:
+    """axon_probes_run_noalloc: device addresses in, nothing allocated; enqueues only."""
+    _check(lib().axon_probes_run_noalloc(
+        net._h, ctypes.c_uint32(count), ctypes.c_void_p(d_stims), ctypes.c_uint32(T), ctypes.c_uint32(clamp_steps),
+        ctypes.c_uint32(use_bias), ctypes.c_uint32(tile), ctypes.c_void_p(d_hist or None),
+        ctypes.c_void_p(d_spikes or None), ctypes.c_void_p(stream or None)))
+
+
 class MultiNet:
     """Independent networks trained in shared launches (per-network work queues)."""
 

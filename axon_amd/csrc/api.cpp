@@ -988,6 +988,56 @@ extern "C" axon_status axon_probes_run(axon_probes* b, uint32_t count, const uin
     return AXON_OK;
 }
 
+// The probe run of axon_probes_run with every piece of slot state carved out
+// of the library's static per-GPU scratch array instead of a probe batch's
+// allocations: it allocates nothing, captures no graph and issues no memset,
+// so a caller may run it under stream capture.  All calls on one GPU share
+// the array; the setup kernel clears what must start at zero.
+extern "C" axon_status axon_probes_run_noalloc(axon_net* n, uint32_t count, const uint32_t* d_stims, uint32_t T,
+                                               uint32_t clamp_steps, uint32_t use_bias, uint32_t tile,
+                                               uint32_t* d_hist, uint32_t* d_spikes, void* stream) {
+    if (!n || !d_stims) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    bool known = false;
+    for (uint32_t t : kProbeTiles) known |= t == tile;
+    if (!known) return fail(AXON_ERR_INVALID_ARG, "probe tile must be 1, 2, 4, 8, 16, 32 or 64 (got %u)", tile);
+    if (count == 0 || T == 0 || T > 4096) return fail(AXON_ERR_INVALID_ARG, "count >= 1 and T in 1..4096");
+    if (use_bias > 1) return fail(AXON_ERR_INVALID_ARG, "use_bias must be 0 or 1 for probes");
+    const size_t NC = n->NCOL, NN = n->N, gw = (n->NCOL + 31) / 32;
+    const size_t ns = ((size_t)count + tile - 1) / tile * tile;
+    auto al = [](size_t w) { return (w + 63) & ~(size_t)63; };  // 256-byte aligned pieces
+    const size_t per[] = {ns * NC, ns * gw, ns * NN, ns * NC, 2 * ns, ns, ns, ns, ns, ns,
+                          d_hist ? 0 : ns * (size_t)T * NC};
+    size_t need = 0;
+    for (size_t w : per) need += al(w);
+    if (need > axon::NOALLOC_SCRATCH_WORDS)
+        return fail(AXON_ERR_CAPACITY,
+                    "allocation-free probe run: %u probes (%zu slots at tile %u) of ncol=%u need %zu bytes of "
+                    "state, the static buffer holds %zu",
+                    count, ns, tile, n->NCOL, need * 4, axon::NOALLOC_SCRATCH_WORDS * 4);
+    HIPCHECK(hipSetDevice(n->device));
+    uint32_t* base = nullptr;
+    HIPCHECK(axon::noalloc_scratch(n->device, &base));
+    uint32_t* cur = base;
+    auto take = [&](size_t w) { uint32_t* p = cur; cur += al(w); return p; };
+    axon::ProbeDev d{};
+    d.W3 = n->d.W3; d.conn = n->d.conn; d.bias = n->d.bias;
+    d.spk = take(per[0]); d.gact = take(per[1]); d.acc = take(per[2]); d.drive = take(per[3]);
+    d.theta = take(per[4]); d.st_t = take(per[5]); d.st_T = take(per[6]); d.st_clamp = take(per[7]);
+    d.st_bias = take(per[8]); d.selDone = take(per[9]);
+    d.hist = d_hist ? d_hist : take(per[10]);
+    d.stim = const_cast<uint32_t*>(d_stims);  // read only: slots below count, during their clamped steps
+    d.nslots = (uint32_t)ns; d.ncol = n->NCOL; d.incols = n->cfg.incols; d.maxt = T; d.gw = (uint32_t)gw;
+    d.k = n->kTest ? n->kTest : n->K;  // readout k (mm:818)
+    d.facil = n->facil;
+    d.dthr = n->P.dthr; d.boost = n->P.boost; d.mode = n->P.mode; d.biasShift = n->P.biasShift;
+    d.target = n->P.targetCols; d.topkpar = n->d.topkpar;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : n->stream;
+    HIPCHECK(axon::launch_probe_setup(d, count, T, clamp_steps, use_bias, s));
+    for (uint32_t t = 0; t < T; ++t) HIPCHECK(axon::launch_probe_step(d, tile, d.nslots, 0u, s));
+    if (d_spikes) HIPCHECK(axon::launch_copy_words(d_spikes, d.spk, (size_t)count * NC, s));
+    return AXON_OK;
+}
+
 // Recall cue with replacement (an extension of loadStim, mm:800-804): start
 // from pattern p's stimulus columns, replace `nreplace` of them by input
 // columns outside the pattern.  Both choices are partial Fisher-Yates draws

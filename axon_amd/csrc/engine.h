@@ -119,6 +119,15 @@ hipError_t launch_multi_end(const MultiDev& m, hipStream_t s);
 // wavesPerCU (0: no limit) caps the propagation's resident waves per CU
 hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, uint32_t wavesPerCU, hipStream_t s);
 hipError_t launch_probe_reset(const ProbeDev& d, uint32_t first, uint32_t count, hipStream_t s);
+// Allocation-free probe runs: the static per-GPU scratch array (words), its
+// device address on `dev` (the current device; looked up once per device),
+// the start-of-run slot setup, and a word copy that is a kernel (no memset or
+// runtime copy node when a caller captures the stream).
+constexpr size_t NOALLOC_SCRATCH_WORDS = (size_t)(32u << 20) / 4u;  // 32 MiB
+hipError_t noalloc_scratch(int dev, uint32_t** out);
+hipError_t launch_probe_setup(const ProbeDev& d, uint32_t count, uint32_t T, uint32_t clampSteps, uint32_t useBias,
+                              hipStream_t s);
+hipError_t launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 hipError_t launch_probe_admit(const ProbeDev& d, const uint32_t* list, uint32_t n, hipStream_t s);
 hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const uint32_t* offs, uint32_t n,
                                uint32_t* out, hipStream_t s);

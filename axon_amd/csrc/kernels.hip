@@ -13,6 +13,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "engine.h"
@@ -2082,6 +2083,74 @@ hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const u
                                uint32_t* out, hipStream_t s) {
     if (!n) return hipSuccess;
     k_probe_gather<<<dim3(n, std::max(d.maxt, 1u)), 256, 0, s>>>(d, slots, offs, n, out);
+    return hipGetLastError();
+}
+}  // namespace axon
+
+namespace axon {
+
+// ============================================================== allocation-free probe runs
+// Probe state for callers that cannot allocate device memory: one static
+// buffer per GPU, NOALLOC_SCRATCH_WORDS words, held until the process ends.
+// Host code never passes `scratch` itself (its host-side name is not a device
+// address); it passes the address noalloc_scratch() obtained from
+// hipGetSymbolAddress on that device.
+namespace {
+__device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
+// Start of an allocation-free run: every slot's spikes and spiking-column
+// bits to zero, its step counter, split-select counter and theta to zero,
+// and its step count (T for slots below `count`, 0 = idle above), clamped
+// steps and bias switch set.  One workgroup per slot.
+__global__ __launch_bounds__(256) void k_probe_setup(ProbeDev d, uint32_t count, uint32_t T, uint32_t clampSteps,
+                                                     uint32_t useBias) {
+    const uint32_t sl = blockIdx.x;
+    if (sl >= d.nslots) return;
+    for (uint32_t i = threadIdx.x; i < d.ncol; i += 256u) d.spk[(size_t)sl * d.ncol + i] = 0u;
+    for (uint32_t i = threadIdx.x; i < d.gw; i += 256u) d.gact[(size_t)sl * d.gw + i] = 0u;
+    if (threadIdx.x == 0u) {
+        d.st_t[sl] = 0u;
+        d.st_T[sl] = sl < count ? T : 0u;
+        d.st_clamp[sl] = clampSteps;
+        d.st_bias[sl] = useBias;
+        d.selDone[sl] = 0u;
+        d.theta[2u * sl] = 0u;
+        d.theta[2u * sl + 1u] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                                    size_t n) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t noalloc_scratch(int dev, uint32_t** out) {
+    constexpr int MAXDEV = 64;
+    static std::mutex mu;
+    static uint32_t* addr[MAXDEV] = {};
+    if (dev < 0 || dev >= MAXDEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        // the caller has made `dev` the current device: this is its copy of the array
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_probe_setup(const ProbeDev& d, uint32_t count, uint32_t T, uint32_t clampSteps, uint32_t useBias,
+                              hipStream_t s) {
+    k_probe_setup<<<d.nslots, 256, 0, s>>>(d, count, T, clampSteps, useBias);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_copy_words<<<(uint32_t)std::min<size_t>((n + 255) / 256, 4096), 256, 0, s>>>(dst, src, n);
     return hipGetLastError();
 }
 }  // namespace axon

@@ -193,6 +193,26 @@ axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t count, uin
  * hist is count*T*NCOL (packed), ms the device wall time of the steps */
 axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
                             uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes, double* ms);
+/* Allocation-free probe run.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same results as
+ * axon_probes_run on a batch of `tile` probes per propagation wave: `count`
+ * readout presentations of T steps against n.  Every piece of probe state
+ * lives in one static device buffer of AXON_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends); a run whose state does not fit returns
+ * AXON_ERR_CAPACITY (per probe slot: 3*NCOL + N + NCOL/32 words, plus T*NCOL
+ * when d_hist is NULL; slots are count rounded up to the tile).
+ * Device pointers: d_stims count*NCOL clamp patterns (input); d_hist
+ * count*T*NCOL words (each probe's T rows, packed) or NULL; d_spikes
+ * count*NCOL final ballots or NULL.  stream: a hipStream_t, NULL = the
+ * network's stream.  Asynchronous: enqueues kernels only (no allocation, no
+ * memset, no host synchronisation), so it may be captured.  All calls on one
+ * GPU share the buffer, which keeps the last call's contents: calls that
+ * overlap on one GPU (different streams) race; serialise them. */
+#define AXON_NOALLOC_SCRATCH_BYTES (32u << 20)
+axon_status axon_probes_run_noalloc(axon_net* n, uint32_t count, const uint32_t* d_stims, uint32_t T,
+                                    uint32_t clamp_steps, uint32_t use_bias, uint32_t tile, uint32_t* d_hist,
+                                    uint32_t* d_spikes, void* stream);
 /* waves per CU the batch's propagation may keep resident: 0 (as many as
  * fit), 32 or 16; fewer leaves each wave more of the CU's caches */
 axon_status axon_probes_set_occupancy(axon_probes* b, uint32_t waves_per_cu);
