@@ -1945,8 +1945,9 @@ hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, ui
     // tiles per wave: up to AXON_PROBE_TPW while the launch keeps >= 16384 waves (64 per CU)
     const uint32_t tpw = std::max<uint32_t>(1u, std::min<uint32_t>(AXON_PROBE_TPW, (uint32_t)((uint64_t)ntiles * d.ncol / 16384u)));
     const uint32_t iblocks = ((ncg + 7u) / 8u) * 8u * ((ntiles + tpw - 1u) / tpw);
-    if ((tile == 1u || tile == 2u || tile == 4u || tile == 8u) && (uint64_t)tile * d.ncol > (1u << 20)) {
-        // item words hold spike indices below 2^20: wider batches take the per-branch kernel (same results)
+    if ((tile == 1u || tile == 2u || tile == 4u || tile == 8u) && (uint64_t)tile * d.ncol >= (1u << 20)) {
+        // item words hold spike indices below 2^20, and the pad item's index 2^20 - 1 must lie past a
+        // tile's spike words: wider batches take the per-branch kernel (same results)
         const uint32_t nt16 = (nact + 15u) / 16u;
         probe_launch(k_probe_drive<16>, ((ncg + 7u) / 8u) * 8u * nt16, wg, s, d, ncg, nt16);
         tile = 0u;

@@ -43,7 +43,8 @@ def cap10_fields(out):
         if len(p) > 4 and p[1] == "FREE":
             free.append(p[4])
     sustained = f"{float(free[-1]) / float(free[0]):.3f}"
-    k = [k for k, l in enumerate(lines) if "free-phase drive" in l][0]
+    # grep -A1 'free-phase drive' | tail -1: the line after the last match
+    k = [k for k, l in enumerate(lines) if "free-phase drive" in l][-1]
     d = [t.split(":")[1] for t in lines[k + 1].split() if t.startswith("t6:")][0]
     return w, s, o, m, sustained, d
 
