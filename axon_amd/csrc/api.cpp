@@ -709,7 +709,7 @@ extern "C" axon_status axon_wdiag(axon_net* n, const uint32_t* memb, const uint3
     hipError_t e = hipMemcpyAsync(n->d_mask, memb, (size_t)n->N * 4, hipMemcpyHostToDevice, n->stream);
     if (e == hipSuccess && nstim) e = hipMemcpyAsync(d_asm, asm_spk, (size_t)nstim * n->NCOL * 4, hipMemcpyHostToDevice, n->stream);
     if (e == hipSuccess) e = hipMemsetAsync(n->d_small, 0, 48 * 4, n->stream);
-    if (e == hipSuccess) e = axon::launch_wdiag(n->d, n->d_mask, d_asm, nstim, sel, n->d_small, n->stream);
+    if (e == hipSuccess) e = axon::launch_wdiag(n->d, n->d_mask, d_asm, nstim, sel, n->d_ball, n->d_small, n->stream);
     uint32_t tmp[48];
     if (e == hipSuccess) e = hipMemcpyAsync(tmp, n->d_small, 48 * 4, hipMemcpyDeviceToHost, n->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(n->stream);

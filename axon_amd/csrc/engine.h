@@ -146,8 +146,9 @@ hipError_t launch_within(const NetDev& d, const uint32_t* ball, const uint32_t* 
                          uint32_t* out, hipStream_t s);
 hipError_t launch_indeg(const NetDev& d, uint32_t* out, hipStream_t s);
 hipError_t launch_untouched(const NetDev& d, uint32_t* out, hipStream_t s);
+// anyAsm: ncol words of scratch (the OR of the assemblies' ballots per column)
 hipError_t launch_wdiag(const NetDev& d, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
-                        uint32_t sel, uint32_t* hist, hipStream_t s);
+                        uint32_t sel, uint32_t* anyAsm, uint32_t* hist, hipStream_t s);
 hipError_t launch_fill(uint32_t* p, uint32_t value, size_t nwords, hipStream_t s);
 // memory probes: kind 0 streaming read, 1 128-byte line gathers, 2 1 KiB tile gathers
 hipError_t launch_mem_probe(uint32_t kind, const uint32_t* p, size_t bytes, uint32_t blocks, uint32_t iters,

@@ -108,13 +108,6 @@ __device__ __forceinline__ uint32_t* list_count(const NetDev& d, uint32_t par, u
 __device__ __forceinline__ size_t word_idx(uint32_t tile, uint32_t w, uint32_t n) {
     return ((size_t)tile * 8u + w) * 32u + n;
 }
-// Words 4h..4h+3 of neuron n (the presynaptic columns 8g+4h..8g+4h+3): one
-// lane's share of a tile when lanes 32h..32h+31 cover half h.
-__device__ __forceinline__ uint4 lane_words(const uint32_t* __restrict__ plane, uint32_t tile, uint32_t h,
-                                            uint32_t n) {
-    const uint32_t* p = plane + word_idx(tile, 4u * h, n);
-    return make_uint4(p[0], p[32], p[64], p[96]);
-}
 // Range-checked buffer access: a lane whose byte offset is at or past `bytes`
 // reads 0 and writes nothing, and issues no memory request.  Used to load or
 // store only the words a lane needs without branching around the access.
@@ -1002,117 +995,217 @@ __global__ __launch_bounds__(256) void k_wstat(const uint4* __restrict__ W3, siz
     if (lane_id() == 0u) atomicAdd(out, s);
 }
 
-// k_within, mm:524-562.  One thread per postsynaptic neuron (lane = neuron of a
-// 32-lane half); the ten counters are reduced per wave before the atomics.
+// ---- readout reductions over the word-major planes (one wave per column) ----
+// A (column, branch) tile of a plane is 256 words = 1 KiB, and lane L reads
+// words 4L..4L+3 of it with one dwordx4: presynaptic word w = L / 8 (column
+// 8g + w of the branch's group g) of neurons 4 (L % 8) .. 4 (L % 8) + 3.  A
+// column's tiles are contiguous, so a wave streams them as 1 KiB coalesced
+// requests.  Column totals are summed over the wave, per-neuron totals over
+// the eight lanes that hold the same neurons; every sum is u32 wrap-around,
+// so the order of additions does not change a result.
+__device__ __forceinline__ uint4 tile_quad(const uint32_t* __restrict__ plane, uint32_t tile) {
+    return reinterpret_cast<const uint4*>(plane + (size_t)tile * TILE_WORDS)[lane_id()];
+}
+// Sum over the eight lanes with the same lane % 8 (lane bits 3-5).
+__device__ __forceinline__ uint32_t octet_sum(uint32_t x) {
+    x += __shfl_xor(x, 8);
+    x += __shfl_xor(x, 16);
+    x += __shfl_xor(x, 32);
+    return x;
+}
+// Lane i (< NV) adds counter i of the wave's totals to out[i]: one atomic
+// instruction per wave.
+template <int NV>
+__device__ __forceinline__ void wave_add_counters(const uint32_t (&v)[NV], uint32_t* __restrict__ out) {
+    const uint32_t lane = lane_id();
+    uint32_t mine = 0u;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const uint32_t t = wave_sum(v[i]);
+        if (lane == (uint32_t)i) mine = t;
+    }
+    if (lane < (uint32_t)NV && mine != 0u) atomicAdd(&out[lane], mine);
+}
+
+// Per-neuron counts over the recurrent branches (s >= inbr), for every
+// neuron: KIND 0 functional synapses, the popcount of W3 (k_indeg,
+// mm:566-581); KIND 1 synapses still at an initial counter value 5 or 6,
+// i.e. W3 = 0, W2 = 1 and W1 != W0 (k_untouched, mm:501-519).  Four tiles'
+// loads are in flight per wave.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_neuron_counts(NetDev d, uint32_t* __restrict__ out) {
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (col >= d.ncol) return;  // wave-uniform
+    constexpr uint32_t U = 4;
+    constexpr int NP = KIND == 0 ? 1 : 4;
+    uint32_t c[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t s0 = d.inbr; s0 < NSLOT; s0 += U) {
+        uint4 a[U][NP];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t tile = col * NSLOT + min(s0 + u, NSLOT - 1u);  // past the end: re-read, not counted
+            a[u][0] = tile_quad(d.W3, tile);
+            if (KIND == 1) {
+                a[u][1] = tile_quad(d.W2, tile);
+                a[u][2] = tile_quad(d.W1, tile);
+                a[u][3] = tile_quad(d.W0, tile);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (s0 + u >= NSLOT) break;
+            const uint32_t x3[4] = {a[u][0].x, a[u][0].y, a[u][0].z, a[u][0].w};
+            if (KIND == 0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[j] += popc(x3[j]);
+            } else {
+                const uint32_t x2[4] = {a[u][1].x, a[u][1].y, a[u][1].z, a[u][1].w};
+                const uint32_t x1[4] = {a[u][2].x, a[u][2].y, a[u][2].z, a[u][2].w};
+                const uint32_t x0[4] = {a[u][3].x, a[u][3].y, a[u][3].z, a[u][3].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[j] += popc(~x3[j] & x2[j] & (x1[j] ^ x0[j]));
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = octet_sum(c[j]);
+    const uint32_t lane = lane_id();
+    if (lane < 8u) reinterpret_cast<uint4*>(out + (size_t)col * NPC)[lane] = make_uint4(c[0], c[1], c[2], c[3]);
+}
+
+// Functional recurrent density from one assembly onto selected postsynaptic
+// neurons (k_within, mm:524-562).  Selected: neurons of recurrent columns
+// whose postMask word is nonzero (useMask) or whose bit is set in `ball`.
+// For each selected neuron and recurrent branch, the branch's 256 presynaptic
+// bits are split by the ballot of their column in `ball`: in (set) and out.
+// out[0..3] = functional in, all in, functional out, all out; out[4], out[5] =
+// functional / all in for selected neurons shared with another assembly
+// (mult > 1), out[6], out[7] the same for the others; out[8], out[9] = the
+// numbers of shared / other selected neurons.  A column with no selected
+// neuron costs one ballot.
 __global__ __launch_bounds__(256) void k_within(NetDev d, const uint32_t* __restrict__ ball,
                                                 const uint32_t* __restrict__ postMask, uint32_t useMask,
-                                                uint32_t* out) {
-    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t col = gid >> 5, n = gid & 31u;
-    bool act = gid < d.ncol * NPC && col >= d.incols;
-    if (act) act = useMask ? (postMask[gid] != 0u) : (((ball[col] >> n) & 1u) != 0u);
-    uint32_t fIn = 0, pIn = 0, fOut = 0, pOut = 0;
-    bool shared = false;
-    if (act) {
-        const uint32_t base = col * NSLOT;
-        for (uint32_t s = d.inbr; s < NSLOT; ++s) {
-            const uint32_t g = d.conn[base + s];
-            for (uint32_t j = 0; j < 2u; ++j) {
-                const uint4 ap = reinterpret_cast<const uint4*>(ball)[2u * g + j];
-                const uint4 w = lane_words(d.W3, base + s, j, n);
-                fIn += popc(w.x & ap.x) + popc(w.y & ap.y) + popc(w.z & ap.z) + popc(w.w & ap.w);
-                pIn += popc(ap.x) + popc(ap.y) + popc(ap.z) + popc(ap.w);
-                fOut += popc(w.x & ~ap.x) + popc(w.y & ~ap.y) + popc(w.z & ~ap.z) + popc(w.w & ~ap.w);
-                pOut += popc(~ap.x) + popc(~ap.y) + popc(~ap.z) + popc(~ap.w);
-            }
-        }
-        shared = d.mult[gid] > 1u;
-    }
-    const uint32_t v[10] = {fIn, pIn, fOut, pOut, shared ? fIn : 0u, shared ? pIn : 0u, shared ? 0u : fIn,
-                            shared ? 0u : pIn, (act && shared) ? 1u : 0u, (act && !shared) ? 1u : 0u};
+                                                uint32_t* __restrict__ out) {
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (col >= d.ncol || col < d.incols) return;  // wave-uniform
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const uint32_t gid = col * NPC + n;
+    const uint32_t sel = useMask ? ballot32(postMask[gid] != 0u) : ball[col];
+    if (sel == 0u) return;
+    const uint32_t sh = ballot32(d.mult[gid] > 1u);
+    const uint32_t cn = d.conn[col * NSLOT + n];
+    const uint32_t q0 = (lane & 7u) * 4u, w = lane >> 3;
+    const uint32_t selq = (sel >> q0) & 0xFu, shq = (sh >> q0) & 0xFu;
+    uint32_t fS = 0, pS = 0, fE = 0, pE = 0, fO = 0, pO = 0;
+    constexpr uint32_t U = 4;  // tiles in flight
+    for (uint32_t s0 = d.inbr; s0 < NSLOT; s0 += U) {
+        uint32_t ap[U];
+        uint4 x[U];
 #pragma unroll
-    for (int i = 0; i < 10; ++i) {
-        const uint32_t s = wave_sum(v[i]);
-        if (lane_id() == 0u && s) atomicAdd(&out[i], s);
-    }
-}
-
-// k_indeg, mm:566-581
-__global__ __launch_bounds__(256) void k_indeg(NetDev d, uint32_t* __restrict__ out) {
-    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
-    if (gid >= d.ncol * NPC) return;
-    const uint32_t col = gid >> 5, n = gid & 31u;
-    uint32_t dd = 0;
-    for (uint32_t s = d.inbr; s < NSLOT; ++s)
-        for (uint32_t j = 0; j < 2u; ++j) {
-            const uint4 w = lane_words(d.W3, col * NSLOT + s, j, n);
-            dd += popc(w.x) + popc(w.y) + popc(w.z) + popc(w.w);
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t s = min(s0 + u, NSLOT - 1u);  // past the end: re-read, not counted
+            ap[u] = ball[8u * __builtin_amdgcn_readlane(cn, s) + w];
+            x[u] = tile_quad(d.W3, col * NSLOT + s);
         }
-    out[gid] = dd;
-}
-
-// k_untouched, mm:501-519: synapses still at the initial values 5 or 6
-__global__ __launch_bounds__(256) void k_untouched(NetDev d, uint32_t* __restrict__ out) {
-    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
-    if (gid >= d.ncol * NPC) return;
-    const uint32_t col = gid >> 5, n = gid & 31u;
-    uint32_t u = 0;
-    for (uint32_t s = d.inbr; s < NSLOT; ++s)
-        for (uint32_t j = 0; j < 2u; ++j) {
-            const uint32_t tl = col * NSLOT + s;
-            const uint4 a0 = lane_words(d.W0, tl, j, n), a1 = lane_words(d.W1, tl, j, n),
-                        a2 = lane_words(d.W2, tl, j, n), a3 = lane_words(d.W3, tl, j, n);
-            auto f = [](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) {
-                return popc(((~x3) & x2 & (~x1) & x0) | ((~x3) & x2 & x1 & (~x0)));
-            };
-            u += f(a0.x, a1.x, a2.x, a3.x) + f(a0.y, a1.y, a2.y, a3.y) + f(a0.z, a1.z, a2.z, a3.z) +
-                 f(a0.w, a1.w, a2.w, a3.w);
-        }
-    out[gid] = u;
-}
-
-// k_wdiag, mm:591-638: 4-bit counter histogram split by pre/post assembly relation
-__global__ __launch_bounds__(256) void k_wdiag(NetDev d, const uint32_t* __restrict__ memb,
-                                               const uint32_t* __restrict__ asmSpk, uint32_t nstim, uint32_t sel,
-                                               uint32_t* __restrict__ hist) {
-    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
-    if (gid >= d.ncol * NPC) return;
-    const uint32_t mypost = memb[gid];
-    if (mypost == 0u) return;
-    const uint32_t col = gid >> 5, n = gid & 31u, base = col * NSLOT;
-    uint32_t acc[48];
-    for (int i = 0; i < 48; ++i) acc[i] = 0u;
-    for (uint32_t s = 0; s < NSLOT; ++s) {
-        const bool isAff = s < d.inbr;
-        if (sel == 0u && isAff) continue;
-        if (sel == 1u && !isAff) continue;
-        const uint32_t g = d.conn[base + s];
-        for (uint32_t j = 0; j < 2u; ++j) {
-            const uint32_t tl = base + s;
-            const uint4 A0 = lane_words(d.W0, tl, j, n), A1 = lane_words(d.W1, tl, j, n),
-                        A2 = lane_words(d.W2, tl, j, n), A3 = lane_words(d.W3, tl, j, n);
-            const uint32_t a0[4] = {A0.x, A0.y, A0.z, A0.w}, a1[4] = {A1.x, A1.y, A1.z, A1.w},
-                           a2[4] = {A2.x, A2.y, A2.z, A2.w}, a3[4] = {A3.x, A3.y, A3.z, A3.w};
-            for (int q = 0; q < 4; ++q) {
-                uint32_t within = 0, anyp = 0;
-                for (uint32_t st = 0; st < nstim; ++st) {
-                    const uint32_t ap = asmSpk[(size_t)st * d.ncol + 8u * g + 4u * j + q];
-                    anyp |= ap;
-                    if (((mypost >> st) & 1u) != 0u) within |= ap;
-                }
-                const uint32_t between = anyp & ~within;
-                for (uint32_t v = 0; v < 16u; ++v) {
-                    const uint32_t eq = ((v & 1u) ? a0[q] : ~a0[q]) & ((v & 2u) ? a1[q] : ~a1[q]) &
-                                        ((v & 4u) ? a2[q] : ~a2[q]) & ((v & 8u) ? a3[q] : ~a3[q]);
-                    acc[v] += popc(eq & within);
-                    acc[16 + v] += popc(eq & between);
-                    acc[32 + v] += popc(eq);
-                }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (s0 + u >= NSLOT) break;
+            const uint32_t xv[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+            const uint32_t pa = popc(ap[u]);
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j) {
+                if (((selq >> j) & 1u) == 0u) continue;
+                const uint32_t fi = popc(xv[j] & ap[u]);
+                if ((shq >> j) & 1u) { fS += fi; pS += pa; } else { fE += fi; pE += pa; }
+                fO += popc(xv[j] & ~ap[u]);
+                pO += 32u - pa;
             }
         }
     }
-    for (int i = 0; i < 48; ++i)
-        if (acc[i]) atomicAdd(&hist[i], acc[i]);
+    // the ballots are wave-uniform: their counts enter the wave sum from lane 0 only
+    const uint32_t nS = lane == 0u ? popc(sel & sh) : 0u, nE = lane == 0u ? popc(sel & ~sh) : 0u;
+    const uint32_t v[10] = {fS + fE, pS + pE, fO, pO, fS, pS, fE, pE, nS, nE};
+    wave_add_counters<10>(v, out);
+}
+
+// OR of the assemblies' ballots of each column (the presynaptic columns that
+// are members of some assembly), for k_wdiag.
+__global__ __launch_bounds__(256) void k_asm_union(const uint32_t* __restrict__ asmSpk, uint32_t nstim, uint32_t ncol,
+                                                   uint32_t* __restrict__ any) {
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= ncol) return;
+    uint32_t o = 0u;
+    for (uint32_t q = 0; q < nstim; ++q) o |= asmSpk[(size_t)q * ncol + c];
+    any[c] = o;
+}
+
+// Counter-value histogram split by the assembly relation of the synapse's two
+// neurons (k_wdiag, mm:591-638), over postsynaptic neurons that belong to an
+// assembly (memb bit q: member of assembly q < nstim) and over the branches
+// `sel` names: 0 recurrent, 1 afferent, otherwise all.  A synapse from
+// presynaptic neuron x is "within" when x is in the ballot of an assembly the
+// neuron belongs to, "between" when x is in some assembly's ballot but none
+// of those.  hist[v], hist[16 + v], hist[32 + v]: synapses with counter value
+// v that are within, between, and all.  The sixteen value masks of a word are
+// built as a decision tree over the four planes (W3 first); a neuron's
+// "within" word ORs only the ballots of its own assemblies (few per neuron).
+// `any` is k_asm_union's output.
+__global__ __launch_bounds__(256) void k_wdiag(NetDev d, const uint32_t* __restrict__ memb,
+                                               const uint32_t* __restrict__ asmSpk,
+                                               const uint32_t* __restrict__ any, uint32_t nstim, uint32_t sel,
+                                               uint32_t* __restrict__ hist) {
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (col >= d.ncol) return;  // wave-uniform
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    // a neuron takes part when its membership word is nonzero; its "within"
+    // word ORs the ballots of its assemblies below nstim
+    const uint32_t qmask = nstim >= 32u ? 0xFFFFFFFFu : ((1u << nstim) - 1u);
+    if (ballot32(memb[col * NPC + n] != 0u) == 0u) return;
+    const uint4 mq4 = reinterpret_cast<const uint4*>(memb + (size_t)col * NPC)[lane & 7u];
+    const uint32_t mq[4] = {mq4.x, mq4.y, mq4.z, mq4.w};
+    const uint32_t cn = d.conn[col * NSLOT + n];
+    const uint32_t w = lane >> 3;
+    const uint32_t sBeg = sel == 0u ? d.inbr : 0u, sEnd = sel == 1u ? d.inbr : NSLOT;
+    uint32_t acc[48];
+#pragma unroll
+    for (int i = 0; i < 48; ++i) acc[i] = 0u;
+    for (uint32_t s = sBeg; s < sEnd; ++s) {
+        const uint32_t c = 8u * __builtin_amdgcn_readlane(cn, s) + w;
+        const uint32_t tile = col * NSLOT + s;
+        const uint4 A3 = tile_quad(d.W3, tile), A2 = tile_quad(d.W2, tile), A1 = tile_quad(d.W1, tile),
+                    A0 = tile_quad(d.W0, tile);
+        const uint32_t anyp = any[c];
+        const uint32_t a3[4] = {A3.x, A3.y, A3.z, A3.w}, a2[4] = {A2.x, A2.y, A2.z, A2.w},
+                       a1[4] = {A1.x, A1.y, A1.z, A1.w}, a0[4] = {A0.x, A0.y, A0.z, A0.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (mq[j] == 0u) continue;
+            uint32_t within = 0u;
+            for (uint32_t m = mq[j] & qmask; m; m &= m - 1u)
+                within |= asmSpk[(size_t)__builtin_ctz(m) * d.ncol + c];
+            const uint32_t between = anyp & ~within;
+            uint32_t e[16];  // e[v]: the synapses whose counter is v
+            e[0] = ~a3[j];
+            e[8] = a3[j];
+#pragma unroll
+            for (int b = 2; b >= 0; --b) {  // split every mask on plane b
+                const uint32_t pb = b == 2 ? a2[j] : b == 1 ? a1[j] : a0[j];
+                const int step = 1 << (b + 1);
+#pragma unroll
+                for (int v = 0; v < 16; v += step) {
+                    e[v + (step >> 1)] = e[v] & pb;
+                    e[v] &= ~pb;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                acc[v] += popc(e[v] & within);
+                acc[16 + v] += popc(e[v] & between);
+                acc[32 + v] += popc(e[v]);
+            }
+        }
+    }
+    wave_add_counters<48>(acc, hist);
 }
 
 // k_bstest, mm:424-434
@@ -1865,23 +1958,26 @@ hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s) {
 
 hipError_t launch_within(const NetDev& d, const uint32_t* ball, const uint32_t* postMask, uint32_t useMask,
                          uint32_t* out, hipStream_t s) {
-    k_within<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, ball, postMask, useMask, out);
+    k_within<<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, ball, postMask, useMask, out);  // a wave per column
     return hipGetLastError();
 }
 
 hipError_t launch_indeg(const NetDev& d, uint32_t* out, hipStream_t s) {
-    k_indeg<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, out);
+    k_neuron_counts<0><<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, out);
     return hipGetLastError();
 }
 
 hipError_t launch_untouched(const NetDev& d, uint32_t* out, hipStream_t s) {
-    k_untouched<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, out);
+    k_neuron_counts<1><<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, out);
     return hipGetLastError();
 }
 
 hipError_t launch_wdiag(const NetDev& d, const uint32_t* memb, const uint32_t* asmSpk, uint32_t nstim,
-                        uint32_t sel, uint32_t* hist, hipStream_t s) {
-    k_wdiag<<<blocks_for((size_t)d.ncol * NPC, 256), 256, 0, s>>>(d, memb, asmSpk, nstim, sel, hist);
+                        uint32_t sel, uint32_t* anyAsm, uint32_t* hist, hipStream_t s) {
+    k_asm_union<<<blocks_for(d.ncol, 256), 256, 0, s>>>(asmSpk, nstim, d.ncol, anyAsm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_wdiag<<<blocks_for(d.ncol, 4), 256, 0, s>>>(d, memb, asmSpk, anyAsm, nstim, sel, hist);
     return hipGetLastError();
 }
 

@@ -319,6 +319,50 @@ def c4_serving(args):
     return out
 
 
+def readouts(args):
+    """The readout reductions on the trained full-scale network (12 patterns):
+    wall time per entry point call (device work plus the copy of its result)
+    and the algorithmic bytes it reads: the 28 recurrent tiles of every column
+    in W3 (indeg; within with every recurrent neuron selected) or in all four
+    planes (untouched), and the branches of assembly-member columns (wdiag)."""
+    import axon_amd
+
+    net = axon_amd.Net(axon_amd.Config.parse(C1))
+    c = net.config
+    train_presentations(net, 0, c.epochs * c.stim, c.T, c.trainclamp, c.stim)
+    net.sync()
+    refs = np.stack([net.present(p, 1.0, c.T, c.T, False, True, 2000 + p).spikes for p in range(c.stim)])
+    refs[:, :c.incols] = 0
+    n, ncol = net.n, net.ncol
+    memb = np.zeros(n, np.uint32)
+    for q in range(c.stim):
+        bits = np.unpackbits(refs[q].view(np.uint8), bitorder="little").astype(np.uint32)
+        memb |= bits << np.uint32(q)
+    allrec = np.zeros(n, np.uint32)
+    allrec[c.incols * 32:] = 1
+    rec_w3 = ncol * (32 - c.inbr) * 1024
+    member_cols = int(np.count_nonzero(memb.reshape(ncol, 32).any(axis=1)))
+    calls = {
+        "indeg": (lambda: net.indeg(), rec_w3),
+        "untouched": (lambda: net.untouched(), 4 * rec_w3),
+        "within_all_recurrent": (lambda: net.within(refs[0], allrec), (ncol - c.incols) * (32 - c.inbr) * 1024),
+        "within_one_assembly": (lambda: net.within(refs[0]), 0),
+        "wdiag_recurrent": (lambda: net.wdiag(memb, refs.ravel(), c.stim, 0), member_cols * (32 - c.inbr) * 4096),
+    }
+    out = {}
+    for name, (fn, nbytes) in calls.items():
+        fn()
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            fn()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        out[name] = {"ms_per_call": round(ms, 3), "algorithmic_bytes": nbytes,
+                     "GBps_wall": round(nbytes / (ms / 1e3) / 1e9, 1) if nbytes else None}
+    net.close()
+    return out
+
+
 def cpu_bench(args):
     """The oracle (C++ restatement of the reference kernels), column-parallel on
     the host cores, on the first presentations of the same training run."""
@@ -373,7 +417,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="c1,probes,networks,serving",
-                    help="comma list of c1, probes, networks, serving, calib (copy: counter calibration only)")
+                    help="comma list of c1, probes, networks, serving, calib, readouts (copy: counter "
+                         "calibration only)")
     ap.add_argument("--net-epochs", type=int, default=40)
     ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
@@ -399,10 +444,11 @@ def main():
     if "networks" in sections:
         batched["networks"] = c2_networks(args)
     serving = c4_serving(args) if "serving" in sections else None
+    reads = readouts(args) if "readouts" in sections else None
     cpu = None if args.no_cpu else cpu_bench(args)
     if g is None:  # a run of the batched sections alone (profiling): no headline figure
         print(json.dumps({"sections": sorted(sections), "batched_vs_plain": batched, "serving": serving,
-                          "hbm_calibration": calib, "cpu_baseline": cpu}), flush=True)
+                          "readouts": reads, "hbm_calibration": calib, "cpu_baseline": cpu}), flush=True)
         return
     out = {
         "metric": METRIC,

@@ -127,8 +127,21 @@ def test_wdiag_parity():
                 if (int(asm[q, c]) >> l) & 1:
                     m |= 1 << q
             memb[c * 32 + l] = m
-    for sel in (0, 1):
+    for sel in (0, 1, 2):
         assert np.array_equal(net.wdiag(memb, asm.ravel(), 12, sel), ora.wdiag(memb, asm.ravel(), 12, sel))
+    # fewer assemblies than membership bits: a member of any assembly takes part, but its "within"
+    # synapses come from assemblies below nstim only (mm:602, 618-622)
+    assert np.array_equal(net.wdiag(memb, asm[:5].ravel(), 5, 0), ora.wdiag(memb, asm[:5].ravel(), 5, 0))
+    # k_within's shared / exclusive split follows the multiplicity buffer
+    mult = np.array([bin(int(m)).count("1") for m in memb], np.uint32)
+    net.set_mult(mult)
+    ora.buf("MULT")[:] = mult
+    for q in (0, 7):
+        assert np.array_equal(net.within(asm[q]), ora.within(asm[q]))
+        assert np.array_equal(net.within(asm[q], (memb != 0).astype(np.uint32)),
+                              ora.within(asm[q], (memb != 0).astype(np.uint32)))
+    assert np.array_equal(net.indeg(), ora.indeg())
+    assert np.array_equal(net.untouched(), ora.untouched())
 
 
 def test_capacity_and_argument_errors():
