@@ -519,6 +519,16 @@ def probe_autotune(net: Net, stims: np.ndarray, T: int, clamp_steps: int, use_bi
     return best.value, waves.value, table
 
 
+def run(args: Sequence[str], trained: Optional[Net] = None, report_path: Optional[str] = None) -> None:
+    """axon_run: the reference program's whole run (main(), mm:669-1911) with
+    these key=value arguments; `trained` replaces training (its epochs done,
+    same training keys); the report goes to report_path (None = stdout)."""
+    argv = ["axon"] + list(args)
+    arr = (ctypes.c_char_p * len(argv))(*[a.encode() for a in argv])
+    _check(lib().axon_run(len(argv), arr, trained._h if trained is not None else None,
+                          os.fsencode(report_path) if report_path else None))
+
+
 def mem_probe(kind: int, nbytes: int, reps: int = 5) -> float:
     """Read GB/s of a streaming pass (0), 128-byte line gathers (1) or 1 KiB tile gathers (2)."""
     g = ctypes.c_double()

@@ -44,6 +44,35 @@ axon_status fail(axon_status s, const char* fmt, ...) {
     return s;
 }
 
+}  // namespace
+
+namespace axon {
+// Sets the calling thread's axon_last_error() message for the other source
+// files of the library (report.cpp).
+axon_status set_error(axon_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+}  // namespace axon
+
+namespace {
+
+// Entry points that size host memory from their arguments or a file run
+// through this, so an exception never crosses the C ABI: host memory
+// exhaustion becomes AXON_ERR_OUT_OF_MEMORY.
+template <class F>
+axon_status guarded(const char* what, F&& f) noexcept {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(AXON_ERR_OUT_OF_MEMORY, "%s: host memory exhausted", what);
+    } catch (const std::exception& e) {
+        return fail(AXON_ERR_DEVICE, "%s: %s", what, e.what());
+    } catch (...) {
+        return fail(AXON_ERR_DEVICE, "%s: unexpected exception", what);
+    }
+}
+
 #define HIPCHECK(x)                                                                                  \
     do {                                                                                             \
         hipError_t e_ = (x);                                                                         \
@@ -252,7 +281,7 @@ extern "C" void axon_net_destroy(axon_net* n) {
     delete n;
 }
 
-extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
+static axon_status net_create(const axon_config* c, axon_net** out) {
     if (!c || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     axon_status st = axon_config_validate(c);
@@ -698,8 +727,8 @@ static axon_status per_neuron(axon_net* n, uint32_t* out, hipError_t (*fn)(const
 extern "C" axon_status axon_indeg(axon_net* n, uint32_t* out) { return per_neuron(n, out, axon::launch_indeg); }
 extern "C" axon_status axon_untouched(axon_net* n, uint32_t* out) { return per_neuron(n, out, axon::launch_untouched); }
 
-extern "C" axon_status axon_wdiag(axon_net* n, const uint32_t* memb, const uint32_t* asm_spk, uint32_t nstim,
-                                  uint32_t sel, uint32_t hist[48]) {
+static axon_status wdiag(axon_net* n, const uint32_t* memb, const uint32_t* asm_spk, uint32_t nstim,
+                         uint32_t sel, uint32_t hist[48]) {
     if (!n || !memb || !asm_spk || !hist) return fail(AXON_ERR_INVALID_ARG, "null argument");
     if (nstim > 32) return fail(AXON_ERR_INVALID_ARG, "nstim must be <= 32 (membership is a u32 mask)");
     HIPCHECK(hipSetDevice(n->device));
@@ -812,8 +841,7 @@ extern "C" void axon_probes_destroy(axon_probes* b) {
     delete b;
 }
 
-extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t max_t, uint32_t tile,
-                                          axon_probes** out) {
+static axon_status probes_create(axon_net* n, uint32_t slots, uint32_t max_t, uint32_t tile, axon_probes** out) {
     if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     bool known = false;
@@ -960,9 +988,8 @@ extern "C" axon_status axon_probes_read(axon_probes* b, uint32_t first, uint32_t
     return AXON_OK;
 }
 
-extern "C" axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
-                                       uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes,
-                                       double* ms) {
+static axon_status probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
+                              uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes, double* ms) {
     if (!b || !stims) return fail(AXON_ERR_INVALID_ARG, "null argument");
     if (count > b->d.nslots) return fail(AXON_ERR_CAPACITY, "%u probes exceed %u slots", count, b->d.nslots);
     std::vector<uint32_t> Ts(b->d.nslots, 0);
@@ -1116,7 +1143,7 @@ extern "C" void axon_multi_destroy(axon_multi* b) {
 
 // Networks must share every dynamics parameter; seeds, stored-pattern counts
 // and epochs may differ.
-extern "C" axon_status axon_multi_create(axon_net* const* nets, uint32_t nnet, axon_multi** out) {
+static axon_status multi_create(axon_net* const* nets, uint32_t nnet, axon_multi** out) {
     if (!nets || !out || nnet == 0) return fail(AXON_ERR_INVALID_ARG, "null argument or no networks");
     *out = nullptr;
     const axon_config& c0 = nets[0]->cfg;
@@ -1194,7 +1221,7 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
 // extra = 1, the additional presentation of pattern 0 (mm:1033).  Network i's
 // work queue holds its own presentations; within a group all networks
 // advance one presentation per round and drop out when their queue is empty.
-extern "C" axon_status axon_multi_train(axon_multi* b, const uint32_t* epochs, uint32_t extra, double* ms) {
+static axon_status multi_train(axon_multi* b, const uint32_t* epochs, uint32_t extra, double* ms) {
     if (!b || !epochs) return fail(AXON_ERR_INVALID_ARG, "null argument");
     const uint32_t nnet = (uint32_t)b->nets.size();
     std::vector<uint32_t> len(nnet);
@@ -1389,8 +1416,8 @@ extern "C" void axon_server_destroy(axon_server* s) {
     delete s;
 }
 
-extern "C" axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
-                                          axon_server** out) {
+static axon_status server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
+                                 axon_server** out) {
     if (!nets || !cfg || !out || nnet == 0) return fail(AXON_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     if (cfg->slots_per_net == 0 || cfg->steps_per_tick == 0) return fail(AXON_ERR_INVALID_ARG, "slots and steps_per_tick >= 1");
@@ -1425,8 +1452,8 @@ extern "C" axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, 
 
 extern "C" double axon_server_clock(const axon_server* s) { return s ? s->now() : 0.0; }
 
-extern "C" axon_status axon_server_submit(axon_server* s, uint32_t net, const uint32_t* stim, uint32_t T,
-                                          uint32_t clamp_steps, uint32_t use_bias, double arrival_s, uint64_t* id) {
+static axon_status server_submit(axon_server* s, uint32_t net, const uint32_t* stim, uint32_t T,
+                                 uint32_t clamp_steps, uint32_t use_bias, double arrival_s, uint64_t* id) {
     if (!s || !stim || !id) return fail(AXON_ERR_INVALID_ARG, "null argument");
     if (net >= s->nets.size()) return fail(AXON_ERR_INVALID_ARG, "network %u out of range", net);
     if (T == 0 || T > s->cfg.max_t) return fail(AXON_ERR_CAPACITY, "T=%u outside 1..max_t=%u", T, s->cfg.max_t);
@@ -1446,7 +1473,7 @@ extern "C" axon_status axon_server_submit(axon_server* s, uint32_t net, const ui
     return AXON_OK;
 }
 
-extern "C" axon_status axon_server_tick(axon_server* s, uint32_t* completed) {
+static axon_status server_tick(axon_server* s, uint32_t* completed) {
     if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
     const uint32_t nnet = (uint32_t)s->nets.size(), NC = s->ncol;
     HIPCHECK(hipSetDevice(s->nets[0]->device));
@@ -1663,7 +1690,7 @@ extern "C" axon_status axon_net_config(const axon_net* n, axon_config* out) {
     return AXON_OK;
 }
 
-extern "C" axon_status axon_net_save(axon_net* n, const char* path) {
+static axon_status net_save(axon_net* n, const char* path) {
     if (!n || !path) return fail(AXON_ERR_INVALID_ARG, "null argument");
     SnapHeader h{};
     memcpy(h.magic, kSnapMagic, 8);
@@ -1715,7 +1742,7 @@ extern "C" axon_status axon_net_save(axon_net* n, const char* path) {
     return AXON_OK;
 }
 
-extern "C" axon_status axon_net_load(const char* path, axon_net** out) {
+static axon_status net_load(const char* path, axon_net** out) {
     if (!path || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     FILE* f = fopen(path, "rb");
@@ -1736,6 +1763,16 @@ extern "C" axon_status axon_net_load(const char* path, axon_net** out) {
         if (h.words[i] != want[i]) return fail(AXON_ERR_BAD_SNAPSHOT, "%s: buffer %d size differs from its config", path, i);
     if (h.nstim != h.cfg.stim || h.stimcols != h.cfg.stimcols)
         return fail(AXON_ERR_BAD_SNAPSHOT, "%s: stimulus table size differs from its config", path);
+    // the file must hold exactly header + buffers + stimulus table + checksum
+    // before anything is sized from the header
+    uint64_t expect = sizeof h + 8u + 4ull * h.nstim * h.stimcols;
+    for (int i = 0; i < 6; ++i) expect += 4ull * h.words[i];
+    if (fseeko(f, 0, SEEK_END) != 0) return fail(AXON_ERR_IO, "%s: cannot seek", path);
+    const off_t flen = ftello(f);
+    if (flen < 0 || (uint64_t)flen != expect)
+        return fail(AXON_ERR_BAD_SNAPSHOT, "%s: %lld bytes, its header describes %llu", path, (long long)flen,
+                    (unsigned long long)expect);
+    if (fseeko(f, (off_t)sizeof h, SEEK_SET) != 0) return fail(AXON_ERR_IO, "%s: cannot seek", path);
     Fnv1a sum;
     sum.add(&h, sizeof h);
     std::vector<std::vector<uint32_t>> bufs(6);
@@ -1814,4 +1851,49 @@ extern "C" axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t re
     if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "memory probe: %s", hipGetErrorString(e));
     *gbps = best;
     return AXON_OK;
+}
+
+// ================================================================== guarded entry points
+// The entry points whose host allocations depend on their arguments or on a
+// file: the bodies above, behind guarded() (no exception crosses the C ABI).
+extern "C" axon_status axon_net_create(const axon_config* c, axon_net** out) {
+    return guarded("axon_net_create", [&] { return net_create(c, out); });
+}
+extern "C" axon_status axon_net_save(axon_net* n, const char* path) {
+    return guarded("axon_net_save", [&] { return net_save(n, path); });
+}
+extern "C" axon_status axon_net_load(const char* path, axon_net** out) {
+    return guarded("axon_net_load", [&] { return net_load(path, out); });
+}
+extern "C" axon_status axon_wdiag(axon_net* n, const uint32_t* memb, const uint32_t* asm_spk, uint32_t nstim,
+                                  uint32_t sel, uint32_t hist[48]) {
+    return guarded("axon_wdiag", [&] { return wdiag(n, memb, asm_spk, nstim, sel, hist); });
+}
+extern "C" axon_status axon_probes_create(axon_net* n, uint32_t slots, uint32_t max_t, uint32_t tile,
+                                          axon_probes** out) {
+    return guarded("axon_probes_create", [&] { return probes_create(n, slots, max_t, tile, out); });
+}
+extern "C" axon_status axon_probes_run(axon_probes* b, uint32_t count, const uint32_t* stims, uint32_t T,
+                                       uint32_t clamp_steps, uint32_t use_bias, uint32_t* hist, uint32_t* spikes,
+                                       double* ms) {
+    return guarded("axon_probes_run",
+                   [&] { return probes_run(b, count, stims, T, clamp_steps, use_bias, hist, spikes, ms); });
+}
+extern "C" axon_status axon_multi_create(axon_net* const* nets, uint32_t nnet, axon_multi** out) {
+    return guarded("axon_multi_create", [&] { return multi_create(nets, nnet, out); });
+}
+extern "C" axon_status axon_multi_train(axon_multi* b, const uint32_t* epochs, uint32_t extra, double* ms) {
+    return guarded("axon_multi_train", [&] { return multi_train(b, epochs, extra, ms); });
+}
+extern "C" axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
+                                          axon_server** out) {
+    return guarded("axon_server_create", [&] { return server_create(nets, nnet, cfg, out); });
+}
+extern "C" axon_status axon_server_submit(axon_server* s, uint32_t net, const uint32_t* stim, uint32_t T,
+                                          uint32_t clamp_steps, uint32_t use_bias, double arrival_s, uint64_t* id) {
+    return guarded("axon_server_submit",
+                   [&] { return server_submit(s, net, stim, T, clamp_steps, use_bias, arrival_s, id); });
+}
+extern "C" axon_status axon_server_tick(axon_server* s, uint32_t* completed) {
+    return guarded("axon_server_tick", [&] { return server_tick(s, completed); });
 }

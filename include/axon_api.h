@@ -64,6 +64,20 @@ void axon_config_defaults(axon_config* c);
 axon_status axon_config_parse(int argc, const char* const* argv, axon_config* c);
 axon_status axon_config_validate(const axon_config* c);
 
+/* ---- the whole program: replaces the reference's main() (mm:669-1911) ----
+ * Parses the key=value arguments (argv[0] is the program name, as for the
+ * CLI), builds and trains the network, runs the reference's protocol and
+ * prints its report (every block the keys switch on, byte for byte the
+ * reference's lines) to report_path (NULL = stdout).  save=<path> and
+ * load=<path> store or reuse the network after its training epochs.
+ * trained: NULL, or a network created with the same training keys whose
+ * epochs are done (e.g. by axon_multi_train with extra = 0); the run uses it
+ * in place of training and continues on it (its extra learning presentation,
+ * mm:1033, changes it); the caller keeps ownership.  Keys that only select
+ * readouts or printing (quiet, split, unionchar, diag, profile, vtopk, vbs,
+ * deep, wave, wavet) may differ from the trained network's. */
+axon_status axon_run(int argc, const char* const* argv, axon_net* trained, const char* report_path);
+
 /* ---- network: replaces the buffer block mm:749-768, k_init_w/k_init_conn
  * dispatch mm:770-782 and the stimulus draw mm:793-799 ---- */
 axon_status axon_net_create(const axon_config* c, axon_net** out);

@@ -1,8 +1,11 @@
 """Writes the oracle-derived golden fixtures used by the GPU tests.
 
 Runs the CPU oracle (oracle/axon_oracle) on the full-scale golden-row
-configuration and on its untrained control, and stores the report lines the
-reference's bench scripts parse.  Usage: python tests/golden/make_fixtures.py
+configuration, on its untrained control and on three small (ncol=512)
+configurations, each with every report block switched on (deep, wave, diag,
+split, unionchar), and stores the full report (every line the oracle
+restates; the GPU tests look for each of them, in order, in the CLI's
+output).  Usage: python tests/golden/make_fixtures.py
 """
 import json
 import os
@@ -16,10 +19,15 @@ sys.path.insert(0, os.path.dirname(HERE))
 import oracle  # noqa: E402
 from conftest import CONFIG_C  # noqa: E402
 
+BLOCKS = "diag=1 split=1 unionchar=1".split()
 RUNS = {
-    "c1_trained": CONFIG_C + "stim=12 epochs=40 deep=1 wave=1 wavet=35 seed=0".split(),
-    "c1_untrained": CONFIG_C + "stim=12 epochs=0 deep=1 wave=1 wavet=35 seed=0".split(),
-    "c0_small": CONFIG_C + "ncol=512 incols=128 stim=12 epochs=10 deep=1 wave=1 wavet=35 seed=0".split(),
+    "c1_trained": CONFIG_C + "stim=12 epochs=40 deep=1 wave=1 wavet=35 seed=0".split() + BLOCKS,
+    "c1_untrained": CONFIG_C + "stim=12 epochs=0 deep=1 wave=1 wavet=35 seed=0".split() + BLOCKS,
+    "c0_small": CONFIG_C + "ncol=512 incols=128 stim=12 epochs=10 deep=1 wave=1 wavet=35 seed=0".split() + BLOCKS,
+    # the reference's built-in defaults (legacy rule, margin gate, trainclamp = T), small
+    "c0_defaults": "ncol=512 incols=128 stimcols=64 stim=8 epochs=4 deep=1 wave=1 wavet=30 seed=3".split() + BLOCKS,
+    # Configuration C with longer presentations (deep trace 40 steps, 20-step tables)
+    "c0_t20": CONFIG_C + "ncol=512 incols=128 stim=6 epochs=3 T=20 deep=1 wave=1 wavet=45 seed=1".split() + BLOCKS,
 }
 
 

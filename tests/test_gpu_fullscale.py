@@ -77,6 +77,51 @@ def test_untrained_control_load48_table1_and_table3():
         close(de, t3["dens_exclusive"], 2), (frac, ds, de, t3)
 
 
+def _table(name, tmp_path):
+    from axon_amd import sweep
+
+    loads, seeds = sweep.TABLES[name][0], sweep.TABLES[name][1]
+    outs = sweep.run_table(name, CONFIG_C, loads, seeds, 40, str(tmp_path), lambda m: print(m, flush=True))
+    lines, rows = sweep.format_table(name, loads, seeds, 40, outs)
+    print("\n".join(lines), flush=True)
+    return rows
+
+
+def test_table2_five_seeds_networks_trained_together(tmp_path):
+    """bench/seeds.sh over loads 24-60 x seeds 0-4 (paper:284-293): 25 networks
+    trained in shared launches, each reported in-process."""
+    rows = _table("seeds", tmp_path)
+    for pub in PUB["table2"]:
+        r = rows[pub["stim"]]
+        for key, dec in (("sustain_mean", 3), ("sustain_sd", 3), ("margin_mean", 3), ("margin_sd", 3),
+                         ("within_mean", 2), ("within_sd", 2)):
+            assert close(r[key], pub[key], dec), (pub["stim"], key, r[key], pub[key])
+        # the paper's CV is the printed SD over the printed mean (3 decimals each)
+        cv = 100.0 * round(r["sustain_sd"], 3) / round(r["sustain_mean"], 3)
+        assert close(cv, pub["cv_pct"], 1), (pub["stim"], cv, r)
+
+
+def test_table4_loop_gain_five_seeds(tmp_path):
+    """bench/table4_gaintraj.sh over loads 12, 36, 60 x seeds 0-4 (paper:391-399)."""
+    rows = _table("table4", tmp_path)
+    for pub in PUB["table4"]:
+        for got, want in zip(rows[pub["stim"]], pub["gain"]):
+            assert close(got, want, 3), (pub["stim"], rows[pub["stim"]], pub["gain"])
+
+
+def test_table5_visited_population_five_seeds(tmp_path):
+    """bench/table5_union.sh over loads 12, 36, 60 x seeds 0-4 and the untrained
+    load-36 control over seeds 0-2 (paper:454-463)."""
+    rows = _table("table5", tmp_path)
+    for pub in PUB["table5"]:
+        m = rows[(pub["stim"], pub["epochs"])]
+        got = dict(in_asm=m[0], outside=m[1], dens_in=m[2], dens_out=m[3], dens_all=m[4], mult_out=m[5],
+                   mult_pool=m[6], pct_outside=m[7])
+        for key, dec in (("in_asm", 0), ("outside", 0), ("pct_outside", 1), ("dens_in", 2), ("dens_out", 2),
+                         ("dens_all", 2), ("mult_out", 2), ("mult_pool", 2)):
+            assert close(got[key], pub[key], dec), (pub["stim"], pub["epochs"], key, got[key], pub[key])
+
+
 def train_alone(net):
     """The CLI's training: epochs x patterns, then the extra presentation of
     pattern 0 (src/axon.mm:1009-1010, 1033)."""

@@ -76,9 +76,14 @@ def test_untrained_control_full_scale():
     assert_lines_in_order(out, ORA["c1_untrained"]["report"])
 
 
-def test_small_config_report_matches_oracle():
-    out = run_cli(ORA["c0_small"]["args"])
-    assert_lines_in_order(out, ORA["c0_small"]["report"])
+@pytest.mark.parametrize("name", ["c0_small", "c0_defaults", "c0_t20"])
+def test_small_config_report_matches_oracle(name):
+    """Every report block (settle, epochs, firing steps, diag, A1, A3, A4, A4b,
+    A4c, UNIONCHAR, SHARED vs EXCLUSIVE, scaffold, multiplicity, PARITY, WAVE,
+    free-run diagnostic, final table): the oracle's lines, in order."""
+    out = run_cli(ORA[name]["args"])
+    assert len(ORA[name]["report"]) > 150
+    assert_lines_in_order(out, ORA[name]["report"])
 
 
 def test_cli_self_checks():

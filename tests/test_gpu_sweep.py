@@ -1,7 +1,8 @@
 """Batched sweep: a CLI run that starts from a snapshot (load=) prints the same
 report as the full run that saved it (save=), apart from the training lines;
-networks trained together in shared launches save the same snapshot as the
-CLI's own training; the sweep driver's table parses those reports."""
+the in-process report (axon_run) on networks trained together in shared
+launches is the CLI's full-run report; the sweep driver's tables parse those
+reports."""
 import os
 import subprocess
 
@@ -45,25 +46,43 @@ def test_report_from_snapshot_matches_full_run(tmp_path):
     assert len(a) == len(b) and not diff, diff[:5]
 
 
-def test_shared_training_saves_the_cli_network(tmp_path):
-    specs = [(4, 0), (6, 1), (4, 2)]
-    paths, _ = sweep.train_and_save(KEYS, specs, 2, str(tmp_path), lambda m: None)
-    for (n, sd), p in zip(specs, paths):
-        own = str(tmp_path / f"cli{n}_{sd}.axs")
-        cli(KEYS + [f"stim={n}", "epochs=2", f"seed={sd}", "quiet=1"] + sweep.READOUT + [f"save={own}"])
-        x, y = axon_amd.Net.load(p), axon_amd.Net.load(own)
-        for b in ("W0", "W1", "W2", "W3", "CONN", "BIAS"):
-            assert axon_amd.checksum(x, b) == axon_amd.checksum(y, b), (n, sd, b)
-        x.close()
-        y.close()
+def test_snapshot_reused_for_other_readouts(tmp_path):
+    """A snapshot saved by a plain run serves a run with other readout keys
+    (split, unionchar, diag): its report equals the full run with those keys;
+    a training key that differs is refused."""
+    snap = str(tmp_path / "b.axs")
+    base = KEYS + ["stim=5", "epochs=2", "seed=4"]
+    cli(base + ["quiet=1", f"save={snap}"])
+    extra = ["split=1", "unionchar=1", "diag=1", "deep=1", "wave=1", "wavet=30"]
+    full = comparable(cli(base + extra))
+    again = comparable(cli(base + extra + [f"load={snap}"]))
+    diff = [(x, y) for x, y in zip(full, again) if x != y]
+    assert len(full) == len(again) and not diff, diff[:5]
+    assert any(ln.startswith("UNIONCHAR") for ln in again)
+    r = subprocess.run([sweep.AXON_BIN] + base + ["biasup=70", f"load={snap}"], capture_output=True, text=True)
+    assert r.returncode == 2 and "training keys" in r.stderr
 
 
-def test_sweep_table(tmp_path, capsys):
-    sweep.main(["--table", "seeds", "--stim", "4,6", "--seeds", "0,1", "--epochs", "2", "--keys", " ".join(SMALL),
+def test_report_on_networks_trained_together_is_the_cli_report(tmp_path):
+    specs = [(4, 0, 2), (6, 1, 2), (4, 2, 2), (5, 3, 0)]
+    nets, _ = sweep.train_together(KEYS, specs, lambda m: None)
+    extra = ["split=1", "unionchar=1", "diag=1"] + sweep.READOUT
+    for spec, net in zip(specs, nets):
+        n, sd, ep = spec
+        got = comparable(sweep.report(KEYS, spec, extra, net, str(tmp_path)))
+        want = comparable(cli(KEYS + [f"stim={n}", f"epochs={ep}", f"seed={sd}"] + extra))
+        diff = [(x, y) for x, y in zip(want, got) if x != y]
+        assert len(got) == len(want) and not diff, (spec, diff[:5])
+        net.close()
+
+
+@pytest.mark.parametrize("table", sorted(sweep.TABLES))
+def test_sweep_tables_small(table, tmp_path, capsys):
+    loads = {"cap10": "4,6", "seeds": "4,6", "table3": "4,6", "table4": "4,6", "table5": "4,6"}[table]
+    sweep.main(["--table", table, "--stim", loads, "--seeds", "0,1", "--epochs", "2", "--keys", " ".join(SMALL),
                 "--out", str(tmp_path)])
     out = capsys.readouterr().out
-    rows = [ln for ln in out.splitlines() if ln.strip().startswith("NSTIM=")]
-    assert len(rows) == 2 and "nan" not in out, out
+    assert "nan" not in out, out
+    print(out)
     for n in (4, 6):
-        for sd in (0, 1):
-            assert os.path.getsize(tmp_path / f"stim{n}_seed{sd}.txt") > 1000
+        assert os.path.getsize(tmp_path / f"stim{n}_seed0_ep2.txt") > 1000
