@@ -316,8 +316,10 @@ static axon_status net_create(const axon_config* c, axon_net** out) {
         {&d.refBall, n->NCOL}, {&d.mult, n->N}, {&n->d_small, 64}, {&n->d_ball, n->NCOL}, {&n->d_mask, n->N},
         {&d.cntProp, (size_t)n->NCOL * 2}, {&d.cntLearn, (size_t)n->NCOL * 2},
         {(uint32_t**)&d.firedW, (size_t)n->NCOL * axon::NSLOT / 4}, {(uint32_t**)&d.tileBand, (size_t)n->NCOL * axon::NSLOT / 4},
-        {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4}, {&d.learnList, (size_t)n->NCOL * 2},
-        {(uint32_t**)&d.tileList, (size_t)n->NCOL * axon::NSLOT * 2 * 4}, {&d.learnCnt, 2 * 8 * 32},
+        {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4},
+        {&d.learnList, (size_t)2 * axon::NLISTS * axon::list_cols(n->NCOL)},
+        {(uint32_t**)&d.tileList, (size_t)2 * axon::NLISTS * axon::list_cols(n->NCOL) * axon::NSLOT * 4},
+        {&d.learnCnt, 2 * axon::NLISTS * 32},
     };
     for (auto& b : bufs) {
         if ((st = dev_alloc(n, &p, b.words * 4)) != AXON_OK) return cleanup(st);

@@ -19,6 +19,11 @@ constexpr uint32_t COLS_PER_GRP = 8;
 // kernel reads only the lines of presynaptic columns that spiked.
 constexpr uint32_t TILE_WORDS = 256;  // uint32 per tile per plane
 
+// Plasticity work lists per step parity, and the columns one list can hold
+// (column c is in list (c / 4) % NLISTS).
+constexpr uint32_t NLISTS = 8;
+__host__ __device__ constexpr uint32_t list_cols(uint32_t ncol) { return (ncol + 31u) / 32u * 4u; }
+
 // Per-step kernel parameters.  Same 26 words as the reference Params
 // (mm:56-83); the fields no kernel reads (learn, freeze, theta) are kept so
 // the struct stays a faithful mirror.
@@ -64,10 +69,10 @@ struct NetDev {
     // {column * 32 + slot, presynaptic group, learning-neuron mask, flags};
     // flags = fired-column mask of the group | 0x100 when all 8 words of
     // the tile must be rewritten).
-    // Each parity holds 8 lists (column c in list c % 8) so that columns take
-    // list slots on 8 different counters.
-    uint32_t* learnList; // [2][8][ncol / 8]
-    uint4* tileList;     // [2][8][ncol / 8 * 32]
+    // Each parity holds 8 lists (column c in list (c / 4) % 8) so that
+    // columns take list slots on 8 different counters.
+    uint32_t* learnList; // [2][8][list_cols(ncol)]
+    uint4* tileList;     // [2][8][list_cols(ncol) * 32]
     uint32_t* learnCnt;  // [2][8] list lengths, 32 words apart (reset by the step after next's propagation)
     uint32_t ncol, incols, inbr, topkpar;
     // active: batched training only -- this round's pattern of the network

@@ -96,13 +96,17 @@ __device__ __forceinline__ uint32_t half_or(uint32_t x) {
     return x;
 }
 
-// Plasticity work lists: per step parity, NLISTS lists (column c goes to list
-// c % NLISTS) so the list-slot atomics of different columns hit different
-// counters; each counter sits on its own 128-byte line.
-constexpr uint32_t NLISTS = 8;
+// Plasticity work lists: per step parity, NLISTS lists, so the list-slot
+// atomics of different columns hit different counters; each counter sits on
+// its own 128-byte line.  Column c goes to list (c / 4) % 8: workgroups are
+// dealt to the 8 XCDs in turn and propagation runs 4 columns per workgroup,
+// so that is the XCD that propagated the column, and list l is rewritten by
+// workgroups on XCD l (its W3 lines and spike words are in that XCD's L2).
+// A list holds at most list_cols(ncol) columns.
 __device__ __forceinline__ uint32_t* list_count(const NetDev& d, uint32_t par, uint32_t l) {
     return d.learnCnt + (par * NLISTS + l) * 32u;
 }
+__device__ __forceinline__ uint32_t list_of(uint32_t col) { return (col >> 2) & (NLISTS - 1u); }
 
 // Word-major tile addressing (see TILE_WORDS): word w of neuron n of a tile.
 __device__ __forceinline__ size_t word_idx(uint32_t tile, uint32_t w, uint32_t n) {
@@ -729,7 +733,7 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
         const uint32_t lines = wave_sum(v ? (full_tile_update(P, band) ? 8u : popc(fwb)) : 0u);
         if (lane == 0u) {
             reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
-            const uint32_t l = col % NLISTS, cap = d.ncol / NLISTS;
+            const uint32_t l = list_of(col), cap = list_cols(d.ncol);
             d.learnList[(par * NLISTS + l) * cap + atomicAdd(list_count(d, par, l), 1u)] = col;
         }
         return;
@@ -742,15 +746,15 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
     const uint32_t lines = wave_sum(want != 0u ? (full ? 8u : popc(fwb)) : 0u);
     uint32_t base = 0u;
     if (lane == 0u) {
-        base = atomicAdd(list_count(d, par, col % NLISTS), popc(voted));
+        base = atomicAdd(list_count(d, par, list_of(col)), popc(voted));
         // only this column's wave writes its counters: a plain read-modify-write
         reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
     }
     base = __shfl(base, 0);
     if (want != 0u) {
         const uint32_t at = base + popc(voted & ((1u << lane) - 1u));
-        const size_t cap = (size_t)d.ncol / NLISTS * NSLOT;  // a list's columns x 32 tiles
-        d.tileList[(par * NLISTS + col % NLISTS) * cap + at] =
+        const size_t cap = (size_t)list_cols(d.ncol) * NSLOT;  // a list's columns x 32 tiles
+        d.tileList[(par * NLISTS + list_of(col)) * cap + at] =
             make_uint4(col * NSLOT + lane, cn, want, fwb | (full ? 0x100u : 0u));
     }
 }
@@ -782,7 +786,7 @@ __device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, 
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                           uint32_t* __restrict__ spkOut, uint32_t col, const EmitIn& in,
-                                          uint32_t th) {
+                                          bool gatePass) {
     const uint32_t lane = lane_id();
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
@@ -790,10 +794,9 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
     const uint32_t gid = col * NPC + n;
     const bool clampCol = P.clamp != 0u && col < P.inCols;
     const bool rec = col >= P.inCols;
-    const uint32_t dv = in.dv, v = in.v, stimW = in.stimW, bbv = in.bbv, cn = in.cn, fwb = in.fwb, band = in.band;
+    const uint32_t v = in.v, stimW = in.stimW, bbv = in.bbv, cn = in.cn, fwb = in.fwb, band = in.band;
     const uint32_t bia = in.bia;
     const uint2 lc = in.lc;
-    const bool gatePass = dv >= th;
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
     if (clampCol || gatePass) {
         const Kwta kw = kwta32(v, P.k, n);
@@ -887,7 +890,7 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
     const uint32_t par = P.step & 1u;
-    const uint32_t l = blockIdx.x % NLISTS, cap = d.ncol / NLISTS;  // workgroup b serves list b % NLISTS
+    const uint32_t l = blockIdx.x % NLISTS, cap = list_cols(d.ncol);  // workgroup b serves list b % NLISTS
     if (blockIdx.x / NLISTS >= gridDim.x / NLISTS) return;  // workgroup-uniform: no barrier is skipped by part of it
     const uint32_t nlist = *list_count(d, par, l);
     for (uint32_t li = blockIdx.x / NLISTS; li < nlist; li += gridDim.x / NLISTS) {
@@ -940,7 +943,7 @@ __device__ __forceinline__ void learn_tiles(const NetDev& d, const Params& P, co
     const uint32_t par = P.step & 1u;
     const uint32_t l = wg % NLISTS, w0 = wg / NLISTS, nwaves = nwg / NLISTS;
     if (w0 >= nwaves) return;  // a grid that is not a multiple of NLISTS waves leaves these idle
-    const uint32_t cap = d.ncol / NLISTS * NSLOT;
+    const uint32_t cap = list_cols(d.ncol) * NSLOT;
     const uint4* __restrict__ list = d.tileList + (size_t)(par * NLISTS + l) * cap;
     // the first entry is read together with the count, unconditionally (the
     // index is clamped into the list; used only if w0 < count)
@@ -962,19 +965,80 @@ __global__ __launch_bounds__(64) void k_learn_tiles(NetDev d, Params P, const ui
     learn_tiles(d, P, spkIn, blockIdx.x, gridDim.x);
 }
 
-// Column top-K and emit in one kernel for a single network: every workgroup
-// (one per CU at 16 columns each) selects theta over all column drives itself
-// -- a few microseconds of LDS work on 16 KiB that the drive kernel left in
-// cache -- instead of waiting for a separate one-workgroup launch; workgroup 0
-// also stores (theta, need).  Then each wave emits one column.
+// The column gate (drive >= max(theta, 1), mm:211) of a 1024-thread
+// workgroup's 16 columns, one per wave, when 1 <= target <= recurrent columns
+// <= 4096.  Theta is then the target-th largest recurrent drive under both
+// top-K rules (every radix pass has a crossing bin), so a column passes
+// exactly when its drive is at least 1 and fewer than `target` recurrent
+// drives exceed it.  The workgroup holds the recurrent drives in registers
+// (4 per thread); each wave counts, for every one of the 16 columns, the
+// drives of its 256 that exceed the column's (ballots), and after one
+// barrier each wave adds up its own column's 16 partial counts.  This takes
+// the place of the two or three histogram passes of the radix select.
+// Column of wave w of emit workgroup b: the workgroups on XCD x = b % 8 take
+// the columns that propagation ran on XCD x (its 4-column workgroups j with
+// j % 8 = x), so a column's accumulators, NMDA bits and fired masks are read
+// from the L2 that received them.  The grid is ceil(ncol / 128) * 8
+// workgroups; waves past ncol emit nothing.
+__device__ __forceinline__ uint32_t emit_col(uint32_t b, uint32_t w) {
+    const uint32_t j = (b & 7u) + 8u * (4u * (b >> 3) + (w >> 2));
+    return 4u * j + (w & 3u);
+}
+
+__device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive, uint32_t ncol, uint32_t inCols,
+                                              uint32_t target, uint32_t dv) {
+    __shared__ uint32_t part[16][16];  // [wave][column of the workgroup]
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    uint32_t rv[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+        const uint32_t c = inCols + tid + i * 1024u;
+        rv[i] = c < ncol ? drive[c] : 0u;  // past the end: 0, which exceeds no drive
+    }
+    const uint32_t dl = drive[min(emit_col(blockIdx.x, lane & 15u), ncol - 1u)];  // lane j < 16: wave j's column
+    uint32_t mine = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; ++j) {
+        const uint32_t dj = __builtin_amdgcn_readlane(dl, j);
+        uint32_t cnt = 0u;
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) cnt += (uint32_t)__popcll(__ballot(rv[i] > dj));
+        if (lane == j) mine = cnt;
+    }
+    if (lane < 16u) part[wave][lane] = mine;
+    __syncthreads();
+    uint32_t tot = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 16u; ++w) tot += part[w][wave];
+    return dv >= 1u && tot < target;
+}
+
+// Column gate and emit in one kernel for a single network, one wave per
+// column, 16 columns per workgroup.  In the usual case (gate_by_count) each
+// workgroup counts drives above its columns' instead of selecting theta;
+// theta itself is then computed only where it is an output (the side
+// statistics of a synchronous presentation: workgroup 0 runs the radix
+// select and stores (theta, need)).  Otherwise every workgroup selects theta
+// over all column drives itself (k_topk_par / k_topk_serial, mm:259-352) --
+// LDS work on 16 KiB the drive kernel left in cache -- instead of waiting for
+// a separate one-workgroup launch.
 template <bool LEARN, bool STATS>
 __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
                                                uint32_t* __restrict__ spkOut) {
-    const uint32_t col = blockIdx.x * 16u + (threadIdx.x >> 6);
-    const EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the top-K
-    const uint32_t th = topk_block<1024>(d.drive, blockIdx.x == 0u ? d.theta : nullptr, d.ncol, P.inCols,
-                                         P.targetCols, d.topkpar);
-    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, th);
+    const uint32_t col = emit_col(blockIdx.x, threadIdx.x >> 6);
+    const EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the gate
+    const uint32_t nrec = d.ncol - P.inCols;
+    bool gate;
+    if (P.targetCols >= 1u && P.targetCols <= nrec && nrec <= 4096u) {  // uniform over the grid
+        if (STATS && blockIdx.x == 0u)
+            (void)topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+        gate = gate_by_count(d.drive, d.ncol, P.inCols, P.targetCols, in.dv);
+    } else {
+        const uint32_t th = topk_block<1024>(d.drive, blockIdx.x == 0u ? d.theta : nullptr, d.ncol, P.inCols,
+                                             P.targetCols, d.topkpar);
+        gate = in.dv >= th;
+    }
+    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, gate);
 }
 
 template <bool STATS>
@@ -1788,14 +1852,14 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
     const uint32_t par = P.step & 1u;
     uint32_t base = 0u;
     if (n == 0u) {
-        base = atomicAdd(list_count(d, par, col % NLISTS), popc(voted));
+        base = atomicAdd(list_count(d, par, list_of(col)), popc(voted));
         reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
     }
     base = __shfl(base, (int)(32u * h));
     if (want != 0u) {
         const uint32_t at = base + popc(voted & ((1u << n) - 1u));
-        const size_t cap = (size_t)d.ncol / NLISTS * NSLOT;
-        d.tileList[(par * NLISTS + col % NLISTS) * cap + at] =
+        const size_t cap = (size_t)list_cols(d.ncol) * NSLOT;
+        d.tileList[(par * NLISTS + list_of(col)) * cap + at] =
             make_uint4(col * NSLOT + n, cn, want, fwb | (full ? 0x100u : 0u));
     }
 }
@@ -1943,7 +2007,7 @@ hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn,
                        hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
     uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    const dim3 g(blocks_for(d.ncol, 16)), b(1024);  // column top-K included
+    const dim3 g(blocks_for(d.ncol, 128) * 8u), b(1024);  // 16 columns per workgroup, XCD-aligned (emit_col)
     if (learn && stats) return launch(k_emit<true, true>, g, b, 0u, s, e0, e1, d, P, in, out);
     if (learn) return launch(k_emit<true, false>, g, b, 0u, s, e0, e1, d, P, in, out);
     if (stats) return launch(k_emit<false, true>, g, b, 0u, s, e0, e1, d, P, in, out);

@@ -21,6 +21,10 @@ VARIANTS = {
     "hetero_mode0_serialtopk": SMALL + CONFIG_C + ["hetero=1", "mode=0", "topkpar=0", "epochs=2", "ltdrep=2"],
     "mode2_biasevery": SMALL + CONFIG_C + ["mode=2", "dthr=3", "biasdownevery=3", "epochs=2", "seed=3"],
     "readout_knobs": SMALL + CONFIG_C + ["ktest=10", "facil=40", "epochs=2", "seed=1", "nand=3"],
+    # the column gate at its boundaries: target = the 384 recurrent columns (gate by counting), and
+    # beyond them (every workgroup runs the radix select)
+    "target_all_recurrent": SMALL + CONFIG_C + ["target=384", "epochs=1", "seed=2"],
+    "target_beyond_recurrent": SMALL + CONFIG_C + ["target=400", "epochs=1", "seed=2", "topkpar=0"],
 }
 
 

@@ -23,6 +23,34 @@ def trained():
     return net
 
 
+def test_snapshot_holds_the_oracles_state(tmp_path):
+    """A snapshot of the network after its training epochs, loaded back, holds
+    the planes, connectivity, bias and stimulus columns of the CPU oracle's
+    network after the same epochs; a recall on the loaded network is the
+    oracle's."""
+    import oracle
+
+    args = C0 + ["epochs=2"]
+    net = trained()
+    ora = oracle.OracleNet(oracle.parse(args), threads=4)
+    c = net.config
+    for ep in range(c.epochs):
+        for p in range(c.stim):
+            ora.present(p, 1.0, c.T, c.trainclamp, True, 1000 + p)
+    path = str(tmp_path / "net.axs")
+    net.save(path)
+    net.close()
+    back = axon_amd.Net.load(path)
+    for b in STATE:
+        assert np.array_equal(back.read(b), ora.buf(b)), b
+    for p in range(c.stim):
+        assert np.array_equal(back.stim_cols(p), ora.stim_cols(p)), p
+    r = back.present(3, 0.75, 35, 5, False, True, 7003, 0)
+    ora.present(3, 0.75, 35, 5, False, 7003, 0)
+    assert np.array_equal(r.hist.ravel(), ora.buf("HIST")[: 35 * c.ncol])
+    back.close()
+
+
 def test_save_load_round_trip(tmp_path):
     net = trained()
     path = str(tmp_path / "net.axs")
