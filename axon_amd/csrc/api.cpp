@@ -826,6 +826,22 @@ static const uint32_t kProbeTiles[AXON_PROBE_TILES] = {1, 2, 4, 8, 16, 32, 64};
 // waves per CU the probe propagation may keep resident, in the autotuner's order (0 = as many as fit)
 static const uint32_t kProbeWaves[AXON_PROBE_OCCUPANCIES] = {0, 32, 16};
 
+// The readout parameters of a probe batch on network n (present() with
+// learn = 0, mm:818-822), and its accumulator width: a neuron's accumulator
+// is at most 32 branches x (256 presynaptic bits + boost) in mode 1 (x 256 in
+// mode 0, x 1 in mode 2) plus the facilitation, before the bias is taken
+// off; when that bound is below 2^16 the probe propagation stores 16-bit
+// accumulators, which hold every value exactly.
+static void probe_params(const axon_net* n, axon::ProbeDev& d) {
+    d.k = n->kTest ? n->kTest : n->K;  // readout k (mm:818)
+    d.facil = n->facil;
+    d.dthr = n->P.dthr; d.boost = n->P.boost; d.mode = n->P.mode; d.biasShift = n->P.biasShift;
+    d.target = n->P.targetCols; d.topkpar = n->d.topkpar;
+    const uint64_t perBranch = d.mode == 0u ? 256u : d.mode == 1u ? 256ull + d.boost : 1u;
+    const uint64_t bound = (uint64_t)axon::NSLOT * perBranch + d.facil;
+    d.acc16 = bound <= 0xFFFFu ? 1u : 0u;
+}
+
 struct axon_probes {
     axon_net* net = nullptr;
     axon::ProbeDev d{};
@@ -860,10 +876,7 @@ static axon_status probes_create(axon_net* n, uint32_t slots, uint32_t max_t, ui
     d.W3 = n->d.W3; d.conn = n->d.conn; d.bias = n->d.bias;
     d.nslots = ns; d.ncol = n->NCOL; d.incols = n->cfg.incols; d.maxt = max_t;
     d.gw = (n->NCOL + 31) / 32;
-    d.k = n->kTest ? n->kTest : n->K;  // readout k (mm:818)
-    d.facil = n->facil;
-    d.dthr = n->P.dthr; d.boost = n->P.boost; d.mode = n->P.mode; d.biasShift = n->P.biasShift;
-    d.target = n->P.targetCols; d.topkpar = n->d.topkpar;
+    probe_params(n, d);
     struct { uint32_t** p; size_t words; } bufs[] = {
         {&d.spk, ns * NC}, {&d.gact, (size_t)ns * d.gw}, {&d.acc, ns * NN}, {&d.drive, ns * NC},
         {&d.stim, ns * NC}, {&d.hist, (size_t)ns * max_t * NC}, {&d.theta, (size_t)ns * 2},
@@ -1056,10 +1069,7 @@ extern "C" axon_status axon_probes_run_noalloc(axon_net* n, uint32_t count, cons
     d.hist = d_hist ? d_hist : take(per[10]);
     d.stim = const_cast<uint32_t*>(d_stims);  // read only: slots below count, during their clamped steps
     d.nslots = (uint32_t)ns; d.ncol = n->NCOL; d.incols = n->cfg.incols; d.maxt = T; d.gw = (uint32_t)gw;
-    d.k = n->kTest ? n->kTest : n->K;  // readout k (mm:818)
-    d.facil = n->facil;
-    d.dthr = n->P.dthr; d.boost = n->P.boost; d.mode = n->P.mode; d.biasShift = n->P.biasShift;
-    d.target = n->P.targetCols; d.topkpar = n->d.topkpar;
+    probe_params(n, d);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : n->stream;
     HIPCHECK(axon::launch_probe_setup(d, count, T, clamp_steps, use_bias, s));
     for (uint32_t t = 0; t < T; ++t) HIPCHECK(axon::launch_probe_step(d, tile, d.nslots, 0u, s));

@@ -101,6 +101,10 @@ struct ProbeDev {
     uint32_t* selDone;   // [slot] workgroups of a split select that finished this step (reset by the last)
     uint32_t nslots, ncol, incols, maxt, gw;
     uint32_t k, facil, dthr, boost, mode, biasShift, target, topkpar;
+    // 1: `acc` holds 16-bit accumulators ([slot][ncol*32] uint16), chosen by
+    // the host when no accumulator can reach 2^16 (probe_acc16), which halves
+    // the propagation's largest write; 0: 32-bit
+    uint32_t acc16, pad;
 };
 
 // Lock-step batch of independent networks (same dynamics parameters).

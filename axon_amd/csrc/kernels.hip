@@ -125,6 +125,9 @@ __device__ __forceinline__ uint32_t buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t of
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
     __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
 }
+__device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, off, 0, 0);
+}
 
 // 8-bit mask of the nonzero bytes of x (bit i = byte i).
 __device__ __forceinline__ uint32_t byte_mask(uint64_t x) {
@@ -759,10 +762,12 @@ __device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P
     }
 }
 
-// Every global value a column's emit can need is loaded up front (before
-// the column top-K that k_emit runs first), so the wave waits for memory once
-// rather than once per use (a single network's timestep is latency-bound;
-// emit_pair is the form for many networks).
+// A column's emit inputs.  Every column needs its drive, accumulators, clamp
+// word and bias (read before the column gate, which they overlap); only a
+// column that can spike (gate passed or input clamped) needs its NMDA bits
+// and, when learning, its plasticity inputs (branch groups, fired masks, band
+// flags, counters): those are requested right after the gate, so their round
+// trip overlaps the k-of-32 selection, and the other columns never read them.
 struct EmitIn { uint32_t dv, v, stimW, bbv, cn, fwb, band, bia; uint2 lc; };
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, uint32_t col) {
@@ -774,13 +779,22 @@ __device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, 
     in.dv = d.drive[col];
     in.v = d.acc[gid];
     in.stimW = clampCol ? d.stim[col] : 0u;
-    in.bbv = (LEARN || STATS) ? d.bbits[gid] : 0u;
-    in.cn = LEARN ? d.conn[col * NSLOT + n] : 0u;
-    in.fwb = LEARN ? d.firedW[col * NSLOT + n] : 0u;
-    in.band = LEARN ? d.tileBand[col * NSLOT + n] : 0u;
-    in.lc = LEARN ? reinterpret_cast<const uint2*>(d.cntLearn)[col] : make_uint2(0u, 0u);
+    in.bbv = 0u; in.cn = 0u; in.fwb = 0u; in.band = 0u;
+    in.lc = make_uint2(0u, 0u);
     in.bia = (LEARN && rec) ? d.bias[gid] : 0u;
     return in;
+}
+template <bool LEARN, bool STATS>
+__device__ __forceinline__ void emit_inputs_spiking(const NetDev& d, const Params& P, uint32_t col, EmitIn& in) {
+    const uint32_t n = lane_id() & 31u;
+    const uint32_t gid = col * NPC + n;
+    if (LEARN || STATS) in.bbv = d.bbits[gid];
+    if (LEARN && P.phase != 3u) {
+        in.cn = d.conn[col * NSLOT + n];
+        in.fwb = d.firedW[col * NSLOT + n];
+        in.band = d.tileBand[col * NSLOT + n];
+        in.lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
+    }
 }
 
 template <bool LEARN, bool STATS>
@@ -1026,7 +1040,7 @@ template <bool LEARN, bool STATS>
 __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
                                                uint32_t* __restrict__ spkOut) {
     const uint32_t col = emit_col(blockIdx.x, threadIdx.x >> 6);
-    const EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the gate
+    EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the gate
     const uint32_t nrec = d.ncol - P.inCols;
     bool gate;
     if (P.targetCols >= 1u && P.targetCols <= nrec && nrec <= 4096u) {  // uniform over the grid
@@ -1038,6 +1052,8 @@ __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_
                                              P.targetCols, d.topkpar);
         gate = in.dv >= th;
     }
+    if (col < d.ncol && (gate || (P.clamp != 0u && col < P.inCols)))  // wave-uniform: this column can spike
+        emit_inputs_spiking<LEARN, STATS>(d, P, col, in);
     emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, gate);
 }
 
@@ -1476,7 +1492,10 @@ __global__ __launch_bounds__(256, PT > 4 ? 4 : 8) void k_probe_items(ProbeDev d,
         uint32_t av[PT];
 #pragma unroll
         for (int p = 0; p < PT; ++p) av[p] = A[p * 32 + n];
-        const __amdgpu_buffer_rsrc_t ar = buf_rsrc(d.acc + (size_t)slot0 * d.ncol * NPC, nsl * d.ncol * NPC * 4u);
+        const uint32_t aw = d.acc16 ? 2u : 4u;  // bytes per accumulator
+        const __amdgpu_buffer_rsrc_t ar = buf_rsrc(reinterpret_cast<const uint8_t*>(d.acc) +
+                                                       (size_t)slot0 * d.ncol * NPC * aw,
+                                                   nsl * d.ncol * NPC * aw);
         const __amdgpu_buffer_rsrc_t dr = buf_rsrc(d.drive + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
 #pragma unroll
         for (int p = 0; p < PT; ++p) {
@@ -1484,7 +1503,9 @@ __global__ __launch_bounds__(256, PT > 4 ? 4 : 8) void k_probe_items(ProbeDev d,
             uint32_t a = av[p];
             if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
             if (((ubm >> p) & 1u) != 0u) a = (a > bi) ? (a - bi) : 0u;
-            buf_st(ar, hi ? SKIP : ((uint32_t)p * d.ncol * NPC + gid) * 4u, a);
+            const uint32_t ao = hi ? SKIP : ((uint32_t)p * d.ncol * NPC + gid) * aw;
+            if (d.acc16) buf_st16(ar, ao, a);
+            else buf_st(ar, ao, a);
             const uint32_t drv = half_scan(a);
             buf_st(dr, lane == 31u ? ((uint32_t)p * d.ncol + col) * 4u : SKIP, drv);
         }
@@ -1608,7 +1629,11 @@ __global__ __launch_bounds__(256) void k_probe_drive(ProbeDev d, uint32_t ncolgr
         uint32_t a = acc[p];
         if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
         if (ub[p] != 0u) a = (a > bi) ? (a - bi) : 0u;
-        if (!hi) d.acc[(size_t)sl * d.ncol * NPC + gid] = a;
+        const size_t ai = (size_t)sl * d.ncol * NPC + gid;
+        if (!hi) {
+            if (d.acc16) reinterpret_cast<uint16_t*>(d.acc)[ai] = (uint16_t)a;
+            else d.acc[ai] = a;
+        }
         const uint32_t drv = half_sum(a);
         if (lane == 0u) d.drive[(size_t)sl * d.ncol + col] = drv;
     }
@@ -1659,6 +1684,7 @@ __global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d, uint32_t part
     const uint32_t th = topk_block<1024>(d.drive + so, part == 0u ? d.theta + 2u * sl : nullptr, d.ncol, d.incols,
                                          d.target, d.topkpar);
     const uint32_t* __restrict__ acc = d.acc + so * NPC;
+    const uint16_t* __restrict__ acc16 = reinterpret_cast<const uint16_t*>(d.acc) + so * NPC;
     uint32_t* __restrict__ hist = d.hist + ((size_t)sl * d.maxt + t) * d.ncol;
     uint32_t* __restrict__ bw = bres[wave];
     for (uint32_t g0 = gFirst; g0 < cEnd; g0 += SEL_CHUNKS * 1024u) {
@@ -1695,8 +1721,10 @@ __global__ __launch_bounds__(1024) void k_probe_select(ProbeDev d, uint32_t part
             }
             uint32_t v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                v[q] = ci[q] < 256u ? acc[(size_t)(g0 + (ci[q] >> 6) * 1024u + (ci[q] & 63u)) * NPC + n] : 0u;
+            for (int q = 0; q < 4; ++q) {
+                const size_t ai = (size_t)(g0 + (ci[q] >> 6) * 1024u + (ci[q] & 63u)) * NPC + n;
+                v[q] = ci[q] < 256u ? (d.acc16 ? (uint32_t)acc16[ai] : acc[ai]) : 0u;
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const Kwta kw = kwta32(v[q], d.k, n, h);

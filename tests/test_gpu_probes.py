@@ -155,12 +155,14 @@ def test_autotuner_sweeps_tiles_and_occupancies():
         axon_amd.Probes(net, slots=8, max_t=32, tile=4, waves_per_cu=24)
 
 
-@pytest.mark.parametrize("mode_args", [["mode=0"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=2"]],
-                         ids=["mode0", "mode1-dthr2", "mode2"])
+@pytest.mark.parametrize("mode_args", [["mode=0"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=2"],
+                                       ["mode=1", "boost=3000"]],
+                         ids=["mode0", "mode1-dthr2", "mode2", "mode1-boost3000-acc32"])
 def test_tile_walks_and_dendritic_modes(mode_args):
     """130 probes at ncol=512: tiles 1 and 2 walk several tiles per wave (the last run ragged), tiles 4 and 8
     one; every mode of the branch output; all equal to the per-branch kernel (tile 16), to single
-    presentations and, for a sample, to the oracle."""
+    presentations and, for a sample, to the oracle.  A boost of 3000 lets accumulators pass 2^16, so the
+    probes keep 32-bit accumulators; the other cases store 16-bit ones."""
     args = C0 + mode_args
     net, ora = trained(args, 1)
     stims = cues(net, 130)
