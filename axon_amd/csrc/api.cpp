@@ -1357,13 +1357,25 @@ extern "C" axon_status axon_checksum(axon_net* n, axon_buffer which, uint64_t* s
 
 // ================================================================== serving
 // Continuous batching of recall queries over resident networks.  Each network
-// owns a probe batch (its slots); queued requests are admitted FIFO into free
-// slots while the HBM budget allows, every busy slot advances together, and
-// finished slots are gathered and freed at once, so later requests join
-// while earlier ones are still running.  A slot's computation depends only on
-// its own cue and the network, which is what makes a result the same alone
-// and in any mix.
+// owns a probe batch (its slots); queued requests are admitted in arrival
+// order into free slots while the HBM budget allows, every busy slot
+// advances together, and finished slots are gathered and freed at once, so
+// later requests join while earlier ones are still running.  A slot's
+// computation depends only on its own cue and the network, which is what
+// makes a result the same alone and in any mix.
+//
+// Ticks are pipelined two deep.  The host keeps each slot's step count
+// itself (every busy slot advances one step per step launched), so it knows
+// which queries a tick finishes without reading counters back: tick n
+// enqueues, on each network's stream, its admissions, its steps, the gather
+// of the slots it finishes and the copy of their results, and only then
+// waits for tick n - 1's results.  The device is never left idle while the
+// host admits and collects.  A finished slot is free for the next tick's
+// admissions at once (its gather precedes them on the stream); its bytes
+// leave the budget then too.  Host buffers alternate between ticks.
 #include <deque>
+#include <queue>
+#include <set>
 #include <unordered_map>
 
 namespace {
@@ -1377,7 +1389,7 @@ struct SrvReq {
 struct SrvSlot {
     bool busy = false;
     uint64_t id = 0, bytes = 0;
-    uint32_t T = 0;
+    uint32_t T = 0, done = 0;  // steps to run, steps launched
     double arrival = 0;
 };
 struct SrvResult {
@@ -1386,13 +1398,26 @@ struct SrvResult {
     std::vector<uint32_t> spikes;
     std::vector<uint64_t> digest;
 };
+struct SrvFinished {
+    uint64_t id;
+    uint32_t T, off;
+    double arrival;
+};
+// one tick's host and device buffers of a network (two alternate)
+struct SrvTick {
+    uint32_t *h_list = nullptr, *d_list = nullptr;  // admit entries
+    uint32_t *h_g = nullptr, *d_g = nullptr;        // gather slots + offsets
+    uint32_t *h_out = nullptr, *d_out = nullptr;    // gathered results
+    hipEvent_t ev = nullptr;
+    bool pending = false;                           // work enqueued by this tick not yet waited for
+    std::vector<SrvFinished> fin;
+};
 struct SrvNet {
     axon_probes* pb = nullptr;
     std::vector<SrvSlot> slots;
-    uint32_t *h_list = nullptr, *d_list = nullptr;    // admit entries
-    uint32_t* h_t = nullptr;                          // slot step counters
-    uint32_t *h_g = nullptr, *d_g = nullptr;          // gather slots + offsets
-    uint32_t *h_out = nullptr, *d_out = nullptr;      // gathered results
+    std::set<uint32_t> free;  // free slots, taken lowest first
+    std::deque<SrvReq> queue;
+    SrvTick tk[2];
 };
 }  // namespace
 
@@ -1400,11 +1425,12 @@ struct axon_server {
     std::vector<axon_net*> nets;
     std::vector<SrvNet> sn;
     axon_server_config cfg{};
-    std::deque<SrvReq> queue;
     std::unordered_map<uint64_t, SrvResult> done;
-    std::vector<uint64_t> last_done;  // ids finished by the last tick
+    std::vector<uint64_t> last_done;  // ids the last tick collected
     uint64_t next_id = 1, reserved = 0;
-    uint32_t ncol = 0, inflight = 0;
+    // inflight: admitted and not yet collected; reserved: bytes of the
+    // queries still running (a finished query's leave at its gather)
+    uint32_t ncol = 0, inflight = 0, queued = 0, cur = 0;
     std::chrono::steady_clock::time_point t0;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -1419,10 +1445,13 @@ extern "C" void axon_server_destroy(axon_server* s) {
     for (size_t i = 0; i < s->sn.size(); ++i) {
         SrvNet& v = s->sn[i];
         if (s->nets[i]) (void)hipStreamSynchronize(s->nets[i]->stream);
-        for (uint32_t* p : {v.h_list, v.h_t, v.h_g, v.h_out})
-            if (p) (void)hipHostFree(p);
-        for (uint32_t* p : {v.d_list, v.d_g, v.d_out})
-            if (p) (void)hipFree(p);
+        for (SrvTick& t : v.tk) {
+            for (uint32_t* p : {t.h_list, t.h_g, t.h_out})
+                if (p) (void)hipHostFree(p);
+            for (uint32_t* p : {t.d_list, t.d_g, t.d_out})
+                if (p) (void)hipFree(p);
+            if (t.ev) (void)hipEventDestroy(t.ev);
+        }
         axon_probes_destroy(v.pb);
     }
     delete s;
@@ -1447,15 +1476,19 @@ static axon_status server_create(axon_net* const* nets, uint32_t nnet, const axo
         if (st != AXON_OK) return bail(st);
         const uint32_t ns = v.pb->d.nslots;
         v.slots.resize(ns);
+        for (uint32_t k = 0; k < ns; ++k) v.free.insert(k);
         const size_t listW = (size_t)ns * (4 + s->ncol), outW = (size_t)ns * (s->ncol + 2 * cfg->max_t);
-        if (hipHostMalloc((void**)&v.h_list, listW * 4, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&v.h_t, (size_t)ns * 4, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&v.h_g, (size_t)ns * 8, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&v.h_out, outW * 4, hipHostMallocDefault) != hipSuccess)
-            return bail(fail(AXON_ERR_OUT_OF_MEMORY, "pinned host buffers"));
-        if (hipMalloc((void**)&v.d_list, listW * 4) != hipSuccess || hipMalloc((void**)&v.d_g, (size_t)ns * 8) != hipSuccess ||
-            hipMalloc((void**)&v.d_out, outW * 4) != hipSuccess)
-            return bail(fail(AXON_ERR_OUT_OF_MEMORY, "server device buffers"));
+        for (SrvTick& t : v.tk) {
+            if (hipHostMalloc((void**)&t.h_list, listW * 4, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc((void**)&t.h_g, (size_t)ns * 8, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc((void**)&t.h_out, outW * 4, hipHostMallocDefault) != hipSuccess)
+                return bail(fail(AXON_ERR_OUT_OF_MEMORY, "pinned host buffers"));
+            if (hipMalloc((void**)&t.d_list, listW * 4) != hipSuccess || hipMalloc((void**)&t.d_g, (size_t)ns * 8) != hipSuccess ||
+                hipMalloc((void**)&t.d_out, outW * 4) != hipSuccess)
+                return bail(fail(AXON_ERR_OUT_OF_MEMORY, "server device buffers"));
+            if (hipEventCreateWithFlags(&t.ev, hipEventDisableTiming) != hipSuccess)
+                return bail(fail(AXON_ERR_DEVICE, "hipEventCreate failed"));
+        }
     }
     s->t0 = std::chrono::steady_clock::now();
     *out = s;
@@ -1480,93 +1513,121 @@ static axon_status server_submit(axon_server* s, uint32_t net, const uint32_t* s
     r.arrival = arrival_s >= 0 ? arrival_s : s->now();
     r.bytes = bytes;
     r.stim.assign(stim, stim + s->ncol);
-    s->queue.push_back(std::move(r));
-    *id = s->queue.back().id;
+    s->sn[net].queue.push_back(std::move(r));
+    s->queued++;
+    *id = s->next_id - 1;
+    return AXON_OK;
+}
+
+// Wait for a tick's work, then record the results it copied back.
+static axon_status server_collect(axon_server* s, SrvTick& t, uint32_t& ndone) {
+    if (!t.pending) return AXON_OK;
+    HIPCHECK(hipEventSynchronize(t.ev));
+    t.pending = false;
+    const uint32_t NC = s->ncol;
+    const double tdone = s->now();
+    for (const SrvFinished& f : t.fin) {
+        const uint32_t* o = t.h_out + f.off;
+        SrvResult r;
+        r.T = f.T; r.arrival = f.arrival; r.done = tdone;
+        r.spikes.assign(o, o + NC);
+        r.digest.resize(f.T);
+        for (uint32_t k = 0; k < f.T; ++k) r.digest[k] = (uint64_t)o[NC + 2 * k] | ((uint64_t)o[NC + 2 * k + 1] << 32);
+        s->done.emplace(f.id, std::move(r));
+        s->last_done.push_back(f.id);
+        s->inflight--;
+        ++ndone;
+    }
+    t.fin.clear();
     return AXON_OK;
 }
 
 static axon_status server_tick(axon_server* s, uint32_t* completed) {
     if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
-    const uint32_t nnet = (uint32_t)s->nets.size(), NC = s->ncol;
+    const uint32_t nnet = (uint32_t)s->nets.size(), NC = s->ncol, spt = s->cfg.steps_per_tick;
     HIPCHECK(hipSetDevice(s->nets[0]->device));
-    // 1. admission: FIFO; a request waits for a free slot on its network, and
-    //    admission stops at the first request the HBM budget cannot take
+    const uint32_t cur = s->cur, prev = cur ^ 1u;
+    // 1. admission in arrival order over the networks' queues: the earliest
+    //    request whose network has a free slot goes next; admission stops at
+    //    the first such request the HBM budget cannot take
     std::vector<uint32_t> nadm(nnet, 0);
-    for (auto it = s->queue.begin(); it != s->queue.end();) {
-        SrvNet& v = s->sn[it->net];
-        if (s->cfg.hbm_budget && s->reserved + it->bytes > s->cfg.hbm_budget) break;
-        uint32_t slot = UINT32_MAX;
-        for (uint32_t k = 0; k < v.slots.size(); ++k)
-            if (!v.slots[k].busy) { slot = k; break; }
-        if (slot == UINT32_MAX) { ++it; continue; }
-        uint32_t* e = v.h_list + (size_t)nadm[it->net]++ * (4 + NC);
-        e[0] = slot; e[1] = it->T; e[2] = it->clamp; e[3] = it->bias;
-        memcpy(e + 4, it->stim.data(), (size_t)NC * 4);
-        v.slots[slot] = SrvSlot{true, it->id, it->bytes, it->T, it->arrival};
-        s->reserved += it->bytes;
-        s->inflight++;
-        it = s->queue.erase(it);
+    {
+        using Head = std::pair<uint64_t, uint32_t>;  // (request id, network)
+        std::priority_queue<Head, std::vector<Head>, std::greater<Head>> heads;
+        for (uint32_t i = 0; i < nnet; ++i)
+            if (!s->sn[i].queue.empty() && !s->sn[i].free.empty()) heads.emplace(s->sn[i].queue.front().id, i);
+        while (!heads.empty()) {
+            const uint32_t i = heads.top().second;
+            heads.pop();
+            SrvNet& v = s->sn[i];
+            SrvReq& r = v.queue.front();
+            if (s->cfg.hbm_budget && s->reserved + r.bytes > s->cfg.hbm_budget) break;
+            const uint32_t slot = *v.free.begin();
+            v.free.erase(v.free.begin());
+            uint32_t* e = v.tk[cur].h_list + (size_t)nadm[i]++ * (4 + NC);
+            e[0] = slot; e[1] = r.T; e[2] = r.clamp; e[3] = r.bias;
+            memcpy(e + 4, r.stim.data(), (size_t)NC * 4);
+            v.slots[slot] = SrvSlot{true, r.id, r.bytes, r.T, 0u, r.arrival};
+            s->reserved += r.bytes;
+            s->inflight++;
+            s->queued--;
+            v.queue.pop_front();
+            if (!v.queue.empty() && !v.free.empty()) heads.emplace(v.queue.front().id, i);
+        }
     }
-    // 2. enqueue per network: admissions, steps, step-counter read-back
+    // 2. per network, on its stream: admissions, steps, gather of the slots
+    //    these steps finish, the copy of their results, an event
     for (uint32_t i = 0; i < nnet; ++i) {
         SrvNet& v = s->sn[i];
+        SrvTick& t = v.tk[cur];
         hipStream_t st = s->nets[i]->stream;
         if (nadm[i]) {
-            HIPCHECK(hipMemcpyAsync(v.d_list, v.h_list, (size_t)nadm[i] * (4 + NC) * 4, hipMemcpyHostToDevice, st));
-            HIPCHECK(axon::launch_probe_admit(v.pb->d, v.d_list, nadm[i], st));
+            HIPCHECK(hipMemcpyAsync(t.d_list, t.h_list, (size_t)nadm[i] * (4 + NC) * 4, hipMemcpyHostToDevice, st));
+            HIPCHECK(axon::launch_probe_admit(v.pb->d, t.d_list, nadm[i], st));
         }
         uint32_t hi = 0;  // slots are taken lowest-first: launches cover [0, last busy slot]
         for (uint32_t k = 0; k < v.slots.size(); ++k)
             if (v.slots[k].busy) hi = k + 1u;
-        if (!hi) continue;
-        axon_status r = probes_advance(v.pb, s->cfg.steps_per_tick, hi);
-        if (r != AXON_OK) return r;
-        HIPCHECK(hipMemcpyAsync(v.h_t, v.pb->d.st_t, v.slots.size() * 4, hipMemcpyDeviceToHost, st));
-    }
-    for (uint32_t i = 0; i < nnet; ++i) HIPCHECK(hipStreamSynchronize(s->nets[i]->stream));
-    // 3. gather the slots that reached their T
-    std::vector<std::vector<uint32_t>> fin(nnet);
-    for (uint32_t i = 0; i < nnet; ++i) {
-        SrvNet& v = s->sn[i];
-        hipStream_t st = s->nets[i]->stream;
-        uint32_t off = 0;
-        for (uint32_t k = 0; k < v.slots.size(); ++k)
-            if (v.slots[k].busy && v.h_t[k] >= v.slots[k].T) {
-                const uint32_t j = (uint32_t)fin[i].size();
-                fin[i].push_back(k);
-                v.h_g[j] = k;
-                v.h_g[v.slots.size() + j] = off;
-                off += NC + 2 * v.slots[k].T;
+        if (hi) {
+            axon_status r = probes_advance(v.pb, spt, hi);
+            if (r != AXON_OK) return r;
+            const uint32_t ns = (uint32_t)v.slots.size();
+            uint32_t off = 0, nf = 0;
+            for (uint32_t k = 0; k < hi; ++k) {
+                SrvSlot& sl = v.slots[k];
+                if (!sl.busy) continue;
+                sl.done = std::min(sl.T, sl.done + spt);
+                if (sl.done < sl.T) continue;
+                t.h_g[nf] = k;
+                t.h_g[ns + nf] = off;
+                t.fin.push_back(SrvFinished{sl.id, sl.T, off, sl.arrival});
+                off += NC + 2 * sl.T;
+                ++nf;
+                s->reserved -= sl.bytes;
+                sl = SrvSlot{};
+                v.free.insert(k);
             }
-        const uint32_t nf = (uint32_t)fin[i].size();
-        if (!nf) continue;
-        HIPCHECK(hipMemcpyAsync(v.d_g, v.h_g, v.slots.size() * 8, hipMemcpyHostToDevice, st));
-        HIPCHECK(axon::launch_probe_gather(v.pb->d, v.d_g, v.d_g + v.slots.size(), nf, v.d_out, st));
-        HIPCHECK(hipMemcpyAsync(v.h_out, v.d_out, (size_t)off * 4, hipMemcpyDeviceToHost, st));
+            if (nf) {
+                HIPCHECK(hipMemcpyAsync(t.d_g, t.h_g, (size_t)ns * 8, hipMemcpyHostToDevice, st));
+                HIPCHECK(axon::launch_probe_gather(v.pb->d, t.d_g, t.d_g + ns, nf, t.d_out, st));
+                HIPCHECK(hipMemcpyAsync(t.h_out, t.d_out, (size_t)off * 4, hipMemcpyDeviceToHost, st));
+            }
+        }
+        // the event also guards this tick's host buffers, which the tick
+        // after next fills again (the copies from them are asynchronous)
+        if (nadm[i] || hi) {
+            HIPCHECK(hipEventRecord(t.ev, st));
+            t.pending = true;
+        }
     }
+    // 3. collect the previous tick's results while this tick runs
     uint32_t ndone = 0;
     s->last_done.clear();
     for (uint32_t i = 0; i < nnet; ++i) {
-        if (fin[i].empty()) continue;
-        SrvNet& v = s->sn[i];
-        HIPCHECK(hipStreamSynchronize(s->nets[i]->stream));
-        const double tdone = s->now();
-        for (uint32_t j = 0; j < fin[i].size(); ++j) {
-            SrvSlot& sl = v.slots[fin[i][j]];
-            const uint32_t* o = v.h_out + v.h_g[v.slots.size() + j];
-            SrvResult r;
-            r.T = sl.T; r.arrival = sl.arrival; r.done = tdone;
-            r.spikes.assign(o, o + NC);
-            r.digest.resize(sl.T);
-            for (uint32_t t = 0; t < sl.T; ++t) r.digest[t] = (uint64_t)o[NC + 2 * t] | ((uint64_t)o[NC + 2 * t + 1] << 32);
-            s->done.emplace(sl.id, std::move(r));
-            s->last_done.push_back(sl.id);
-            s->reserved -= sl.bytes;
-            s->inflight--;
-            sl = SrvSlot{};
-            ++ndone;
-        }
+        axon_status r = server_collect(s, s->sn[i].tk[prev], ndone);
+        if (r != AXON_OK) return r;
     }
+    s->cur = prev;
     if (completed) *completed = ndone;
     return AXON_OK;
 }
@@ -1588,7 +1649,7 @@ extern "C" axon_status axon_server_result(axon_server* s, uint64_t id, uint32_t*
 extern "C" axon_status axon_server_status(const axon_server* s, uint32_t* queued, uint32_t* inflight,
                                           uint64_t* reserved_bytes) {
     if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
-    if (queued) *queued = (uint32_t)s->queue.size();
+    if (queued) *queued = s->queued;
     if (inflight) *inflight = s->inflight;
     if (reserved_bytes) *reserved_bytes = s->reserved;
     return AXON_OK;

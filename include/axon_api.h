@@ -271,10 +271,15 @@ axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t reps, double*
 /* ---- serving: continuous batching of recall queries ----
  * Extension with no reference counterpart.  Trained networks stay resident;
  * each query is one readout presentation (a probe) of T steps on one network.
- * axon_server_tick admits queued queries FIFO into free slots while the sum of
- * in-flight footprints stays within hbm_budget (0 = unlimited), advances every
- * in-flight query by steps_per_tick steps (networks on separate streams, so
- * their work overlaps), and collects the finished ones.  A result is the final
+ * axon_server_tick admits queued queries in arrival order into free slots
+ * while the sum of running footprints stays within hbm_budget (0 = unlimited),
+ * advances every running query by steps_per_tick steps (networks on separate
+ * streams, so their work overlaps), gathers the queries those steps finish,
+ * and collects the results the previous tick gathered: ticks are pipelined
+ * two deep, so a query is returned by the tick after the one that finished
+ * it, and the host never waits for the tick it has just enqueued.  `inflight`
+ * (axon_server_status) counts queries admitted and not yet returned; tick
+ * until it and `queued` are both 0 to drain the server.  A result is the final
  * spike ballots plus one 64-bit checksum per step's ballot row (the
  * axon_checksum formula over that row), bit-identical to the same query run
  * alone.  Latency is completion time minus the arrival time given at submit
@@ -296,7 +301,7 @@ double axon_server_clock(const axon_server* s);
 axon_status axon_server_submit(axon_server* s, uint32_t net, const uint32_t* stim, uint32_t T, uint32_t clamp_steps,
                                uint32_t use_bias, double arrival_s, uint64_t* id);
 axon_status axon_server_tick(axon_server* s, uint32_t* completed);
-/* ids of the queries the last tick finished (up to max; *n = how many finished) */
+/* ids of the queries the last tick returned (up to max; *n = how many) */
 axon_status axon_server_completed(const axon_server* s, uint64_t* ids, uint32_t max, uint32_t* n);
 /* spikes NCOL words, row_digest T words; AXON_ERR_NOT_READY until finished */
 axon_status axon_server_result(axon_server* s, uint64_t id, uint32_t* spikes, uint64_t* row_digest,
