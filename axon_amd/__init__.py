@@ -475,6 +475,12 @@ class Server:
         _check(st)
         return spk, dig[: T.value], lat.value, T.value
 
+    def config(self) -> dict:
+        """The configuration in use (tile and occupancy as chosen for tile=0)."""
+        c = ServerConfig()
+        _check(lib().axon_server_get_config(self._h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in c._fields_ if k != "reserved"}
+
     def status(self):
         q, f, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
         _check(lib().axon_server_status(self._h, ctypes.byref(q), ctypes.byref(f), ctypes.byref(r)))
@@ -519,6 +525,59 @@ def probe_autotune(net: Net, stims: np.ndarray, T: int, clamp_steps: int, use_bi
     return best.value, waves.value, table
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("ncol", ctypes.c_uint32), ("probes", ctypes.c_uint32), ("T", ctypes.c_uint32),
+                ("tile", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("reps", ctypes.c_uint32),
+                ("ms", ctypes.c_double), ("spread_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def probe_autotune_sizes(net: Net, stims: np.ndarray, T: int, clamp_steps: int, sizes: Sequence[int],
+                         use_bias: int = 0) -> list:
+    """The autotuner for the first `size` stimuli, per size: one tuning-table
+    entry each (as dicts)."""
+    stims = np.ascontiguousarray(stims, np.uint32).reshape(-1, net.ncol)
+    sz = np.ascontiguousarray(sizes, np.uint32)
+    out = (TuneEntry * len(sz))()
+    _check(lib().axon_probe_autotune_sizes(net._h, ctypes.c_uint32(stims.shape[0]), _ptr(stims), ctypes.c_uint32(T),
+                                           ctypes.c_uint32(clamp_steps), ctypes.c_uint32(use_bias), _ptr(sz),
+                                           ctypes.c_uint32(len(sz)), out))
+    return [e.as_dict() for e in out]
+
+
+def tune_lookup(ncol: int, probes: int, T: int) -> Optional[dict]:
+    """The tuning-table entry nearest to (probes, T) for ncol, None if none."""
+    e = TuneEntry()
+    st = lib().axon_tune_lookup(ctypes.c_uint32(ncol), ctypes.c_uint32(probes), ctypes.c_uint32(T), ctypes.byref(e))
+    if st == 7:  # AXON_ERR_NOT_READY
+        return None
+    _check(st)
+    return e.as_dict()
+
+
+def tune_entries() -> list:
+    L = lib()
+    L.axon_tune_entries.restype = ctypes.c_uint32
+    n = L.axon_tune_entries(None, ctypes.c_uint32(0))
+    out = (TuneEntry * max(n, 1))()
+    n = L.axon_tune_entries(out, ctypes.c_uint32(n))
+    return [out[i].as_dict() for i in range(n)]
+
+
+def tune_clear() -> None:
+    lib().axon_tune_clear()
+
+
+def tune_save(path: str) -> None:
+    _check(lib().axon_tune_save(os.fsencode(path)))
+
+
+def tune_load(path: str) -> None:
+    _check(lib().axon_tune_load(os.fsencode(path)))
+
+
 def run(args: Sequence[str], trained: Optional[Net] = None, report_path: Optional[str] = None) -> None:
     """axon_run: the reference program's whole run (main(), mm:669-1911) with
     these key=value arguments; `trained` replaces training (its epochs done,
@@ -535,6 +594,17 @@ def mem_probe(kind: int, nbytes: int, reps: int = 5) -> float:
     _check(lib().axon_mem_probe(ctypes.c_uint32(kind), ctypes.c_uint64(nbytes), ctypes.c_uint32(reps),
                                 ctypes.byref(g)))
     return g.value
+
+
+def prop_probe(net: Net, spikes: np.ndarray, reps: int = 20):
+    """(microseconds per propagation launch, W3 lines one launch reads) with
+    `spikes` (ncol words) as the presynaptic activity (axon_prop_probe)."""
+    spikes = np.ascontiguousarray(spikes, np.uint32)
+    if spikes.shape != (net.ncol,):
+        raise ValueError(f"spikes must hold {net.ncol} words")
+    us, lines = ctypes.c_double(), ctypes.c_uint64()
+    _check(lib().axon_prop_probe(net._h, _ptr(spikes), ctypes.c_uint32(reps), ctypes.byref(us), ctypes.byref(lines)))
+    return us.value, lines.value
 
 
 def probe_cue(net: Net, pattern: int, nreplace: int, seed: int) -> np.ndarray:

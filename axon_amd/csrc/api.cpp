@@ -14,8 +14,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cerrno>
 #include <random>
 #include <string>
@@ -555,6 +557,15 @@ extern "C" axon_status axon_buffer_words(const axon_net* n, axon_buffer w, size_
 static bool is_plane(axon_buffer w) {
     return w == AXON_BUF_W0 || w == AXON_BUF_W1 || w == AXON_BUF_W2 || w == AXON_BUF_W3;
 }
+// Host-to-device copies and fills of a network's buffers go on its stream:
+// the stream is non-blocking, so work on the null stream is not ordered
+// before its kernels, and a fill issued there can land after a later kernel.
+static hipError_t put(axon_net* n, void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, n->stream);
+    return e == hipSuccess ? hipStreamSynchronize(n->stream) : e;  // src may go away on return
+}
+static hipError_t zero(axon_net* n, void* dst, size_t bytes) { return hipMemsetAsync(dst, 0, bytes, n->stream); }
+
 static inline size_t ref_to_dev_word(size_t i) {  // index within the plane
     const size_t tile = i >> 8, r = i & 255u;
     const size_t h = r >> 7, nn = (r >> 2) & 31u, k = r & 3u;
@@ -590,7 +601,7 @@ extern "C" axon_status axon_write_buffer(axon_net* n, axon_buffer w, const void*
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
     if (!is_plane(w)) {
-        HIPCHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        HIPCHECK(put(n, p, src, bytes));
         return AXON_OK;
     }
     // read-modify-write of the covered tiles, then forget which tiles were known to lie in the band
@@ -600,8 +611,8 @@ extern "C" axon_status axon_write_buffer(axon_net* n, axon_buffer w, const void*
     for (size_t i = 0; i < ntw; ++i) rv[i] = dv[ref_to_dev_word(i)];
     memcpy(rv.data(), src, bytes);
     for (size_t i = 0; i < ntw; ++i) dv[ref_to_dev_word(i)] = rv[i];
-    HIPCHECK(hipMemcpy(p, dv.data(), ntw * 4, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(n->d.tileBand, 0, (size_t)n->NCOL * axon::NSLOT));
+    HIPCHECK(put(n, p, dv.data(), ntw * 4));
+    HIPCHECK(zero(n, n->d.tileBand, (size_t)n->NCOL * axon::NSLOT));
     return AXON_OK;
 }
 
@@ -634,7 +645,7 @@ static axon_status counter_pairs(axon_net* n, uint32_t* dev, uint64_t out[2], ui
     HIPCHECK(hipMemcpy(v.data(), dev, v.size() * 4, hipMemcpyDeviceToHost));
     out[0] = 0; out[1] = 0;
     for (size_t i = 0; i < v.size(); ++i) out[i & 1u] += v[i];
-    if (reset) HIPCHECK(hipMemset(dev, 0, v.size() * 4));
+    if (reset) HIPCHECK(zero(n, dev, v.size() * 4));
     return AXON_OK;
 }
 
@@ -648,8 +659,8 @@ extern "C" axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t
     if (st != AXON_OK) return st;
     out[0] = p[0]; out[1] = l[0];
     if (reset) {
-        HIPCHECK(hipMemset(n->d.cntProp, 0, (size_t)n->NCOL * 8));
-        HIPCHECK(hipMemset(n->d.cntLearn, 0, (size_t)n->NCOL * 8));
+        HIPCHECK(zero(n, n->d.cntProp, (size_t)n->NCOL * 8));
+        HIPCHECK(zero(n, n->d.cntLearn, (size_t)n->NCOL * 8));
     }
     return AXON_OK;
 }
@@ -664,8 +675,8 @@ extern "C" axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t
     if (st != AXON_OK) return st;
     out[0] = p[1]; out[1] = l[1];
     if (reset) {
-        HIPCHECK(hipMemset(n->d.cntProp, 0, (size_t)n->NCOL * 8));
-        HIPCHECK(hipMemset(n->d.cntLearn, 0, (size_t)n->NCOL * 8));
+        HIPCHECK(zero(n, n->d.cntProp, (size_t)n->NCOL * 8));
+        HIPCHECK(zero(n, n->d.cntLearn, (size_t)n->NCOL * 8));
     }
     return AXON_OK;
 }
@@ -1200,6 +1211,7 @@ static axon_status multi_create(axon_net* const* nets, uint32_t nnet, axon_multi
             return bail(fail(AXON_ERR_DEVICE, "stim table copy"));
         b->tabs[i] = (uint64_t)(uintptr_t)p;
     }
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(AXON_ERR_DEVICE, "stim table copy"));
     *out = b;
     return AXON_OK;
 }
@@ -1320,6 +1332,12 @@ static axon_status multi_train(axon_multi* b, const uint32_t* epochs, uint32_t e
             return fail(AXON_ERR_DEVICE, "multi capture: %s / %s", hipGetErrorString(e), hipGetErrorString(e2));
         }
     }
+    // the descriptor uploads and the round counters' fills above went through
+    // the null stream, which the groups' non-blocking streams do not wait for
+    if (hipDeviceSynchronize() != hipSuccess) {
+        release();
+        return fail(AXON_ERR_DEVICE, "multi train: upload");
+    }
     const auto t0 = std::chrono::steady_clock::now();
     hipError_t e = hipSuccess;
     for (size_t r = 0; r < maxRounds && e == hipSuccess; ++r)
@@ -1435,6 +1453,8 @@ struct axon_server {
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
+static bool tune_find(uint32_t ncol, uint32_t probes, uint32_t T, axon_tune_entry* out);
+
 static uint64_t request_bytes(const axon_server* s, uint32_t T) {
     const uint64_t NC = s->ncol;
     return (uint64_t)T * NC * 4 + NC * 32 * 4 + 4 * NC * 4;  // history + accumulators + spikes/drive/stim/gact
@@ -1459,25 +1479,32 @@ extern "C" void axon_server_destroy(axon_server* s) {
 
 static axon_status server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
                                  axon_server** out) {
-    if (!nets || !cfg || !out || nnet == 0) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (!nets || !nets[0] || !cfg || !out || nnet == 0) return fail(AXON_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     if (cfg->slots_per_net == 0 || cfg->steps_per_tick == 0) return fail(AXON_ERR_INVALID_ARG, "slots and steps_per_tick >= 1");
     axon_server* s = new axon_server();
     s->cfg = *cfg;
     s->ncol = nets[0]->NCOL;
+    if (cfg->tile == 0) {  // the tuning table's choice for this batch, else tile 4 at 32 waves per CU
+        axon_tune_entry e{};
+        const bool hit = tune_find(s->ncol, cfg->slots_per_net, cfg->max_t, &e);
+        s->cfg.tile = hit ? e.tile : 4u;
+        s->cfg.waves_per_cu = hit ? e.waves_per_cu : 32u;
+    }
+    const axon_server_config& C = s->cfg;
     s->nets.assign(nets, nets + nnet);
     s->sn.resize(nnet);
     auto bail = [&](axon_status st) { axon_server_destroy(s); return st; };
     for (uint32_t i = 0; i < nnet; ++i) {
         if (!nets[i] || nets[i]->NCOL != s->ncol) return bail(fail(AXON_ERR_INVALID_ARG, "network %u missing or of another size", i));
         SrvNet& v = s->sn[i];
-        axon_status st = axon_probes_create(nets[i], cfg->slots_per_net, cfg->max_t, cfg->tile, &v.pb);
-        if (st == AXON_OK) st = axon_probes_set_occupancy(v.pb, cfg->waves_per_cu);
+        axon_status st = axon_probes_create(nets[i], C.slots_per_net, C.max_t, C.tile, &v.pb);
+        if (st == AXON_OK) st = axon_probes_set_occupancy(v.pb, C.waves_per_cu);
         if (st != AXON_OK) return bail(st);
         const uint32_t ns = v.pb->d.nslots;
         v.slots.resize(ns);
         for (uint32_t k = 0; k < ns; ++k) v.free.insert(k);
-        const size_t listW = (size_t)ns * (4 + s->ncol), outW = (size_t)ns * (s->ncol + 2 * cfg->max_t);
+        const size_t listW = (size_t)ns * (4 + s->ncol), outW = (size_t)ns * (s->ncol + 2 * C.max_t);
         for (SrvTick& t : v.tk) {
             if (hipHostMalloc((void**)&t.h_list, listW * 4, hipHostMallocDefault) != hipSuccess ||
                 hipHostMalloc((void**)&t.h_g, (size_t)ns * 8, hipHostMallocDefault) != hipSuccess ||
@@ -1646,6 +1673,12 @@ extern "C" axon_status axon_server_result(axon_server* s, uint64_t id, uint32_t*
     return AXON_OK;
 }
 
+extern "C" axon_status axon_server_get_config(const axon_server* s, axon_server_config* cfg) {
+    if (!s || !cfg) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    *cfg = s->cfg;
+    return AXON_OK;
+}
+
 extern "C" axon_status axon_server_status(const axon_server* s, uint32_t* queued, uint32_t* inflight,
                                           uint64_t* reserved_bytes) {
     if (!s) return fail(AXON_ERR_INVALID_ARG, "null server");
@@ -1663,56 +1696,207 @@ extern "C" axon_status axon_server_status(const axon_server* s, uint32_t* queued
 // (branch, word, probe) item; larger tiles walk the union of the probes'
 // active branches and read each W3 line once for all of them.  Which wins
 // depends on how much the probes' activity overlaps and on the number of
-// probes, so it is measured, not assumed.
-extern "C" axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
-                                           uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
-                                           uint32_t* best_waves_per_cu,
-                                           double ms[AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES]) {
-    if (!n || !stims || !best_tile || !best_waves_per_cu) return fail(AXON_ERR_INVALID_ARG, "null argument");
-    const uint32_t* tiles = kProbeTiles;
-    double best = 1e30;
-    *best_tile = 4;
-    *best_waves_per_cu = 0;
+// probes, so it is measured, not assumed.  Each point is the median of
+// kTuneReps timed runs after one untimed run (which also captures the
+// step graph), and every result goes into the process's tuning table.
+
+static constexpr int kTuneReps = 3;
+
+// The tuning table: the best (tile, occupancy) found per (ncol, probes, T).
+static std::mutex g_tune_mu;
+static std::map<std::tuple<uint32_t, uint32_t, uint32_t>, axon_tune_entry> g_tune;
+
+static void tune_record(const axon_tune_entry& e) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[std::make_tuple(e.ncol, e.probes, e.T)] = e;
+}
+
+// Times one (tile, occupancy) point on the first `count` stimuli: median and
+// spread (max - min) of kTuneReps runs of T steps.
+static axon_status tune_point(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T, uint32_t clamp_steps,
+                              uint32_t use_bias, uint32_t tile, uint32_t waves, hipEvent_t e0, hipEvent_t e1,
+                              double* med, double* spread) {
+    axon_probes* b = nullptr;
+    axon_status st = axon_probes_create(n, count, T, tile, &b);
+    if (st != AXON_OK) return st;
+    if ((st = axon_probes_set_occupancy(b, waves)) != AXON_OK) {
+        axon_probes_destroy(b);
+        return st;
+    }
+    std::vector<uint32_t> Ts(b->d.nslots, 0);
+    std::fill(Ts.begin(), Ts.begin() + count, T);
+    std::vector<uint32_t> pad((size_t)(b->d.nslots - count) * n->NCOL, 0);
+    std::vector<double> t;
+    for (int rep = 0; rep <= kTuneReps && st == AXON_OK; ++rep) {
+        st = axon_probes_load(b, 0, count, stims, Ts.data(), clamp_steps, use_bias);
+        if (st == AXON_OK && count < b->d.nslots)
+            st = axon_probes_load(b, count, b->d.nslots - count, pad.data(), Ts.data() + count, clamp_steps, use_bias);
+        if (st != AXON_OK) break;
+        if (hipEventRecord(e0, n->stream) != hipSuccess) { st = fail(AXON_ERR_DEVICE, "event"); break; }
+        if ((st = axon_probes_step(b, T)) != AXON_OK) break;
+        if (hipEventRecord(e1, n->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            st = fail(AXON_ERR_DEVICE, "event");
+            break;
+        }
+        float f = 0;
+        (void)hipEventElapsedTime(&f, e0, e1);
+        if (rep) t.push_back(f);  // run 0 captures the graph
+    }
+    axon_probes_destroy(b);
+    if (st != AXON_OK) return st;
+    std::sort(t.begin(), t.end());
+    *med = t[t.size() / 2];
+    *spread = t.back() - t.front();
+    return AXON_OK;
+}
+
+// Every (tile, occupancy) point for the first `count` stimuli; the fastest
+// goes into *best and the tuning table, ms[] (optional) gets every median.
+static axon_status tune_batch(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T, uint32_t clamp_steps,
+                              uint32_t use_bias, axon_tune_entry* best, double* ms) {
     HIPCHECK(hipSetDevice(n->device));
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(AXON_ERR_DEVICE, "hipEventCreate failed");
+    }
+    *best = axon_tune_entry{n->NCOL, count, T, 4u, 0u, (uint32_t)kTuneReps, 1e30, 0.0};
     axon_status st = AXON_OK;
     for (int c = 0; c < AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES && st == AXON_OK; ++c) {
         const int i = c / AXON_PROBE_OCCUPANCIES, o = c % AXON_PROBE_OCCUPANCIES;
-        axon_probes* b = nullptr;
-        if ((st = axon_probes_create(n, count, T, tiles[i], &b)) != AXON_OK) break;
-        if ((st = axon_probes_set_occupancy(b, kProbeWaves[o])) != AXON_OK) {
-            axon_probes_destroy(b);
-            break;
-        }
-        std::vector<uint32_t> Ts(b->d.nslots, 0);
-        std::fill(Ts.begin(), Ts.begin() + count, T);
-        std::vector<uint32_t> pad((size_t)(b->d.nslots - count) * n->NCOL, 0);
-        double t_ms = 0;
-        for (int rep = 0; rep < 2 && st == AXON_OK; ++rep) {  // first pass captures the graph
-            st = axon_probes_load(b, 0, count, stims, Ts.data(), clamp_steps, use_bias);
-            if (st == AXON_OK && count < b->d.nslots)
-                st = axon_probes_load(b, count, b->d.nslots - count, pad.data(), Ts.data() + count, clamp_steps, use_bias);
-            if (st != AXON_OK) break;
-            if (hipEventRecord(e0, n->stream) != hipSuccess) { st = fail(AXON_ERR_DEVICE, "event"); break; }
-            if ((st = axon_probes_step(b, T)) != AXON_OK) break;
-            if (hipEventRecord(e1, n->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                st = fail(AXON_ERR_DEVICE, "event");
-                break;
-            }
-            float f = 0;
-            (void)hipEventElapsedTime(&f, e0, e1);
-            t_ms = f;
-        }
-        axon_probes_destroy(b);
+        double med = 0, spread = 0;
+        st = tune_point(n, count, stims, T, clamp_steps, use_bias, kProbeTiles[i], kProbeWaves[o], e0, e1, &med,
+                        &spread);
         if (st != AXON_OK) break;
-        if (ms) ms[c] = t_ms;
-        if (t_ms < best) { best = t_ms; *best_tile = tiles[i]; *best_waves_per_cu = kProbeWaves[o]; }
+        if (ms) ms[c] = med;
+        if (med < best->ms) {
+            best->tile = kProbeTiles[i];
+            best->waves_per_cu = kProbeWaves[o];
+            best->ms = med;
+            best->spread_ms = spread;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (st == AXON_OK) tune_record(*best);
     return st;
+}
+
+static axon_status probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                  uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
+                                  uint32_t* best_waves_per_cu, double* ms) {
+    if (!n || !stims || !best_tile || !best_waves_per_cu) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (count == 0) return fail(AXON_ERR_INVALID_ARG, "no probes");
+    axon_tune_entry best{};
+    axon_status st = tune_batch(n, count, stims, T, clamp_steps, use_bias, &best, ms);
+    if (st != AXON_OK) return st;
+    *best_tile = best.tile;
+    *best_waves_per_cu = best.waves_per_cu;
+    return AXON_OK;
+}
+
+// The batch-size axis: the same tuning for the first sizes[i] stimuli, each
+// size its own table entry (a server's slots per network is such a size).
+static axon_status probe_autotune_sizes(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                        uint32_t clamp_steps, uint32_t use_bias, const uint32_t* sizes,
+                                        uint32_t nsizes, axon_tune_entry* out) {
+    if (!n || !stims || !sizes || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    for (uint32_t i = 0; i < nsizes; ++i)
+        if (sizes[i] == 0 || sizes[i] > count) return fail(AXON_ERR_INVALID_ARG, "size %u outside 1..%u", sizes[i], count);
+    for (uint32_t i = 0; i < nsizes; ++i) {
+        axon_status st = tune_batch(n, sizes[i], stims, T, clamp_steps, use_bias, &out[i], nullptr);
+        if (st != AXON_OK) return st;
+    }
+    return AXON_OK;
+}
+
+// Nearest entry for ncol: the smallest distance in log2(probes) plus
+// log2(T); an exact key has distance 0.
+static bool tune_find(uint32_t ncol, uint32_t probes, uint32_t T, axon_tune_entry* out) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    double bestd = 1e30;
+    for (const auto& kv : g_tune) {
+        const axon_tune_entry& e = kv.second;
+        if (e.ncol != ncol) continue;
+        const double d = std::fabs(std::log2((double)e.probes) - std::log2((double)std::max(probes, 1u))) +
+                         std::fabs(std::log2((double)e.T) - std::log2((double)std::max(T, 1u)));
+        if (d < bestd) { bestd = d; *out = e; }
+    }
+    return bestd < 1e30;
+}
+
+extern "C" axon_status axon_tune_lookup(uint32_t ncol, uint32_t probes, uint32_t T, axon_tune_entry* out) {
+    if (!out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    if (!tune_find(ncol, probes, T, out)) return fail(AXON_ERR_NOT_READY, "no tuning entry for ncol=%u", ncol);
+    return AXON_OK;
+}
+
+extern "C" uint32_t axon_tune_entries(axon_tune_entry* out, uint32_t max) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    uint32_t i = 0;
+    for (const auto& kv : g_tune) {
+        if (out && i < max) out[i] = kv.second;
+        ++i;
+    }
+    return i;
+}
+
+extern "C" void axon_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+// File form: a first line "axon-tune 1", then one line per entry:
+// ncol probes T tile waves_per_cu reps ms spread_ms
+static axon_status tune_save(const char* path) {
+    if (!path) return fail(AXON_ERR_INVALID_ARG, "null path");
+    FILE* f = fopen(path, "w");
+    if (!f) return fail(AXON_ERR_IO, "cannot write %s", path);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    bool ok = fprintf(f, "axon-tune 1\n") > 0;
+    for (const auto& kv : g_tune) {
+        const axon_tune_entry& e = kv.second;
+        ok = ok && fprintf(f, "%u %u %u %u %u %u %.6f %.6f\n", e.ncol, e.probes, e.T, e.tile, e.waves_per_cu, e.reps,
+                           e.ms, e.spread_ms) > 0;
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? AXON_OK : fail(AXON_ERR_IO, "write to %s failed", path);
+}
+
+static bool valid_tile(uint32_t t) {
+    for (uint32_t v : kProbeTiles)
+        if (v == t) return true;
+    return false;
+}
+static bool valid_waves(uint32_t w) {
+    for (uint32_t v : kProbeWaves)
+        if (v == w) return true;
+    return false;
+}
+
+// Merges a saved table into this process's (a loaded key replaces the same
+// key); nothing is merged unless every line parses and holds a valid choice.
+static axon_status tune_load(const char* path) {
+    if (!path) return fail(AXON_ERR_INVALID_ARG, "null path");
+    FILE* f = fopen(path, "r");
+    if (!f) return fail(AXON_ERR_IO, "cannot read %s", path);
+    char magic[16] = {0};
+    int ver = 0;
+    std::vector<axon_tune_entry> got;
+    bool ok = fscanf(f, "%15s %d", magic, &ver) == 2 && !strcmp(magic, "axon-tune") && ver == 1;
+    while (ok) {
+        axon_tune_entry e{};
+        const int r = fscanf(f, "%u %u %u %u %u %u %lf %lf", &e.ncol, &e.probes, &e.T, &e.tile, &e.waves_per_cu,
+                             &e.reps, &e.ms, &e.spread_ms);
+        if (r == EOF) break;
+        ok = r == 8 && e.ncol && e.probes && e.T && valid_tile(e.tile) && valid_waves(e.waves_per_cu);
+        if (ok) got.push_back(e);
+    }
+    fclose(f);
+    if (!ok) return fail(AXON_ERR_IO, "%s is not a tuning table", path);
+    for (const axon_tune_entry& e : got) tune_record(e);
+    return AXON_OK;
 }
 
 extern "C" axon_status axon_server_completed(const axon_server* s, uint64_t* ids, uint32_t max, uint32_t* n) {
@@ -1880,6 +2064,49 @@ static axon_status net_load(const char* path, axon_net** out) {
 }
 
 // ================================================================== memory probes
+// Propagation probe: the counterpart of the reference's probe05 (dense and
+// activity-swept binary propagation, bench/probe05_prop.mm:41-70, 200-212)
+// and probe06 (the same spike count scattered over neurons or gathered into
+// whole columns, bench/probe06_columnar.mm:117-133, 180-200), measured on this
+// engine's own propagation kernel and synapse layout rather than on a
+// separate benchmark kernel: `spikes` becomes the presynaptic activity of a
+// step (the spike words and the per-column "spiked" bytes the kernel stages),
+// and the network's propagation runs on it.
+static axon_status prop_probe(axon_net* n, const uint32_t* spikes, uint32_t reps, double* us, uint64_t* lines) {
+    if (!n || !spikes || !us || !lines || reps == 0u) return fail(AXON_ERR_INVALID_ARG, "null argument or reps 0");
+    const uint32_t NC = n->NCOL;
+    HIPCHECK(hipSetDevice(n->device));
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    std::vector<uint8_t> nz(NC);
+    for (uint32_t c = 0; c < NC; ++c) nz[c] = spikes[c] != 0u ? 1u : 0u;
+    HIPCHECK(put(n, n->d.spk[0], spikes, (size_t)NC * 4));
+    HIPCHECK(put(n, n->d.colNZ, nz.data(), NC));
+    HIPCHECK(zero(n, n->d.cntProp, (size_t)NC * 8));
+    Params P = n->P;
+    P.step = 0;
+    HIPCHECK(axon::launch_drive(n->d, P, 0, n->stream));  // untimed; its line count is the answer
+    HIPCHECK(hipStreamSynchronize(n->stream));
+    uint64_t pr[2];
+    axon_status st = counter_pairs(n, n->d.cntProp, pr, 0);
+    if (st != AXON_OK) return st;
+    *lines = pr[1];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHECK(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, n->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = axon::launch_drive(n->d, P, 0, n->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, n->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "propagation probe: %s", hipGetErrorString(e));
+    *us = 1e3 * (double)ms / reps;
+    HIPCHECK(zero(n, n->d.cntProp, (size_t)NC * 8));
+    return AXON_OK;
+}
+
 extern "C" axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t reps, double* gbps) {
     if (!gbps || kind > 2u || reps == 0u) return fail(AXON_ERR_INVALID_ARG, "kind 0..2, reps >= 1, gbps not null");
     if (bytes < 1024u || bytes % 1024u || bytes > (64ull << 30))
@@ -1969,4 +2196,29 @@ extern "C" axon_status axon_server_submit(axon_server* s, uint32_t net, const ui
 }
 extern "C" axon_status axon_server_tick(axon_server* s, uint32_t* completed) {
     return guarded("axon_server_tick", [&] { return server_tick(s, completed); });
+}
+extern "C" axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                           uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
+                                           uint32_t* best_waves_per_cu,
+                                           double ms[AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES]) {
+    return guarded("axon_probe_autotune", [&] {
+        return probe_autotune(n, count, stims, T, clamp_steps, use_bias, best_tile, best_waves_per_cu, ms);
+    });
+}
+extern "C" axon_status axon_probe_autotune_sizes(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                                 uint32_t clamp_steps, uint32_t use_bias, const uint32_t* sizes,
+                                                 uint32_t nsizes, axon_tune_entry* out) {
+    return guarded("axon_probe_autotune_sizes", [&] {
+        return probe_autotune_sizes(n, count, stims, T, clamp_steps, use_bias, sizes, nsizes, out);
+    });
+}
+extern "C" axon_status axon_tune_save(const char* path) {
+    return guarded("axon_tune_save", [&] { return tune_save(path); });
+}
+extern "C" axon_status axon_tune_load(const char* path) {
+    return guarded("axon_tune_load", [&] { return tune_load(path); });
+}
+extern "C" axon_status axon_prop_probe(axon_net* n, const uint32_t* spikes, uint32_t reps, double* us,
+                                       uint64_t* lines) {
+    return guarded("axon_prop_probe", [&] { return prop_probe(n, spikes, reps, us, lines); });
 }

@@ -85,6 +85,76 @@ def whole_run_hbm(ncol, steps, lines, seconds, bias=True):
             "peak": HBM_PEAK_GBS, "frac": round(gbps / HBM_PEAK_GBS, 4)}
 
 
+def probe_batch_hbm(ncol, probes, T, union_lines, acc16, seconds):
+    """Algorithmic bytes of a probe batch run: every W3 line the batch needs
+    read once per step (union over probes) + conn (128 B/column) once per
+    step; per probe and step, propagation's spike-word and activity reads
+    (4 + 1/8 B/column), accumulator write (2 or 4 B/neuron) and drive write
+    (4 B/column), and the select's accumulator and drive reads and spike,
+    activity and history writes (2 or 4 B/neuron + 12 + 1/8 B/column)."""
+    n = ncol * 32
+    aw = 2 if acc16 else 4
+    shared = 128 * union_lines + T * 128 * ncol
+    per_probe = probes * T * (2 * aw * n + 20 * ncol + ncol // 4)
+    tot = shared + per_probe
+    gbps = tot / seconds / 1e9
+    return {"bytes": int(tot), "w3_union_bytes": 128 * union_lines, "per_probe_state_bytes": int(per_probe),
+            "GBps": round(gbps, 1), "peak": HBM_PEAK_GBS, "frac": round(gbps / HBM_PEAK_GBS, 4)}
+
+
+def propagation_probes():
+    """Counterparts of the reference's probe05 (binary propagation, dense and
+    with a fraction of the presynaptic columns active) and probe06 (the same
+    number of spikes scattered over neurons or gathered into whole columns),
+    run on the engine's own propagation kernel (axon_prop_probe) with
+    synthetic activity.  Synapse-ops count 1,024 per 128-byte W3 line read
+    (32 presynaptic x 32 postsynaptic neurons); dense = all 8,192 synapses
+    of every neuron, the reference's 1.36e12 synapse-ops/s figure.  ncol=4096
+    is the reference's geometry (W3 = 128 MiB, which the 256 MiB Infinity
+    Cache holds between launches); ncol=131072 puts 4 GiB of W3 behind every
+    launch, out of any cache."""
+    import axon_amd
+
+    out = {}
+    for ncol in (4096, 131072):
+        net = axon_amd.Net(axon_amd.Config.parse(CONFIG_C + [f"ncol={ncol}", "incols=512", "stim=2"]))
+        rng = np.random.default_rng(ncol)
+        n_neur = ncol * 32
+        res = {}
+
+        def run(spk, reps=20):
+            us, lines = axon_amd.prop_probe(net, spk, reps)
+            return {"us": round(us, 3), "lines": lines, "GBps": round(128 * lines / us / 1e3, 1),
+                    "gsyn_ops_s": round(1024 * lines / us / 1e3, 1)}
+
+        dense = run(np.full(ncol, 0xFFFFFFFF, np.uint32))
+        dense["vs_reference_1.36e12"] = round(dense["gsyn_ops_s"] * 1e9 / 1.36e12, 2)
+        res["dense"] = dense
+        sweep = {}
+        for pm in (10, 30, 50, 100, 250, 500):  # per mille of columns spiking
+            spk = np.zeros(ncol, np.uint32)
+            cols = rng.choice(ncol, max(1, ncol * pm // 1000), replace=False)
+            spk[cols] = rng.integers(1, 1 << 32, len(cols), dtype=np.uint64).astype(np.uint32)
+            sweep[f"{pm / 10:g}%"] = run(spk)
+        res["active_columns_sweep"] = sweep
+        col = {}
+        for pct in (0.25, 1.0, 5.0):  # probe06: the same spike count, scattered vs columnar
+            nspk = int(n_neur * pct / 100)
+            scat = np.zeros(n_neur, bool)
+            scat[rng.choice(n_neur, nspk, replace=False)] = True
+            scat_w = np.packbits(scat.reshape(ncol, 32), axis=1, bitorder="little").view(np.uint32).ravel()
+            clus_w = np.zeros(ncol, np.uint32)
+            clus_w[rng.choice(ncol, nspk // 32, replace=False)] = 0xFFFFFFFF
+            a, b = run(scat_w), run(clus_w)
+            col[f"{pct:g}%"] = {"spikes": nspk, "scattered": a, "columnar": b,
+                                "speedup": round(a["us"] / b["us"], 2),
+                                "line_ratio": round(a["lines"] / max(1, b["lines"]), 2)}
+        res["scattered_vs_columnar"] = col
+        out[f"ncol_{ncol}"] = res
+        net.close()
+    return out
+
+
 def hbm_calibration():
     """Achieved read bandwidth of the engine's access shapes, beyond the 256 MiB
     Infinity Cache (4 GiB) and within it (one plane, 128 MiB)."""
@@ -95,6 +165,7 @@ def hbm_calibration():
         for label, nbytes in (("4GiB", 4 << 30), ("128MiB", 128 << 20)):
             g = axon_amd.mem_probe(kind, nbytes, 5)
             out[f"{name}_{label}"] = {"GBps": round(g, 1), "frac_of_peak": round(g / HBM_PEAK_GBS, 4)}
+    out["propagation_probes"] = propagation_probes()
     return out
 
 
@@ -181,6 +252,27 @@ def probe_set(net, axon_amd):
     return np.stack(cues)
 
 
+_POPC8 = np.array([bin(i).count("1") for i in range(256)], np.int64)
+
+
+def probe_lines(conn, ncol, hists):
+    """W3 lines a readout propagation needs, from each probe's spike history:
+    at step t a (column, branch) tile reads one 128-byte line per presynaptic
+    column of its group (conn) that spiked at step t - 1 (none at step 0).
+    Returns (lines summed over probes, lines of the union over probes per
+    step): the first is what per-probe reads cost, the second the bytes of
+    every tile read once per step for the whole batch."""
+    ng = ncol // 8
+    tiles_per_group = np.bincount(np.asarray(conn, np.int64), minlength=ng)
+    per_probe, union = 0, None
+    for h in hists:
+        nz = (np.asarray(h)[:-1] != 0).reshape(len(h) - 1, ng, 8)
+        gm = np.packbits(nz, axis=2, bitorder="little")[..., 0]
+        per_probe += int((_POPC8[gm] * tiles_per_group).sum())
+        union = gm.copy() if union is None else union | gm
+    return per_probe, int((_POPC8[union] * tiles_per_group).sum())
+
+
 def c3_probes(args):
     import axon_amd
 
@@ -197,9 +289,13 @@ def c3_probes(args):
     t_plain = time.perf_counter() - t0
     lines = net.line_counters()
     tiles_read, _ = net.work_counters(reset=True)
+    per_probe_lines, union_lines = probe_lines(net.read("CONN"), net.ncol, plain)
     log("probes: autotuning the probe tile and the propagation's occupancy")
     best, waves, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
     pb = axon_amd.Probes(net, len(stims), T, best, waves)
+    # 16-bit accumulators when no accumulator can reach 2^16 (32 branches at most 256 + boost each, + facil)
+    per_branch = {0: 256, 1: 256 + cfg.boost}.get(cfg.mode, 1)
+    acc16 = 32 * per_branch + cfg.facil <= 0xFFFF
     pb.run(stims, T, clamp, 0, hist=False)  # warm (graph capture)
     _, hist, ms = pb.run(stims, T, clamp, 0, hist=True)
     exact = all(np.array_equal(hist[i], plain[i]) for i in range(len(stims)))
@@ -213,7 +309,11 @@ def c3_probes(args):
             "best_waves_per_cu": waves,
             "tile_sweep_ms": {f"{t}/{w}": round(v, 3) for (t, w), v in sweep.items()},
             "plain_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, t_plain, bias=False),
-            "batched_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, ms / 1e3, bias=False),
+            # W3 lines counted once per (tile, line) per step for the whole batch; the per-probe
+            # state (spikes, accumulators at 2 bytes when 16-bit, drives) once per probe
+            "batched_hbm": probe_batch_hbm(net_ncol, len(stims), T, union_lines, acc16, ms / 1e3),
+            "w3_lines": {"per_probe_sum": per_probe_lines, "union_per_step": union_lines,
+                         "device_counter_plain": lines[0], "host_equals_device": per_probe_lines == lines[0]},
             "bit_exact_vs_plain": bool(exact)}
 
 
@@ -270,6 +370,14 @@ def c4_serving(args):
     mb.train([40] * len(nets), extra=True)
     mb.close()
     out = {"networks": len(nets), "loads": list(loads), "T_range": [10, 200], "rates": []}
+    if args.serve_tile == 0:
+        # the tuning table's entry for (ncol, slots per network, T about the mean query length) is
+        # what a server created with tile 0 takes
+        log(f"serving: tuning the probe tile for {args.serve_slots} slots")
+        k = 3
+        cues = np.stack([axon_amd.probe_cue(nets[k], i % loads[k], (0, 6, 16, 32)[i % 4], 400000 + i)
+                         for i in range(args.serve_slots)])
+        out["tuned"] = axon_amd.probe_autotune_sizes(nets[k], cues, 100, 5, [args.serve_slots])[0]
     for rate in args.serve_rates:
         n_q = max(200, int(rate * args.serve_seconds))
         rs = np.random.RandomState(rate)  # MT19937 stream
@@ -286,6 +394,7 @@ def c4_serving(args):
             qs.append((k, cue, int(rs.randint(10, 201))))
         srv = axon_amd.Server(nets, slots_per_net=args.serve_slots, max_t=200, tile=args.serve_tile,
                               steps_per_tick=args.serve_steps)
+        out["server_tile"], out["server_waves_per_cu"] = srv.config()["tile"], srv.config()["waves_per_cu"]
         log(f"serving: {n_q} queries at {rate}/s")
         start = srv.clock()
         ids, lat, res, nxt, last_done = {}, [], {}, 0, start
@@ -423,7 +532,7 @@ def main():
     ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-slots", type=int, default=128)
-    ap.add_argument("--serve-tile", type=int, default=4)
+    ap.add_argument("--serve-tile", type=int, default=0, help="0: the autotuner's choice")
     ap.add_argument("--serve-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -232,13 +232,40 @@ axon_status axon_probes_run_noalloc(axon_net* n, uint32_t count, const uint32_t*
 axon_status axon_probes_set_occupancy(axon_probes* b, uint32_t waves_per_cu);
 /* autotuner: runs this probe batch with every tile size (1/2/4/8/16/32/64
  * probes per propagation wave) and every occupancy (0/32/16 waves per CU),
- * times the device work, returns the fastest pair; ms[tile_index *
- * AXON_PROBE_OCCUPANCIES + occupancy_index] in those orders */
+ * times the device work (median of 3 timed runs after an untimed one),
+ * returns the fastest pair and records it in the tuning table under (ncol,
+ * count, T); ms[tile_index * AXON_PROBE_OCCUPANCIES + occupancy_index] in
+ * those orders */
 #define AXON_PROBE_TILES 7
 #define AXON_PROBE_OCCUPANCIES 3
 axon_status axon_probe_autotune(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
                                 uint32_t clamp_steps, uint32_t use_bias, uint32_t* best_tile,
                                 uint32_t* best_waves_per_cu, double ms[AXON_PROBE_TILES * AXON_PROBE_OCCUPANCIES]);
+/* Tuning table: the autotuners record, per (ncol, probes, T), the fastest
+ * (tile, waves per CU) with the median time of its timed runs (reps of them)
+ * and their spread (max - min).  The table lives in the process; save and
+ * load keep it across processes (text, first line "axon-tune 1").  A server
+ * created with tile = 0 takes the entry nearest to (ncol, slots_per_net,
+ * max_t).  Extension with no reference counterpart. */
+typedef struct axon_tune_entry {
+    uint32_t ncol, probes, T, tile, waves_per_cu, reps;
+    double ms, spread_ms;
+} axon_tune_entry;
+/* the batch-size axis: axon_probe_autotune for the first sizes[i] stimuli
+ * (1 <= sizes[i] <= count), one entry each, written to out[i] */
+axon_status axon_probe_autotune_sizes(axon_net* n, uint32_t count, const uint32_t* stims, uint32_t T,
+                                      uint32_t clamp_steps, uint32_t use_bias, const uint32_t* sizes, uint32_t nsizes,
+                                      axon_tune_entry* out);
+/* the entry for ncol nearest to (probes, T) in log2(probes) + log2(T);
+ * AXON_ERR_NOT_READY when the table has none for ncol */
+axon_status axon_tune_lookup(uint32_t ncol, uint32_t probes, uint32_t T, axon_tune_entry* out);
+/* copies up to max entries, returns how many the table holds */
+uint32_t axon_tune_entries(axon_tune_entry* out, uint32_t max);
+void axon_tune_clear(void);
+axon_status axon_tune_save(const char* path);
+/* merges a saved table (a key loaded replaces the same key); a file with any
+ * line that does not parse or names an unknown tile/occupancy merges nothing */
+axon_status axon_tune_load(const char* path);
 /* recall cue: pattern's stimulus columns with `nreplace` of them replaced by
  * other input columns (mt19937(seed) draws); pattern 0xFFFFFFFF = a random cue
  * of stimcols input columns.  Extends loadStim (mm:800-804). */
@@ -268,6 +295,17 @@ axon_status axon_checksum(axon_net* n, axon_buffer which, uint64_t* sum);
  * of the percent-of-peak figures, not part of the hot path. */
 axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t reps, double* gbps);
 
+/* Propagation probe, the counterpart of the reference's probe05 (dense and
+ * activity-swept propagation, bench/probe05_prop.mm:41-70) and probe06
+ * (scattered vs columnar activity of one spike count,
+ * bench/probe06_columnar.mm:117-133), run on this engine's propagation
+ * kernel: `spikes` (NCOL words) is the presynaptic activity of a step; one
+ * untimed launch gives *lines, the 128-byte W3 lines a launch reads, then
+ * `reps` launches are timed back to back, *us = mean microseconds per launch.
+ * Overwrites the network's step state (spikes, drive, accumulators, work
+ * counters); the planes, conn and bias are unchanged. */
+axon_status axon_prop_probe(axon_net* n, const uint32_t* spikes, uint32_t reps, double* us, uint64_t* lines);
+
 /* ---- serving: continuous batching of recall queries ----
  * Extension with no reference counterpart.  Trained networks stay resident;
  * each query is one readout presentation (a probe) of T steps on one network.
@@ -288,10 +326,10 @@ typedef struct axon_server axon_server;
 typedef struct axon_server_config {
     uint32_t slots_per_net; /* probe slots per network */
     uint32_t max_t;         /* longest query, steps */
-    uint32_t tile;          /* probe tile: 1, 2, 4, 8, 16, 32 or 64 */
+    uint32_t tile;          /* probe tile: 1, 2, 4, 8, 16, 32 or 64; 0 = the tuning table's choice */
     uint32_t steps_per_tick;
     uint64_t hbm_budget;    /* bytes of in-flight query state, 0 = no limit */
-    uint32_t waves_per_cu;  /* probe propagation occupancy: 0 (no limit), 32 or 16 */
+    uint32_t waves_per_cu;  /* probe propagation occupancy: 0 (no limit), 32 or 16 (set by the table when tile = 0) */
     uint32_t reserved[3];
 } axon_server_config;
 axon_status axon_server_create(axon_net* const* nets, uint32_t nnet, const axon_server_config* cfg,
@@ -306,6 +344,8 @@ axon_status axon_server_completed(const axon_server* s, uint64_t* ids, uint32_t 
 /* spikes NCOL words, row_digest T words; AXON_ERR_NOT_READY until finished */
 axon_status axon_server_result(axon_server* s, uint64_t id, uint32_t* spikes, uint64_t* row_digest,
                                double* latency_s, uint32_t* T);
+/* the configuration in use (tile and waves_per_cu as chosen when tile was 0) */
+axon_status axon_server_get_config(const axon_server* s, axon_server_config* cfg);
 axon_status axon_server_status(const axon_server* s, uint32_t* queued, uint32_t* inflight,
                                uint64_t* reserved_bytes);
 

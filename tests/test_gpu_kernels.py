@@ -82,3 +82,32 @@ def test_mem_probe_runs_and_reports_plausible_bandwidth():
             g = axon_amd.mem_probe(kind, nbytes, 2)
             print(f"mem probe kind {kind} {nbytes >> 20} MiB: {g:.0f} GB/s", flush=True)
             assert 100.0 < g < 8000.0
+
+
+def test_prop_probe_reads_exactly_the_spiking_columns_lines():
+    """The propagation probe runs the network's propagation on the given
+    activity: a launch reads one W3 line per (tile, spiking presynaptic
+    column of its group), dense activity reads every line, none reads none,
+    and the planes are left as they were."""
+    net = axon_amd.Net(axon_amd.Config.parse(["ncol=512", "incols=128", "stim=2", "seed=4"]))
+    conn = net.read("CONN").astype(np.int64)
+    w3 = net.read("W3").copy()
+    tiles_per_group = np.bincount(conn, minlength=512 // 8)
+    rng = np.random.default_rng(9)
+
+    def host_lines(spk):
+        gm = (spk != 0).reshape(-1, 8).sum(axis=1)  # spiking columns per presynaptic group
+        return int((gm * tiles_per_group).sum())
+
+    us, lines = axon_amd.prop_probe(net, np.full(512, 0xFFFFFFFF, np.uint32), 3)
+    assert lines == 512 * 32 * 8 and us > 0
+    assert axon_amd.prop_probe(net, np.zeros(512, np.uint32), 3)[1] == 0
+    for frac in (0.01, 0.1, 0.5):
+        spk = np.zeros(512, np.uint32)
+        cols = rng.choice(512, int(512 * frac), replace=False)
+        spk[cols] = rng.integers(1, 1 << 32, len(cols), dtype=np.uint64).astype(np.uint32)
+        assert axon_amd.prop_probe(net, spk, 2)[1] == host_lines(spk), frac
+    assert np.array_equal(net.read("W3"), w3)
+    with pytest.raises(ValueError):
+        axon_amd.prop_probe(net, np.zeros(100, np.uint32))
+    net.close()

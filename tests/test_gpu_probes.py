@@ -153,6 +153,48 @@ def test_autotuner_sweeps_tiles_and_occupancies():
     assert all(v > 0 for v in table.values()) and table[(best, waves)] == min(table.values())
     with pytest.raises(axon_amd.AxonError):
         axon_amd.Probes(net, slots=8, max_t=32, tile=4, waves_per_cu=24)
+    # the run went into the tuning table under (ncol, probes, T)
+    e = axon_amd.tune_lookup(net.ncol, 16, 20)
+    assert (e["probes"], e["T"], e["tile"], e["waves_per_cu"], e["reps"]) == (16, 20, best, waves, 3)
+    assert e["ms"] == table[(best, waves)] and e["spread_ms"] >= 0
+
+
+def test_autotuner_batch_sizes_and_server_choice(tmp_path):
+    """The batch-size axis gives one table entry per size; a server created
+    with tile 0 takes the nearest entry, and its results stay those of
+    single presentations; a saved table restores the same choice."""
+    net, _ = trained(C0, 2)
+    stims = cues(net, 32)
+    axon_amd.tune_clear()
+    got = axon_amd.probe_autotune_sizes(net, stims, 24, 5, [4, 16, 32])
+    assert [e["probes"] for e in got] == [4, 16, 32]
+    assert all(e["ncol"] == net.ncol and e["T"] == 24 and e["reps"] == 3 and e["ms"] > 0 for e in got)
+    assert all(e["tile"] in axon_amd.PROBE_TILES and e["waves_per_cu"] in axon_amd.PROBE_WAVES for e in got)
+    assert len(axon_amd.tune_entries()) == 3
+    with pytest.raises(axon_amd.AxonError):
+        axon_amd.probe_autotune_sizes(net, stims, 24, 5, [33])
+    f = tmp_path / "tune.txt"
+    axon_amd.tune_save(str(f))
+    axon_amd.tune_clear()
+    axon_amd.tune_load(str(f))
+    srv = axon_amd.Server([net], slots_per_net=16, max_t=24, tile=0, steps_per_tick=6)
+    cfg = srv.config()
+    want = [e for e in got if e["probes"] == 16][0]
+    assert (cfg["tile"], cfg["waves_per_cu"]) == (want["tile"], want["waves_per_cu"])
+    ids = {srv.submit(0, stims[i], 24): i for i in range(20)}
+    results = {}
+    while len(results) < 20:
+        srv.tick()
+        for rid in srv.completed():
+            results[ids[rid]] = srv.result(rid, 24)
+    for i, (spk, dig, _, T) in results.items():
+        r = net.present(0, 1.0, 24, 5, False, True, 0, 0, stim=stims[i])
+        assert np.array_equal(spk, r.spikes) and np.array_equal(dig, axon_amd.row_digest(r.hist)), f"query {i}"
+    srv.close()
+    axon_amd.tune_clear()
+    srv = axon_amd.Server([net], slots_per_net=16, max_t=24, tile=0)  # empty table: tile 4, 32 waves per CU
+    assert (srv.config()["tile"], srv.config()["waves_per_cu"]) == (4, 32)
+    srv.close()
 
 
 @pytest.mark.parametrize("mode_args", [["mode=0"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=2"],
