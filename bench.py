@@ -301,6 +301,9 @@ def c3_probes(args):
     exact = all(np.array_equal(hist[i], plain[i]) for i in range(len(stims)))
     pb.close()
     net_ncol = net.ncol
+    # what the CPU baseline needs to run the same probes on the same trained network
+    cpu_case = {"args": CONFIG_C + "stim=60 epochs=40 seed=0".split(), "T": T, "clamp": clamp, "stims": stims,
+                "hist": plain, "state": {nm: net.read(nm).copy() for nm in ("W3", "CONN", "BIAS")}}
     net.close()
     syn = SYN_PER_TILE * tiles_read
     return {"probes": len(stims), "T": T, "plain_gsyn_s": round(syn / t_plain / 1e9, 1),
@@ -314,7 +317,7 @@ def c3_probes(args):
             "batched_hbm": probe_batch_hbm(net_ncol, len(stims), T, union_lines, acc16, ms / 1e3),
             "w3_lines": {"per_probe_sum": per_probe_lines, "union_per_step": union_lines,
                          "device_counter_plain": lines[0], "host_equals_device": per_probe_lines == lines[0]},
-            "bit_exact_vs_plain": bool(exact)}
+            "bit_exact_vs_plain": bool(exact), "_cpu_case": cpu_case}
 
 
 def c2_networks(args):
@@ -472,9 +475,85 @@ def readouts(args):
     return out
 
 
-def cpu_bench(args):
+def cpu_networks(args, threads):
+    """The many-network workload on the CPU: `threads` of the same networks
+    (loads 12-60, seeds 0-4, in that order), one oracle network per host
+    thread, each running its training presentations in the plain order for
+    the bounded time; aggregate GSyn/s over all of them."""
+    import threading
+
+    import oracle
+
+    loads = (12, 24, 36, 48, 60)
+    specs = [(P, sd) for sd in range(5) for P in loads][:threads]
+    nets = [None] * len(specs)
+    done = [0] * len(specs)
+
+    def make(i):
+        P, sd = specs[i]
+        nets[i] = oracle.OracleNet(oracle.parse(CONFIG_C + f"stim={P} epochs=40 seed={sd}".split()), threads=1)
+
+    ts = [threading.Thread(target=make, args=(i,)) for i in range(len(specs))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    T, tc = nets[0].cfg.T, nets[0].cfg.trainclamp
+    t0 = time.perf_counter()
+    deadline = t0 + args.cpu_seconds
+
+    def train(i):
+        P = specs[i][0]
+        while time.perf_counter() < deadline:
+            p = done[i] % P
+            nets[i].present(p, 1.0, T, tc, True, 1000 + p)
+            done[i] += 1
+
+    ts = [threading.Thread(target=train, args=(i,)) for i in range(len(specs))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    dt = time.perf_counter() - t0
+    tiles = sum(n.work_counters()[0] for n in nets)
+    for n in nets:
+        n.close()
+    return {"value": round(SYN_PER_TILE * tiles / dt / 1e9, 3), "unit": "GSyn/s", "cores": len(specs),
+            "kind": "port",
+            "sample": f"{len(specs)} networks (loads 12-60, seeds 0-4), one host thread each, the first "
+                      f"{sum(done)} learning presentations in all, {dt:.1f} s"}
+
+
+def cpu_probes(args, threads, case):
+    """The probe workload on the CPU: the oracle network given the GPU-trained
+    load-60 network's W3, conn and bias, running the same cues one readout
+    presentation at a time (column-parallel over `threads` host threads) for
+    the bounded time; every probe it finishes is checked against the GPU's
+    history of that probe."""
+    import oracle
+
+    ora = oracle.OracleNet(oracle.parse(case["args"]), threads=threads)
+    for nm, v in case["state"].items():
+        ora.buf(nm)[:] = v
+    T, clamp, stims = case["T"], case["clamp"], case["stims"]
+    ora.work_counters(reset=True)
+    done, same, t0 = 0, True, time.perf_counter()
+    while done < len(stims) and time.perf_counter() - t0 < args.cpu_seconds:
+        ora.present(0, 1.0, T, clamp, False, 0, 0, stim=stims[done])
+        same &= bool(np.array_equal(ora.buf("HIST")[: T * ora.ncol].reshape(T, ora.ncol), case["hist"][done]))
+        done += 1
+    dt = time.perf_counter() - t0
+    tiles = ora.work_counters()[0]
+    ora.close()
+    return {"value": round(SYN_PER_TILE * tiles / dt / 1e9, 3), "unit": "GSyn/s", "cores": threads, "kind": "port",
+            "sample": f"the first {done} of the 256 probes (T={T}) on the GPU-trained load-60 network, "
+                      f"{dt:.1f} s", "probes_per_s": round(done / dt, 2), "same_history_as_gpu": same}
+
+
+def cpu_bench(args, cpu_case=None):
     """The oracle (C++ restatement of the reference kernels), column-parallel on
-    the host cores, on the first presentations of the same training run."""
+    the host cores, on the first presentations of the same training run; then
+    the many-network and probe workloads (cpu_networks, cpu_probes)."""
     import oracle
 
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -493,11 +572,18 @@ def cpu_bench(args):
     dt = time.perf_counter() - t0
     prop, learn = net.work_counters()
     net.close()
-    return {"value": round(SYN_PER_TILE * prop / dt / 1e9, 3), "unit": "GSyn/s", "cores": threads,
-            "kind": "port",
-            "sample": f"first {done} learning presentations ({done * T} timesteps) of the same full-scale "
-                      f"training run, {dt:.1f} s",
-            "timesteps_per_s": round(done * T / dt, 2)}
+    out = {"value": round(SYN_PER_TILE * prop / dt / 1e9, 3), "unit": "GSyn/s", "cores": threads,
+           "kind": "port",
+           "sample": f"first {done} learning presentations ({done * T} timesteps) of the same full-scale "
+                     f"training run, {dt:.1f} s",
+           "timesteps_per_s": round(done * T / dt, 2)}
+    if "networks" in args.sections.split(","):
+        log(f"cpu baseline: {threads} networks, one thread each")
+        out["networks"] = cpu_networks(args, threads)
+    if cpu_case is not None:
+        log("cpu baseline: probes on the trained load-60 network")
+        out["probes"] = cpu_probes(args, threads, cpu_case)
+    return out
 
 
 def copy_calibration(args):
@@ -554,7 +640,8 @@ def main():
         batched["networks"] = c2_networks(args)
     serving = c4_serving(args) if "serving" in sections else None
     reads = readouts(args) if "readouts" in sections else None
-    cpu = None if args.no_cpu else cpu_bench(args)
+    cpu_case = batched.get("probes", {}).pop("_cpu_case", None)
+    cpu = None if args.no_cpu else cpu_bench(args, cpu_case)
     if g is None:  # a run of the batched sections alone (profiling): no headline figure
         print(json.dumps({"sections": sorted(sections), "batched_vs_plain": batched, "serving": serving,
                           "readouts": reads, "hbm_calibration": calib, "cpu_baseline": cpu}), flush=True)

@@ -64,10 +64,14 @@ def test_multi_rejects_mismatched_dynamics():
         axon_amd.MultiNet([a, a])
 
 
-@pytest.mark.parametrize("mode_args", [["mode=0", "hetero=1"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=3"]],
-                         ids=["mode0-hetero", "mode1-dthr2", "mode2"])
+@pytest.mark.parametrize("mode_args", [["mode=0", "hetero=1"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=3"],
+                                       ["target=384"], ["target=400", "topkpar=0"], ["ncol=520"]],
+                         ids=["mode0-hetero", "mode1-dthr2", "mode2", "gate-count-all-recurrent",
+                              "gate-topk-beyond-recurrent", "gate-topk-ncol520"])
 def test_multi_dendritic_modes(mode_args):
-    """Every branch-output mode of the two-column propagation: batched training equals training alone."""
+    """Every branch-output mode of the two-column propagation, and the per-network top-K launch at the
+    target boundaries (every recurrent column, beyond them) and at a column count that is not a multiple
+    of 32: batched training equals training alone."""
     specs = [(4, 0, 2), (6, 1, 1), (5, 2, 3)]
     batched = [axon_amd.Net(axon_amd.Config.parse(cfg_args(s, sd) + mode_args)) for s, sd, _ in specs]
     mb = axon_amd.MultiNet(batched)
