@@ -361,6 +361,10 @@ static axon_status net_create(const axon_config* c, axon_net** out) {
     P.hetero = c->hetero; P.ltdRep = c->ltdrep; P.biasDownEvery = c->biasdownevery; P.histOn = 1u;
     n->K = c->k; n->kTest = c->ktest; n->facil = c->facil; n->contrast = c->contrast;
     n->ltdEvery = c->ltdevery ? c->ltdevery : 1u;
+    // a neuron's accumulator is at most 32 branches x (256 presynaptic bits +
+    // boost) in mode 1 (x 256 in mode 0, x 1 in mode 2) plus the facilitation
+    const uint64_t perBranch = P.mode == 0u ? 256u : P.mode == 1u ? 256ull + P.boost : 1u;
+    d.acc16 = (uint64_t)axon::NSLOT * perBranch + n->facil <= 0xFFFFu ? 1u : 0u;
     *out = n;
     return AXON_OK;
 }
@@ -580,6 +584,13 @@ extern "C" axon_status axon_read_buffer(axon_net* n, axon_buffer w, void* dst, s
     if (bytes > words * 4) return fail(AXON_ERR_INVALID_ARG, "read of %zu bytes exceeds buffer (%zu)", bytes, words * 4);
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
+    if (w == AXON_BUF_ACC && n->d.acc16) {  // 16-bit accumulators, widened
+        std::vector<uint16_t> h(bytes / 4);
+        HIPCHECK(hipMemcpy(h.data(), p, h.size() * 2, hipMemcpyDeviceToHost));
+        uint32_t* o = static_cast<uint32_t*>(dst);
+        for (size_t i = 0; i < h.size(); ++i) o[i] = h[i];
+        return AXON_OK;
+    }
     if (!is_plane(w)) {
         HIPCHECK(hipMemcpy(dst, p, bytes, hipMemcpyDeviceToHost));
         return AXON_OK;
@@ -600,6 +611,16 @@ extern "C" axon_status axon_write_buffer(axon_net* n, axon_buffer w, const void*
     if (bytes > words * 4) return fail(AXON_ERR_INVALID_ARG, "write of %zu bytes exceeds buffer (%zu)", bytes, words * 4);
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
+    if (w == AXON_BUF_ACC && n->d.acc16) {  // 16-bit accumulators: values above 65535 do not fit
+        const uint32_t* v = static_cast<const uint32_t*>(src);
+        std::vector<uint16_t> h(bytes / 4);
+        for (size_t i = 0; i < h.size(); ++i) {
+            if (v[i] > 0xFFFFu) return fail(AXON_ERR_INVALID_ARG, "accumulator %zu = %u does not fit 16 bits", i, v[i]);
+            h[i] = (uint16_t)v[i];
+        }
+        HIPCHECK(put(n, p, h.data(), h.size() * 2));
+        return AXON_OK;
+    }
     if (!is_plane(w)) {
         HIPCHECK(put(n, p, src, bytes));
         return AXON_OK;
@@ -1363,6 +1384,18 @@ extern "C" axon_status axon_checksum(axon_net* n, axon_buffer which, uint64_t* s
     void* p; size_t words;
     axon_status st = buffer_ptr(n, which, &p, &words);
     if (st != AXON_OK) return st;
+    if (which == AXON_BUF_ACC && n->d.acc16) {  // 16-bit accumulators: the sum over their 32-bit values
+        std::vector<uint32_t> v(words);
+        if ((st = axon_read_buffer(n, which, v.data(), words * 4)) != AXON_OK) return st;
+        uint64_t acc = 0;
+        for (size_t i = 0; i < words; ++i) {
+            uint32_t x = v[i] ^ (uint32_t)i ^ (uint32_t)(i >> 32);  // hash32 (lowbias32), as k_checksum
+            x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+            acc += (uint64_t)v[i] * (2ull * i + 1ull) + x;
+        }
+        *sum = acc;
+        return AXON_OK;
+    }
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipMemsetAsync(n->d_small, 0, 8, n->stream));
     HIPCHECK(axon::launch_checksum((const uint32_t*)p, words, (unsigned long long*)n->d_small, n->stream));

@@ -79,6 +79,9 @@ struct NetDev {
     // (~0 = idle), written by k_mb_begin into the batch's device copy of the
     // descriptors so that a kernel learns it with the descriptor itself.
     uint32_t nstats, ndlog, maxt, active;
+    // 1: `acc` holds 16-bit accumulators (u16[ncol*32]), chosen by the host
+    // when no accumulator can reach 2^16; 0: 32-bit
+    uint32_t acc16, pad[3];
 };
 
 // Device pointers of a probe batch: `nslots` independent readout

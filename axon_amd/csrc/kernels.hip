@@ -476,7 +476,11 @@ __device__ __forceinline__ void drive_col(const NetDev& d, const Params& P, cons
     if (P.facil != 0u && ((prev >> n) & 1u) != 0u) acc += P.facil;
     const uint32_t bi = biasw >> P.biasShift;
     acc = (acc > bi) ? (acc - bi) : 0u;
-    if (!hi) { d.acc[gid] = acc; d.bbits[gid] = bb; }
+    if (!hi) {
+        if (d.acc16) reinterpret_cast<uint16_t*>(d.acc)[gid] = (uint16_t)acc;
+        else d.acc[gid] = acc;
+        d.bbits[gid] = bb;
+    }
     const uint32_t drv = half_scan(acc);
     if (lane == 31u) {
         d.drive[col] = drv;
@@ -550,7 +554,8 @@ __device__ __forceinline__ void drive_pair(const NetDev& d, const Params& P, con
     if (P.facil != 0u && ((prev >> n) & 1u) != 0u) acc += P.facil;
     const uint32_t bi = biasw >> P.biasShift;
     acc = (acc > bi) ? (acc - bi) : 0u;
-    d.acc[gid] = acc;
+    if (d.acc16) reinterpret_cast<uint16_t*>(d.acc)[gid] = (uint16_t)acc;
+    else d.acc[gid] = acc;
     d.bbits[gid] = bb;
     const uint32_t drv = half_scan(acc);
     if (n == 31u) {
@@ -777,7 +782,7 @@ __device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, 
     const bool rec = col >= P.inCols;
     EmitIn in;
     in.dv = d.drive[col];
-    in.v = d.acc[gid];
+    in.v = d.acc16 ? (uint32_t)reinterpret_cast<const uint16_t*>(d.acc)[gid] : d.acc[gid];
     in.stimW = clampCol ? d.stim[col] : 0u;
     in.bbv = 0u; in.cn = 0u; in.fwb = 0u; in.band = 0u;
     in.lc = make_uint2(0u, 0u);
@@ -1841,7 +1846,7 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
     const bool rec = col >= P.inCols;
     const uint32_t th = max(d.theta[0], 1u);
     const uint32_t dv = d.drive[col];
-    const uint32_t v = d.acc[gid];
+    const uint32_t v = d.acc16 ? (uint32_t)reinterpret_cast<const uint16_t*>(d.acc)[gid] : d.acc[gid];
     const uint32_t stimW = clampCol ? d.stim[col] : 0u;
     const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
     const bool gatePass = dv >= th;

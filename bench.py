@@ -51,34 +51,41 @@ def train_presentations(net, start, count, T, tc, nstim, outputs=False):
         net.present(p, 1.0, T, tc, True, False, 1000 + p, outputs=outputs)
 
 
-def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True):
+def acc_bytes(cfg):
+    """Bytes per accumulator: 2 when none can reach 2^16 (32 branches of at
+    most 256 + boost in mode 1, 256 in mode 0, 1 in mode 2, + facil), else 4."""
+    per_branch = {0: 256, 1: 256 + cfg.boost}.get(cfg.mode, 1)
+    return 2 if 32 * per_branch + cfg.facil <= 0xFFFF else 4
+
+
+def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2):
     """Algorithmic HBM bytes per kernel over `steps` learning timesteps
     (bias=False: readout steps without the intrinsic bias, which neither read
-    nor update it).
+    nor update it; aw: bytes per accumulator, acc_bytes).
 
     The synapse planes are word-major tiles, so a kernel touches only the
     128-byte lines of presynaptic columns that spiked (device line counters).
     propagation: 128 B per W3 line read + conn (128 B/column) + bias read
-    (4 B/neuron, learning uses the bias) + acc and bbits writes (8 B/neuron)
-    + drive (4 B/column) + fired-column masks (32 B/column).
+    (4 B/neuron, learning uses the bias) + acc (aw B/neuron) and bbits
+    (4 B/neuron) writes + drive (4 B/column) + fired-column masks (32 B/column).
     topk_emit: every workgroup's pass over the column drives for the top-K
-    (L2-resident: counted once, 4 B/column) + acc read (4 B/neuron) + bias
+    (L2-resident: counted once, 4 B/column) + acc read (aw B/neuron) + bias
     read-modify-write (8 B/neuron) + drive read, spike, history and margin
     writes (16 B/column).
     plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read + written)."""
     n = ncol * 32
     bb = 1 if bias else 0
-    prop = 128 * prop_lines + steps * (128 * ncol + 4 * bb * n + 8 * n + 4 * ncol + 32 * ncol)
-    emit = steps * (4 * n + 8 * bb * n + 20 * ncol)
+    prop = 128 * prop_lines + steps * (128 * ncol + 4 * bb * n + (aw + 4) * n + 4 * ncol + 32 * ncol)
+    emit = steps * (aw * n + 8 * bb * n + 20 * ncol)
     plast = 1024 * learn_lines
     return {"propagation": prop, "topk_emit": emit, "plasticity": plast}
 
 
-def whole_run_hbm(ncol, steps, lines, seconds, bias=True):
+def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2):
     """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
     timesteps and device line counters) over the run's wall time: the average
     rate of the whole job, idle launches and gaps included."""
-    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias)
+    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw)
     tot = sum(by.values())
     gbps = tot / seconds / 1e9
     return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
@@ -201,7 +208,7 @@ def gpu_bench(args):
     twin.close()
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
-    kb = kernel_bytes(ncol, steps, lines[0], lines[1])
+    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg))
     names = ["propagation", "unused", "topk_emit", "plasticity"]  # the column top-K runs inside emit
     kern = {}
     for i, nm in enumerate(names):
@@ -222,7 +229,7 @@ def gpu_bench(args):
         "ms_per_step": 1e3 * dt / args.steps,
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
-        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt),
+        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg)),
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dn),
@@ -290,12 +297,13 @@ def c3_probes(args):
     lines = net.line_counters()
     tiles_read, _ = net.work_counters(reset=True)
     per_probe_lines, union_lines = probe_lines(net.read("CONN"), net.ncol, plain)
-    log("probes: autotuning the probe tile and the propagation's occupancy")
-    best, waves, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
+    if args.probe_tile:  # a fixed choice (profiled runs), no sweep
+        best, waves, sweep = args.probe_tile, args.probe_waves, {}
+    else:
+        log("probes: autotuning the probe tile and the propagation's occupancy")
+        best, waves, sweep = axon_amd.probe_autotune(net, stims, T, clamp, 0)
     pb = axon_amd.Probes(net, len(stims), T, best, waves)
-    # 16-bit accumulators when no accumulator can reach 2^16 (32 branches at most 256 + boost each, + facil)
-    per_branch = {0: 256, 1: 256 + cfg.boost}.get(cfg.mode, 1)
-    acc16 = 32 * per_branch + cfg.facil <= 0xFFFF
+    acc16 = acc_bytes(cfg) == 2
     pb.run(stims, T, clamp, 0, hist=False)  # warm (graph capture)
     _, hist, ms = pb.run(stims, T, clamp, 0, hist=True)
     exact = all(np.array_equal(hist[i], plain[i]) for i in range(len(stims)))
@@ -311,7 +319,7 @@ def c3_probes(args):
             "plain_ms": round(1e3 * t_plain, 2), "batched_ms": round(ms, 3), "best_tile": best,
             "best_waves_per_cu": waves,
             "tile_sweep_ms": {f"{t}/{w}": round(v, 3) for (t, w), v in sweep.items()},
-            "plain_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, t_plain, bias=False),
+            "plain_hbm": whole_run_hbm(net_ncol, len(stims) * T, lines, t_plain, bias=False, aw=acc_bytes(cfg)),
             # W3 lines counted once per (tile, line) per step for the whole batch; the per-probe
             # state (spikes, accumulators at 2 bytes when 16-bit, drives) once per probe
             "batched_hbm": probe_batch_hbm(net_ncol, len(stims), T, union_lines, acc16, ms / 1e3),
@@ -357,8 +365,10 @@ def c2_networks(args):
             "plain_gsyn_s": round(SYN_PER_TILE * tiles_plain / t_plain / 1e9, 1),
             "batched_gsyn_s": round(SYN_PER_TILE * tiles / (ms / 1e3) / 1e9, 1),
             "batched_plasticity_gsyn_s": round(SYN_PER_TILE * learn / (ms / 1e3) / 1e9, 1),
-            "plain_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines_plain, t_plain),
-            "batched_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines, ms / 1e3),
+            "plain_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines_plain, t_plain,
+                                       aw=acc_bytes(mk(*specs[0]))),
+            "batched_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines, ms / 1e3,
+                                         aw=acc_bytes(mk(*specs[0]))),
             "bit_exact_vs_plain": bool(exact and tiles == tiles_plain)}
 
 
@@ -611,10 +621,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-calib", action="store_true", help="skip the bandwidth and propagation probes")
     ap.add_argument("--sections", default="c1,probes,networks,serving",
                     help="comma list of c1, probes, networks, serving, calib, readouts (copy: counter "
                          "calibration only)")
     ap.add_argument("--net-epochs", type=int, default=40)
+    ap.add_argument("--probe-tile", type=int, default=0, help="0: the autotuner's choice")
+    ap.add_argument("--probe-waves", type=int, default=0, help="waves per CU with --probe-tile")
     ap.add_argument("--serve-rates", type=lambda x: [int(v) for v in x.split(",")], default=[1000, 5000, 20000])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-slots", type=int, default=128)
@@ -632,7 +645,7 @@ def main():
         copy_calibration(args)
         return
     g = gpu_bench(args) if "c1" in sections else None
-    calib = hbm_calibration() if ("c1" in sections or "calib" in sections) else None
+    calib = hbm_calibration() if (("c1" in sections and not args.no_calib) or "calib" in sections) else None
     batched = {}
     if "probes" in sections:
         batched["probes"] = c3_probes(args)

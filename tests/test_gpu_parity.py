@@ -25,16 +25,20 @@ VARIANTS = {
     # beyond them (every workgroup runs the radix select)
     "target_all_recurrent": SMALL + CONFIG_C + ["target=384", "epochs=1", "seed=2"],
     "target_beyond_recurrent": SMALL + CONFIG_C + ["target=400", "epochs=1", "seed=2", "topkpar=0"],
+    # accumulators past 2^16 (32 branches x (256 + 3000)): 32-bit accumulators instead of 16-bit
+    "boost3000_acc32": SMALL + CONFIG_C + ["boost=3000", "epochs=1", "seed=6"],
 }
 
 
-def compare_present(g, o, T, tag):
+def compare_present(g, o, T, tag, net=None):
     r = g
     assert np.array_equal(r.hist, o.buf("HIST")[: T * o.ncol].reshape(T, o.ncol)), f"{tag}: hist"
     assert np.array_equal(r.spikes, o.final_spikes(T)), f"{tag}: spikes"
     assert np.array_equal(r.stats[:1024], o.buf("STATS")[:1024]), f"{tag}: stats"
     assert np.array_equal(r.drive_log[:256], o.buf("DRIVELOG")[:256]), f"{tag}: driveLog"
     assert np.array_equal(r.theta, o.buf("THETA")[:2]), f"{tag}: theta"
+    if net is not None:  # the last step's accumulators (16- or 32-bit on the device)
+        assert np.array_equal(net.read("ACC"), o.buf("ACC")), f"{tag}: acc"
 
 
 def compare_state(net, ora, tag, planes=True):
@@ -61,7 +65,7 @@ def test_protocol_parity(variant):
         for p in range(nstim):
             g = net.present(p, 1.0, T, tc, True, False, 1000 + p)
             ora.present(p, 1.0, T, tc, True, 1000 + p)
-            compare_present(g, ora, T, f"{variant} ep{ep} p{p}")
+            compare_present(g, ora, T, f"{variant} ep{ep} p{p}", net if p == 0 else None)
             compare_state(net, ora, f"{variant} ep{ep} p{p}", planes=(p == nstim - 1))
     g = net.present(0, 1.0, T, tc, True, False, 31337)
     ora.present(0, 1.0, T, tc, True, 31337)
@@ -75,7 +79,7 @@ def test_protocol_parity(variant):
                                        (0.75, T, clamp, 4300 + p, 1), (1.0, min(35, cfg.max_t), clamp, 7000 + p, 2)]:
             g = net.present(p, keep, TT, cs, False, True, seed, ub)
             ora.present(p, keep, TT, cs, False, seed, ub)
-            compare_present(g, ora, TT, f"{variant} readout p{p} seed{seed}")
+            compare_present(g, ora, TT, f"{variant} readout p{p} seed{seed}", net if p == 0 else None)
             if seed == 2000 + p:
                 refs.append(g.spikes.copy())
     # drive log with a reference ballot and multiplicities (deep / split blocks)
@@ -108,6 +112,8 @@ def test_protocol_parity(variant):
     assert net.wstat() == ora.wstat()
     assert np.array_equal(net.indeg(), ora.indeg())
     assert np.array_equal(net.untouched(), ora.untouched())
+    # the buffer checksum sees the accumulators as 32-bit values, whatever their width on the device
+    assert axon_amd.checksum(net, "ACC") == int(axon_amd.row_digest(ora.buf("ACC")[None, :])[0])
     net.close()
 
 
