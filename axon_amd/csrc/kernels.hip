@@ -1494,25 +1494,35 @@ __global__ __launch_bounds__(256, PT > 4 ? 4 : 8) void k_probe_items(ProbeDev d,
         uint32_t r = 0;
         for (; r + DB <= R; r += DB) probe_round<DB, MODE>(Li + r, w3, sp, n4, A + n, c, d.dthr, d.boost);
         if (r < R) probe_round<4, MODE>(Li + r, w3, sp, n4, A + n, c, d.dthr, d.boost);
-        uint32_t av[PT];
+        // Epilogue: each half-wave finishes its own probes (the low half the
+        // first PE, the high half the rest), so one scan serves two probes.
+        constexpr int PE = PT >= 2 ? PT / 2 : 1;  // probes per half
+        const uint32_t pbase = (PT >= 2 && hi) ? (uint32_t)PE : 0u;
+        const bool halfOn = PT >= 2 || !hi;
+        uint32_t av[PE];
 #pragma unroll
-        for (int p = 0; p < PT; ++p) av[p] = A[p * 32 + n];
+        for (int q = 0; q < PE; ++q) av[q] = A[(pbase + (uint32_t)q) * 32u + n];
         const uint32_t aw = d.acc16 ? 2u : 4u;  // bytes per accumulator
         const __amdgpu_buffer_rsrc_t ar = buf_rsrc(reinterpret_cast<const uint8_t*>(d.acc) +
                                                        (size_t)slot0 * d.ncol * NPC * aw,
                                                    nsl * d.ncol * NPC * aw);
         const __amdgpu_buffer_rsrc_t dr = buf_rsrc(d.drive + (size_t)slot0 * d.ncol, nsl * d.ncol * 4u);
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            if (((live >> p) & 1u) == 0u) continue;
-            uint32_t a = av[p];
-            if (((fs[p] >> n) & 1u) != 0u) a += d.facil;
+        for (int q = 0; q < PE; ++q) {
+            const uint32_t p = pbase + (uint32_t)q;
+            const bool lv = halfOn && ((live >> p) & 1u) != 0u;
+            if (__ballot(lv) == 0ull) continue;  // neither half's probe is live
+            uint32_t fsp;
+            if constexpr (PT >= 2) fsp = hi ? fs[q + PE] : fs[q];
+            else fsp = fs[0];
+            uint32_t a = av[q];
+            if (((fsp >> n) & 1u) != 0u) a += d.facil;
             if (((ubm >> p) & 1u) != 0u) a = (a > bi) ? (a - bi) : 0u;
-            const uint32_t ao = hi ? SKIP : ((uint32_t)p * d.ncol * NPC + gid) * aw;
+            const uint32_t ao = lv ? (p * d.ncol * NPC + gid) * aw : SKIP;
             if (d.acc16) buf_st16(ar, ao, a);
             else buf_st(ar, ao, a);
             const uint32_t drv = half_scan(a);
-            buf_st(dr, lane == 31u ? ((uint32_t)p * d.ncol + col) * 4u : SKIP, drv);
+            buf_st(dr, (lv && n == 31u) ? (p * d.ncol + col) * 4u : SKIP, drv);
         }
     }
 }
