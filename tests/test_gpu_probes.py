@@ -247,6 +247,34 @@ def test_wide_network_takes_the_per_branch_kernel():
     net.close()
 
 
+def test_trained_wide_network_at_the_item_index_limit():
+    """ncol = 131072 after training: tile 4 (4 x 131072 = 2^19 spike words) takes the item kernel, tile 8
+    (exactly 2^20, the item index limit) and tile 16 the per-branch kernel; all equal single
+    presentations."""
+    args = ["ncol=131072", "incols=512", "stim=2", "seed=3"] + CONFIG_C
+    net = axon_amd.Net(axon_amd.Config.parse(args))
+    c = net.config
+    for p in range(2):
+        net.present(p, 1.0, c.T, c.trainclamp, True, False, 1000 + p, outputs=False)
+    net.present(0, 1.0, c.T, c.trainclamp, True, False, 31337, outputs=False)
+    net.sync()
+    stims = np.stack([axon_amd.probe_cue(net, i % 2, (0, 6, 16, 32)[i % 4], 70 + i) for i in range(12)])
+    T, clamp = 10, 5
+    out = {}
+    for tile in (4, 8, 16):
+        pb = axon_amd.Probes(net, slots=12, max_t=16, tile=tile)
+        print(f"ncol 131072 trained tile {tile}", flush=True)
+        out[tile] = pb.run(stims, T, clamp, use_bias=0)[:2]
+        pb.close()
+    for tile in (8, 16):
+        assert np.array_equal(out[4][0], out[tile][0]) and np.array_equal(out[4][1], out[tile][1]), tile
+    for i in (0, 7, 11):
+        r = net.present(0, 1.0, T, clamp, False, True, 0, 0, stim=stims[i])
+        assert np.array_equal(out[4][1][i], r.hist), f"probe {i}"
+    assert np.count_nonzero(out[4][1]) > 0
+    net.close()
+
+
 def test_dense_cues_match_across_tiles():
     """Dense activity (256 clamped input columns, half the recurrent columns winning): long item lists in
     every tile; tiles 1, 4 and 8 equal the per-branch kernel (tile 16) and single presentations."""
