@@ -370,9 +370,15 @@ static axon_status net_create(const axon_config* c, axon_net** out) {
 }
 
 // ================================================================== present
+// Without side statistics, step 0 of a presentation of two or more steps is
+// one launch (k_emit_first: the spikes start cleared, so that step's
+// propagation yields zeros and its plasticity has no voted tile); the
+// buffers the skipped launches would have left (accumulators, drives, NMDA
+// bits of step 0) are overwritten by step 1.
 static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn, bool stats,
                           hipError_t* err) {
-    hipError_t e = axon::launch_clear(n->d, n->stream);
+    const bool first = !stats && T >= 2u;
+    hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {  // mm:837-884
         P.step = t;
         P.learn = learn ? 1u : 0u;
@@ -382,6 +388,10 @@ static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps
         P.phase = (!learn || !n->contrast) ? 0u
                   : (t < clampSteps)      ? 1u
                   : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
+        if (first && t == 0u) {
+            e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, nullptr, nullptr, true);
+            continue;
+        }
         e = axon::launch_drive(n->d, P, t, n->stream);
         if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream);
         if (e == hipSuccess && learn && P.phase != 3u) e = axon::launch_learn(n->d, P, t, stats, n->stream);

@@ -148,8 +148,10 @@ hipError_t launch_clear(const NetDev& d, hipStream_t s);
 // e0 / e1 (optional): timing events recorded at the start / end of the kernel
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
+// first: step 0 of a presentation without side statistics, which needs no
+// clear, propagation or plasticity launch (k_emit_first)
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
-                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool first = false);
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s,
                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 

@@ -1062,6 +1062,37 @@ __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_
     emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, gate);
 }
 
+// The first timestep of a presentation without side statistics.  Its
+// propagation would read the cleared spikes: every accumulator, NMDA bit
+// and drive is 0, so no column passes the gate (drive >= 1) and only clamped
+// input columns spike, with their stimulus word and no voted tile (no NMDA
+// bits).  This kernel emits that step directly -- spikes, history row,
+// margin and spiking-column flags, the bias update -- and clears the
+// plasticity list counts of both step parities, in place of the clear, the
+// propagation and the (empty) plasticity launches.  The previous step's
+// spikes are not read.
+template <bool LEARN>
+__global__ __launch_bounds__(1024) void k_emit_first(NetDev d, Params P, uint32_t* __restrict__ spkOut) {
+    if (blockIdx.x == 0u && threadIdx.x < 2u * NLISTS)
+        *list_count(d, threadIdx.x / NLISTS, threadIdx.x % NLISTS) = 0u;
+    const uint32_t col = emit_col(blockIdx.x, threadIdx.x >> 6);
+    if (col >= d.ncol) return;
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const uint32_t gid = col * NPC + n;
+    const bool rec = col >= P.inCols;
+    const uint32_t b = (P.clamp != 0u && col < P.inCols) ? d.stim[col] : 0u;
+    if (lane == 0u) {
+        spkOut[col] = b;
+        d.marg[col] = 0u;  // all 32 accumulators are 0
+        d.colNZ[col] = b != 0u ? 1u : 0u;
+        if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
+    }
+    if (LEARN && lane < 32u && rec) {  // k_bias (recurrent columns never spike here)
+        const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
+        d.bias[gid] = (uint32_t)max(0, (int)d.bias[gid] - dec);
+    }
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_learn(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
                                                const uint32_t* __restrict__ spkOut) {
@@ -2047,10 +2078,14 @@ hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats
 }
 
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
-                       hipEvent_t e0, hipEvent_t e1) {
+                       hipEvent_t e0, hipEvent_t e1, bool first) {
     const uint32_t* in = d.spk[t & 1u];
     uint32_t* out = d.spk[(t & 1u) ^ 1u];
     const dim3 g(blocks_for(d.ncol, 128) * 8u), b(1024);  // 16 columns per workgroup, XCD-aligned (emit_col)
+    if (first) {  // step 0 without side statistics (k_emit_first)
+        if (learn) return launch(k_emit_first<true>, g, b, 0u, s, e0, e1, d, P, out);
+        return launch(k_emit_first<false>, g, b, 0u, s, e0, e1, d, P, out);
+    }
     if (learn && stats) return launch(k_emit<true, true>, g, b, 0u, s, e0, e1, d, P, in, out);
     if (learn) return launch(k_emit<true, false>, g, b, 0u, s, e0, e1, d, P, in, out);
     if (stats) return launch(k_emit<false, true>, g, b, 0u, s, e0, e1, d, P, in, out);
