@@ -1250,11 +1250,13 @@ static axon_status multi_create(axon_net* const* nets, uint32_t nnet, axon_multi
 // One presentation of every scheduled network of a group (mm:837-884 with
 // learn = 1), followed by the group's round counter advance.
 static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev& m, hipStream_t s) {
-    hipError_t e = axon::launch_multi_begin(m, b->ncol, b->nstats, s);
     Params P = b->P;
     P.facil = 0u;
     P.useBias = 1u;
     const uint32_t T = b->T, clampSteps = b->trainClamp;
+    // with two or more steps, step 0 (spikes start cleared) is one launch that also does the begin work
+    const bool first = T >= 2u;
+    hipError_t e = first ? hipSuccess : axon::launch_multi_begin(m, b->ncol, b->nstats, s);
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
         P.step = t;
         P.learn = 1u;
@@ -1262,6 +1264,10 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
         P.clamp = (t < clampSteps) ? 1u : 0u;
         P.depress = (t % 2u == 1u) ? 1u : 0u;
         P.phase = !b->contrast ? 0u : (t < clampSteps) ? 1u : (((t - clampSteps) % b->ltdEvery == 0u) ? 2u : 3u);
+        if (first && t == 0u) {
+            e = axon::launch_multi_emit_first(m, P, true, b->ncol, s);
+            continue;
+        }
         e = axon::launch_multi_drive(m, P, t, b->ncol, s);
         if (e == hipSuccess) e = axon::launch_multi_topk(m, P, s);
         if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, s);

@@ -127,6 +127,9 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
                              hipStream_t s);
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s);
+// timestep 0 of a round together with its begin work (k_mb_emit_first); then
+// launch_multi_begin and step 0's other launches are not needed
+hipError_t launch_multi_emit_first(const MultiDev& m, const Params& P, bool learn, uint32_t ncol, hipStream_t s);
 // advances slots [0, nact) (nact <= nslots; the others must be idle);
 // wavesPerCU (0: no limit) caps the propagation's resident waves per CU
 hipError_t launch_probe_step(const ProbeDev& d, uint32_t tile, uint32_t nact, uint32_t wavesPerCU, hipStream_t s);

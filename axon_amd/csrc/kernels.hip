@@ -1959,6 +1959,40 @@ __global__ __launch_bounds__(64) void k_mb_learn(MultiDev m, Params P, uint32_t 
     learn_tiles(d, P, (t & 1u) ? s1 : s0, blockIdx.x, gridDim.x);
 }
 
+// The first timestep of a presentation for many networks (the batched form
+// of k_emit_first, which the network's own begin work joins): this round's
+// pattern is read from the schedule and stored in the network's descriptor
+// for the launches that follow, the pattern's clamp row becomes the
+// network's stimulus, the plasticity list counts are cleared, and the step
+// is emitted directly: only clamped input columns spike, every accumulator
+// and drive being 0.
+template <bool LEARN>
+__global__ __launch_bounds__(256) void k_mb_emit_first(MultiDev m, Params P) {
+    const uint32_t net = blockIdx.y;
+    const uint32_t p = mb_pattern(m, net);
+    if (blockIdx.x == 0u && threadIdx.x == 0u) const_cast<NetDev*>(m.nets)[net].active = p;
+    if (p == 0xFFFFFFFFu) return;
+    const NetDev d = m.nets[net];  // its `active` may be the last round's; not read here
+    if (blockIdx.x == 0u && threadIdx.x < 2u * NLISTS) *list_count(d, threadIdx.x / NLISTS, threadIdx.x % NLISTS) = 0u;
+    const uint32_t lane = lane_id(), h = lane >> 5, n = lane & 31u;
+    const uint32_t col = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u + h;  // 8 columns per workgroup
+    if (col >= d.ncol) return;
+    const uint32_t sw = reinterpret_cast<const uint32_t*>(m.stimTabs[net])[(size_t)p * d.ncol + col];
+    const uint32_t b = (P.clamp != 0u && col < P.inCols) ? sw : 0u;
+    if (n == 0u) {
+        d.stim[col] = sw;  // read by the clamped steps that follow
+        d.spk[1][col] = b;
+        d.marg[col] = 0u;
+        d.colNZ[col] = b != 0u ? 1u : 0u;
+        if (P.histOn != 0u) d.hist[(size_t)P.step * d.ncol + col] = b;
+    }
+    if (LEARN && col >= P.inCols) {  // k_bias (recurrent columns never spike here)
+        const uint32_t gid = col * NPC + n;
+        const int dec = ((P.step % max(P.biasDownEvery, 1u)) == 0u) ? (int)P.biasDown : 0;
+        d.bias[gid] = (uint32_t)max(0, (int)d.bias[gid] - dec);
+    }
+}
+
 __global__ void k_mb_end(MultiDev m) {
     if (threadIdx.x == 0u) *m.round += 1u;
 }
@@ -2243,6 +2277,11 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
 }
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
     k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 256u) * 4u, m.nnet), 64, 0, s>>>(m, P, t);
+    return hipGetLastError();
+}
+hipError_t launch_multi_emit_first(const MultiDev& m, const Params& P, bool learn, uint32_t ncol, hipStream_t s) {
+    if (learn) k_mb_emit_first<true><<<dim3((ncol + 7u) / 8u, m.nnet), 256, 0, s>>>(m, P);
+    else k_mb_emit_first<false><<<dim3((ncol + 7u) / 8u, m.nnet), 256, 0, s>>>(m, P);
     return hipGetLastError();
 }
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s) {

@@ -37,8 +37,9 @@ def compare_present(g, o, T, tag, net=None):
     assert np.array_equal(r.stats[:1024], o.buf("STATS")[:1024]), f"{tag}: stats"
     assert np.array_equal(r.drive_log[:256], o.buf("DRIVELOG")[:256]), f"{tag}: driveLog"
     assert np.array_equal(r.theta, o.buf("THETA")[:2]), f"{tag}: theta"
-    if net is not None:  # the last step's accumulators (16- or 32-bit on the device)
+    if net is not None:  # the last step's accumulators (16- or 32-bit on the device) and the stimulus
         assert np.array_equal(net.read("ACC"), o.buf("ACC")), f"{tag}: acc"
+        assert np.array_equal(net.read("STIM"), o.buf("STIM")), f"{tag}: stim"
 
 
 def compare_state(net, ora, tag, planes=True):
