@@ -257,7 +257,7 @@ class Net:
         _check(lib().axon_net_set_profiling(self._h, ctypes.c_uint32(int(enable))))
 
     def kernel_times(self, reset: bool = False):
-        """Summed device ms and launch counts of (propagation, unused, top-K + emit, plasticity)."""
+        """Summed device ms and launch counts of (propagation, first-step launch, top-K + emit, plasticity)."""
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_uint64 * 4)()
         _check(lib().axon_kernel_times(self._h, ms, n, ctypes.c_uint32(int(reset))))

@@ -411,7 +411,8 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         n->prof_ev.push_back(e);
     }
     std::vector<uint8_t> plast(T, 0);  // steps that launch the plasticity kernel
-    hipError_t e = axon::launch_clear(n->d, n->stream);
+    const bool first = !stats && T >= 2u;  // as enqueue_steps
+    hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
         P.step = t;
         P.learn = learn ? 1u : 0u;
@@ -422,6 +423,10 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
                   : (t < clampSteps)      ? 1u
                   : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
         hipEvent_t* ev = &n->prof_ev[(size_t)8 * t];
+        if (first && t == 0u) {
+            e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, ev[2], ev[3], true);
+            continue;
+        }
         plast[t] = learn && P.phase != 3u;
         if ((e = axon::launch_drive(n->d, P, t, n->stream, ev[0], ev[1])) != hipSuccess) break;
         if ((e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, ev[4], ev[5])) != hipSuccess) break;
@@ -431,8 +436,9 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
     for (uint32_t t = 0; t < T; ++t) {
+        const bool one = first && t == 0u;  // the first-step launch alone
         for (int k = 0; k < 4; ++k) {
-            if (k == 1 || (k == 3 && !plast[t])) continue;  // top-K runs inside emit; no plasticity this step
+            if (one != (k == 1) || (k == 3 && !plast[t])) continue;
             float ms = 0.f;
             if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)8 * t + 2 * k], n->prof_ev[(size_t)8 * t + 2 * k + 1])) != hipSuccess)
                 return e;

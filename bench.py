@@ -58,10 +58,14 @@ def acc_bytes(cfg):
     return 2 if 32 * per_branch + cfg.facil <= 0xFFFF else 4
 
 
-def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2):
+def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0):
     """Algorithmic HBM bytes per kernel over `steps` learning timesteps
     (bias=False: readout steps without the intrinsic bias, which neither read
-    nor update it; aw: bytes per accumulator, acc_bytes).
+    nor update it; aw: bytes per accumulator, acc_bytes; firsts: how many of
+    the steps are the first of a presentation without side statistics, one
+    launch that reads the stimulus (4 B/column) and writes the spike,
+    history, margin and spiking flag (16 B/column) and the bias
+    read-modify-write (8 B/neuron), with no propagation).
 
     The synapse planes are word-major tiles, so a kernel touches only the
     128-byte lines of presynaptic columns that spiked (device line counters).
@@ -75,17 +79,19 @@ def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2):
     plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read + written)."""
     n = ncol * 32
     bb = 1 if bias else 0
-    prop = 128 * prop_lines + steps * (128 * ncol + 4 * bb * n + (aw + 4) * n + 4 * ncol + 32 * ncol)
-    emit = steps * (aw * n + 8 * bb * n + 20 * ncol)
+    full = steps - firsts
+    prop = 128 * prop_lines + full * (128 * ncol + 4 * bb * n + (aw + 4) * n + 4 * ncol + 32 * ncol)
+    emit = full * (aw * n + 8 * bb * n + 20 * ncol)
+    first = firsts * (8 * bb * n + 20 * ncol)
     plast = 1024 * learn_lines
-    return {"propagation": prop, "topk_emit": emit, "plasticity": plast}
+    return {"propagation": prop, "first_emit": first, "topk_emit": emit, "plasticity": plast}
 
 
-def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2):
+def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0):
     """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
     timesteps and device line counters) over the run's wall time: the average
     rate of the whole job, idle launches and gaps included."""
-    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw)
+    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts)
     tot = sum(by.values())
     gbps = tot / seconds / 1e9
     return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
@@ -208,8 +214,9 @@ def gpu_bench(args):
     twin.close()
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
-    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg))
-    names = ["propagation", "unused", "topk_emit", "plasticity"]  # the column top-K runs inside emit
+    # training presentations are asynchronous: each one's first step is one launch (first_emit)
+    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg), firsts=args.steps)
+    names = ["propagation", "first_emit", "topk_emit", "plasticity"]  # the column top-K runs inside emit
     kern = {}
     for i, nm in enumerate(names):
         if launches[i] == 0:
@@ -229,7 +236,7 @@ def gpu_bench(args):
         "ms_per_step": 1e3 * dt / args.steps,
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
-        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg)),
+        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg), firsts=args.steps),
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dn),
@@ -366,9 +373,9 @@ def c2_networks(args):
             "batched_gsyn_s": round(SYN_PER_TILE * tiles / (ms / 1e3) / 1e9, 1),
             "batched_plasticity_gsyn_s": round(SYN_PER_TILE * learn / (ms / 1e3) / 1e9, 1),
             "plain_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines_plain, t_plain,
-                                       aw=acc_bytes(mk(*specs[0]))),
+                                       aw=acc_bytes(mk(*specs[0])), firsts=sum(ep * P + 1 for P, _ in specs)),
             "batched_hbm": whole_run_hbm(ncol, T * sum(ep * P + 1 for P, _ in specs), lines, ms / 1e3,
-                                         aw=acc_bytes(mk(*specs[0]))),
+                                         aw=acc_bytes(mk(*specs[0])), firsts=sum(ep * P + 1 for P, _ in specs)),
             "bit_exact_vs_plain": bool(exact and tiles == tiles_plain)}
 
 

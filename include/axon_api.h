@@ -167,8 +167,10 @@ axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t reset);
  * and wait for the device before returning (an AXON_PRESENT_ASYNC
  * presentation still skips its side statistics and outputs);
  * axon_kernel_times returns summed device ms and launch counts for
- * [0] propagation, [1] unused (always 0: the column top-K runs inside emit),
- * [2] column top-K + emit (k-WTA, bias), [3] plasticity. */
+ * [0] propagation, [1] the first timestep of a presentation without side
+ * statistics (one launch in place of the clear, the propagation and the
+ * plasticity of that step), [2] column top-K + emit (k-WTA, bias),
+ * [3] plasticity. */
 axon_status axon_net_set_profiling(axon_net* n, uint32_t enable);
 axon_status axon_kernel_times(axon_net* n, double ms[4], uint64_t launches[4], uint32_t reset);
 
