@@ -253,6 +253,14 @@ class Net:
         _check(lib().axon_line_counters(self._h, out, ctypes.c_uint32(int(reset))))
         return int(out[0]), int(out[1])
 
+    def last_step_counters(self):
+        """(tiles, word lines) of the plasticity counts done by last-timestep plasticity
+        launches (presentations without side statistics; the other timesteps' plasticity
+        runs inside the next propagation launch)."""
+        out = (ctypes.c_uint64 * 2)()
+        _check(lib().axon_last_step_counters(self._h, out))
+        return int(out[0]), int(out[1])
+
     def set_profiling(self, enable: bool) -> None:
         _check(lib().axon_net_set_profiling(self._h, ctypes.c_uint32(int(enable))))
 

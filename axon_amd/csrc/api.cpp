@@ -320,8 +320,7 @@ static axon_status net_create(const axon_config* c, axon_net** out) {
         {(uint32_t**)&d.firedW, (size_t)n->NCOL * axon::NSLOT / 4}, {(uint32_t**)&d.tileBand, (size_t)n->NCOL * axon::NSLOT / 4},
         {(uint32_t**)&d.colNZ, (n->NCOL + 16) / 4},
         {&d.learnList, (size_t)2 * axon::NLISTS * axon::list_cols(n->NCOL)},
-        {(uint32_t**)&d.tileList, (size_t)2 * axon::NLISTS * axon::list_cols(n->NCOL) * axon::NSLOT * 4},
-        {&d.learnCnt, 2 * axon::NLISTS * 32},
+        {&d.learnCnt, 2 * axon::NLISTS * 32}, {&d.cntLast, (size_t)n->NCOL * 2},
     };
     for (auto& b : bufs) {
         if ((st = dev_alloc(n, &p, b.words * 4)) != AXON_OK) return cleanup(st);
@@ -370,37 +369,63 @@ static axon_status net_create(const axon_config* c, axon_net** out) {
 }
 
 // ================================================================== present
-// Without side statistics, step 0 of a presentation of two or more steps is
-// one launch (k_emit_first: the spikes start cleared, so that step's
-// propagation yields zeros and its plasticity has no voted tile); the
-// buffers the skipped launches would have left (accumulators, drives, NMDA
-// bits of step 0) are overwritten by step 1.
-static void enqueue_steps(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn, bool stats,
-                          hipError_t* err) {
-    const bool first = !stats && T >= 2u;
-    hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
-    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {  // mm:837-884
-        P.step = t;
-        P.learn = learn ? 1u : 0u;
-        P.freeze = learn ? 0u : 1u;
-        P.clamp = (t < clampSteps) ? 1u : 0u;
-        P.depress = (learn && (t % 2u == 1u)) ? 1u : 0u;
-        P.phase = (!learn || !n->contrast) ? 0u
-                  : (t < clampSteps)      ? 1u
-                  : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
-        if (first && t == 0u) {
-            e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, nullptr, nullptr, true);
-            continue;
-        }
-        e = axon::launch_drive(n->d, P, t, n->stream);
-        if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream);
-        if (e == hipSuccess && learn && P.phase != 3u) e = axon::launch_learn(n->d, P, t, stats, n->stream);
-    }
-    *err = e;
+// The phase and the other per-step fields of step t (mm:837-884).
+static Params step_params(const axon_net* n, Params P, uint32_t t, uint32_t clampSteps, bool learn) {
+    P.step = t;
+    P.learn = learn ? 1u : 0u;
+    P.freeze = learn ? 0u : 1u;
+    P.clamp = (t < clampSteps) ? 1u : 0u;
+    P.depress = (learn && (t % 2u == 1u)) ? 1u : 0u;
+    P.phase = (!learn || !n->contrast) ? 0u
+              : (t < clampSteps)      ? 1u
+              : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
+    return P;
 }
 
-// Same launches as enqueue_steps, each carrying start / end timing events,
-// so the kernels' device durations can be summed after the presentation.
+// The launches of one presentation.  Without side statistics, step 0 of a
+// presentation of two or more steps is one launch (k_emit_first: the spikes
+// start cleared, so that step's propagation yields zeros and its plasticity
+// has no voted tile; the buffers the skipped launches would have left --
+// accumulators, drives, NMDA bits of step 0 -- are overwritten by step 1),
+// and the plasticity of step t runs inside step t + 1's propagation, the
+// last step's in a launch of its own.  With side statistics every step is
+// clear/propagation/emit/plasticity.  ev (optional): 8 timing events per
+// step -- propagation, first-step emit, emit, plasticity -- and plast[t]
+// records which steps launched the plasticity kernel.
+static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t clampSteps, bool learn, bool stats,
+                                hipEvent_t* ev = nullptr, uint8_t* plast = nullptr) {
+    const bool first = !stats && T >= 2u;
+    hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
+    Params PL{};
+    bool pending = false;  // the previous step's plasticity waits for this step's propagation
+    auto E = [&](uint32_t t, int k) -> hipEvent_t { return ev ? ev[(size_t)8 * t + k] : nullptr; };
+    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
+        const Params P = step_params(n, P0, t, clampSteps, learn);
+        if (first && t == 0u) {
+            e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, E(t, 2), E(t, 3), true);
+            continue;
+        }
+        e = axon::launch_drive(n->d, P, t, n->stream, E(t, 0), E(t, 1), pending ? &PL : nullptr);
+        pending = false;
+        if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, E(t, 4), E(t, 5));
+        if (e != hipSuccess || !learn || P.phase == 3u) continue;
+        if (stats) {
+            e = axon::launch_learn(n->d, P, t, true, n->stream, E(t, 6), E(t, 7));
+            if (plast) plast[t] = 1u;
+        } else {
+            PL = P;
+            pending = true;
+        }
+    }
+    if (e == hipSuccess && pending) {
+        e = axon::launch_learn(n->d, PL, T - 1u, false, n->stream, E(T - 1u, 6), E(T - 1u, 7));
+        if (plast) plast[T - 1u] = 1u;
+    }
+    return e;
+}
+
+// The launches of enqueue_steps, each carrying start / end timing events, so
+// the kernels' device durations can be summed after the presentation.
 static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn,
                                          bool stats) {
     const size_t need = (size_t)8 * std::max<uint32_t>(T, 1);
@@ -410,35 +435,15 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         if (r != hipSuccess) return r;
         n->prof_ev.push_back(e);
     }
-    std::vector<uint8_t> plast(T, 0);  // steps that launch the plasticity kernel
-    const bool first = !stats && T >= 2u;  // as enqueue_steps
-    hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
-    for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
-        P.step = t;
-        P.learn = learn ? 1u : 0u;
-        P.freeze = learn ? 0u : 1u;
-        P.clamp = (t < clampSteps) ? 1u : 0u;
-        P.depress = (learn && (t % 2u == 1u)) ? 1u : 0u;
-        P.phase = (!learn || !n->contrast) ? 0u
-                  : (t < clampSteps)      ? 1u
-                  : (((t - clampSteps) % n->ltdEvery == 0u) ? 2u : 3u);
-        hipEvent_t* ev = &n->prof_ev[(size_t)8 * t];
-        if (first && t == 0u) {
-            e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, ev[2], ev[3], true);
-            continue;
-        }
-        plast[t] = learn && P.phase != 3u;
-        if ((e = axon::launch_drive(n->d, P, t, n->stream, ev[0], ev[1])) != hipSuccess) break;
-        if ((e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, ev[4], ev[5])) != hipSuccess) break;
-        if (!plast[t]) continue;
-        e = axon::launch_learn(n->d, P, t, stats, n->stream, ev[6], ev[7]);
-    }
+    std::vector<uint8_t> plast(T, 0);  // steps that launched the plasticity kernel
+    hipError_t e = enqueue_steps(n, P, T, clampSteps, learn, stats, n->prof_ev.data(), plast.data());
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
+    const bool first = !stats && T >= 2u;  // as enqueue_steps
     for (uint32_t t = 0; t < T; ++t) {
         const bool one = first && t == 0u;  // the first-step launch alone
         for (int k = 0; k < 4; ++k) {
-            if (one != (k == 1) || (k == 3 && !plast[t])) continue;
+            if (k == 3 ? !plast[t] : (one != (k == 1))) continue;
             float ms = 0.f;
             if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)8 * t + 2 * k], n->prof_ev[(size_t)8 * t + 2 * k + 1])) != hipSuccess)
                 return e;
@@ -494,8 +499,7 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         if (it == n->graphs.end()) {
             hipGraph_t g = nullptr;
             HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = hipSuccess;
-            enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, &e);
+            const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async);
             hipError_t e2 = hipStreamEndCapture(n->stream, &g);
             if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "capture launch: %s", hipGetErrorString(e));
             if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -507,8 +511,7 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         }
         HIPCHECK(hipGraphLaunch(it->second, n->stream));
     } else {
-        hipError_t e = hipSuccess;
-        enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, &e);
+        const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
     }
     if (async) {
@@ -686,36 +689,45 @@ static axon_status counter_pairs(axon_net* n, uint32_t* dev, uint64_t out[2], ui
     return AXON_OK;
 }
 
-extern "C" axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset) {
-    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+// The per-column work counter pairs: propagation (cntProp) and plasticity,
+// whose last-timestep launches count apart (cntLast) from the rest (cntLearn).
+static axon_status work_pairs(axon_net* n, uint64_t prop[2], uint64_t learn[2], uint64_t last[2], uint32_t reset) {
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
-    uint64_t p[2], l[2];
-    axon_status st = counter_pairs(n, n->d.cntProp, p, 0);
-    if (st == AXON_OK) st = counter_pairs(n, n->d.cntLearn, l, 0);
+    axon_status st = counter_pairs(n, n->d.cntProp, prop, 0);
+    if (st == AXON_OK) st = counter_pairs(n, n->d.cntLearn, learn, 0);
+    if (st == AXON_OK) st = counter_pairs(n, n->d.cntLast, last, 0);
     if (st != AXON_OK) return st;
-    out[0] = p[0]; out[1] = l[0];
     if (reset) {
         HIPCHECK(zero(n, n->d.cntProp, (size_t)n->NCOL * 8));
         HIPCHECK(zero(n, n->d.cntLearn, (size_t)n->NCOL * 8));
+        HIPCHECK(zero(n, n->d.cntLast, (size_t)n->NCOL * 8));
     }
+    return AXON_OK;
+}
+
+extern "C" axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    uint64_t p[2], l[2], z[2];
+    const axon_status st = work_pairs(n, p, l, z, reset);
+    if (st != AXON_OK) return st;
+    out[0] = p[0]; out[1] = l[0] + z[0];
     return AXON_OK;
 }
 
 extern "C" axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t reset) {
     if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
-    HIPCHECK(hipSetDevice(n->device));
-    HIPCHECK(hipStreamSynchronize(n->stream));
-    uint64_t p[2], l[2];
-    axon_status st = counter_pairs(n, n->d.cntProp, p, 0);
-    if (st == AXON_OK) st = counter_pairs(n, n->d.cntLearn, l, 0);
+    uint64_t p[2], l[2], z[2];
+    const axon_status st = work_pairs(n, p, l, z, reset);
     if (st != AXON_OK) return st;
-    out[0] = p[1]; out[1] = l[1];
-    if (reset) {
-        HIPCHECK(zero(n, n->d.cntProp, (size_t)n->NCOL * 8));
-        HIPCHECK(zero(n, n->d.cntLearn, (size_t)n->NCOL * 8));
-    }
+    out[0] = p[1]; out[1] = l[1] + z[1];
     return AXON_OK;
+}
+
+extern "C" axon_status axon_last_step_counters(axon_net* n, uint64_t out[2]) {
+    if (!n || !out) return fail(AXON_ERR_INVALID_ARG, "null argument");
+    uint64_t p[2], l[2];
+    return work_pairs(n, p, l, out, 0);
 }
 
 extern "C" axon_status axon_net_set_profiling(axon_net* n, uint32_t enable) {
@@ -1254,7 +1266,9 @@ static axon_status multi_create(axon_net* const* nets, uint32_t nnet, axon_multi
 }
 
 // One presentation of every scheduled network of a group (mm:837-884 with
-// learn = 1), followed by the group's round counter advance.
+// learn = 1), followed by the group's round counter advance.  As in
+// enqueue_steps, step t's plasticity runs inside step t + 1's propagation and
+// the last step's in a launch of its own.
 static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev& m, hipStream_t s) {
     Params P = b->P;
     P.facil = 0u;
@@ -1263,6 +1277,8 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
     // with two or more steps, step 0 (spikes start cleared) is one launch that also does the begin work
     const bool first = T >= 2u;
     hipError_t e = first ? hipSuccess : axon::launch_multi_begin(m, b->ncol, b->nstats, s);
+    Params PL{};
+    bool pending = false;
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
         P.step = t;
         P.learn = 1u;
@@ -1274,11 +1290,13 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
             e = axon::launch_multi_emit_first(m, P, true, b->ncol, s);
             continue;
         }
-        e = axon::launch_multi_drive(m, P, t, b->ncol, s);
+        e = axon::launch_multi_drive(m, P, t, b->ncol, s, pending ? &PL : nullptr);
         if (e == hipSuccess) e = axon::launch_multi_topk(m, P, s);
         if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, s);
-        if (e == hipSuccess && P.phase != 3u) e = axon::launch_multi_learn(m, P, t, b->ncol, s);
+        pending = P.phase != 3u;
+        PL = P;
     }
+    if (e == hipSuccess && pending) e = axon::launch_multi_learn(m, PL, T - 1u, b->ncol, s);
     if (e == hipSuccess) e = axon::launch_multi_end(m, s);
     return e;
 }

@@ -64,16 +64,16 @@ struct NetDev {
     // band clip, cleared by host writes of the planes).
     uint8_t* firedW;     // ncol * 32
     uint8_t* tileBand;   // ncol * 32
-    // Plasticity work of step t, in list t & 1: the learning columns when side
-    // statistics are on (learnList), else the voted tiles (tileList entries
-    // {column * 32 + slot, presynaptic group, learning-neuron mask, flags};
-    // flags = fired-column mask of the group | 0x100 when all 8 words of
-    // the tile must be rewritten).
-    // Each parity holds 8 lists (column c in list (c / 4) % 8) so that
-    // columns take list slots on 8 different counters.
+    // Plasticity work of step t with side statistics: the learning columns, in
+    // list t & 1.  Each parity holds 8 lists (column c in list (c / 4) % 8) so
+    // that columns take list slots on 8 different counters.  (Without side
+    // statistics plasticity needs no list: the next propagation rewrites its
+    // own columns' voted tiles.)
     uint32_t* learnList; // [2][8][list_cols(ncol)]
-    uint4* tileList;     // [2][8][list_cols(ncol) * 32]
     uint32_t* learnCnt;  // [2][8] list lengths, 32 words apart (reset by the step after next's propagation)
+    // The part of cntLearn's work done by the last-timestep plasticity launch
+    // (k_learn_last): per column (tiles, lines); the work counters add it in.
+    uint32_t* cntLast;   // ncol * 2
     uint32_t ncol, incols, inbr, topkpar;
     // active: batched training only -- this round's pattern of the network
     // (~0 = idle), written by k_mb_begin into the batch's device copy of the
@@ -121,7 +121,10 @@ struct MultiDev {
 
 // ---- launchers (kernels.hip) ; all asynchronous on `s` ----
 hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats, hipStream_t s);
-hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
+// PL (optional): the parameters of step t - 1, whose plasticity the launch
+// applies first (deferred plasticity); the last step's takes launch_multi_learn
+hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s,
+                              const Params* PL = nullptr);
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s);
 hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
                              hipStream_t s);
@@ -148,13 +151,18 @@ hipError_t launch_probe_gather(const ProbeDev& d, const uint32_t* slots, const u
                                uint32_t* out, hipStream_t s);
 hipError_t launch_init(const NetDev& d, uint32_t nand, uint32_t seed, hipStream_t s);
 hipError_t launch_clear(const NetDev& d, hipStream_t s);
-// e0 / e1 (optional): timing events recorded at the start / end of the kernel
+// e0 / e1 (optional): timing events recorded at the start / end of the kernel.
+// PL (optional): the parameters of step t - 1, whose plasticity (without side
+// statistics) the propagation applies first to its columns' voted tiles.
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr);
+                        hipEvent_t e1 = nullptr, const Params* PL = nullptr);
 // first: step 0 of a presentation without side statistics, which needs no
 // clear, propagation or plasticity launch (k_emit_first)
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool first = false);
+// Plasticity of step t: with side statistics over the learning-column lists
+// (k_learn); without, the last step of a presentation (k_learn_last), the
+// others being deferred into the next propagation (launch_drive's PL).
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s,
                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 

@@ -96,13 +96,10 @@ __device__ __forceinline__ uint32_t half_or(uint32_t x) {
     return x;
 }
 
-// Plasticity work lists: per step parity, NLISTS lists, so the list-slot
-// atomics of different columns hit different counters; each counter sits on
-// its own 128-byte line.  Column c goes to list (c / 4) % 8: workgroups are
-// dealt to the 8 XCDs in turn and propagation runs 4 columns per workgroup,
-// so that is the XCD that propagated the column, and list l is rewritten by
-// workgroups on XCD l (its W3 lines and spike words are in that XCD's L2).
-// A list holds at most list_cols(ncol) columns.
+// Learning-column lists of the side-statistics path: per step parity, NLISTS
+// lists, so the list-slot atomics of different columns hit different
+// counters; each counter sits on its own 128-byte line.  Column c goes to
+// list (c / 4) % 8.  A list holds at most list_cols(ncol) columns.
 __device__ __forceinline__ uint32_t* list_count(const NetDev& d, uint32_t par, uint32_t l) {
     return d.learnCnt + (par * NLISTS + l) * 32u;
 }
@@ -569,26 +566,17 @@ constexpr uint32_t FIRED_LDS_COLS = 16384;  // up to this many columns the fired
 // Dynamic LDS bytes of the fired-byte stage (0: read from global memory).
 inline uint32_t fired_lds_bytes(uint32_t ncol) { return ncol <= FIRED_LDS_COLS ? (ncol + 15u) / 16u * 16u : 0u; }
 
-// Stage the fired bytes of all columns in LDS while this wave's conn entries load.
-__device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds) {
-    if (d.ncol > FIRED_LDS_COLS) return nullptr;
+// Stage the fired bytes of all columns in LDS while this wave's conn entries
+// load.  sync: end with a workgroup barrier even when nothing is staged.
+__device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds, bool sync = false) {
+    if (d.ncol > FIRED_LDS_COLS) {  // uniform over the grid
+        if (sync) __syncthreads();
+        return nullptr;
+    }
     const uint4* src = reinterpret_cast<const uint4*>(d.colNZ);
     for (uint32_t i = threadIdx.x; i < (d.ncol + 15u) / 16u; i += blockDim.x) lds[i] = src[i];
     __syncthreads();
     return reinterpret_cast<const uint8_t*>(lds);
-}
-
-template <int MODE>
-__global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
-    extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
-    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
-    // the learning-column list of step t-1 has been consumed; free it for step t+1
-    if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
-    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const DriveIn in = drive_inputs(d, spkIn, min(col, d.ncol - 1u));
-    const uint8_t* f = stage_fired(d, fired);
-    if (col >= d.ncol) return;
-    drive_col<MODE>(d, P, spkIn, col, in, f, items[threadIdx.x >> 6]);
 }
 
 // ============================================================== column top-K
@@ -706,22 +694,14 @@ __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__
 
 // ============================================================== readout + plasticity
 // k_emit (mm:192-236) and k_bias (mm:240-253) fused: everything after the
-// global theta is local to a column.  A workgroup owns 4 columns, one wave
+// global theta is local to a column.  A workgroup owns 16 columns, one wave
 // each.  The k-of-32 selection is computed only where a spike can leave (gate
 // passed or input clamped); every other column is silent, and its margin
-// flag -- which k_learn reads only for spiking columns -- is written as 0.
+// flag -- which plasticity reads only for spiking columns -- is written as 0.
 // The wave also records whether its column fired (read by the next
-// propagation) and whether it learns this step (read by k_learn).
+// propagation).
 // STATS = false drops the side outputs (stats and drive-log atomics, LTP/LTD
 // counters); spikes, history, margin, bias and synapses are unaffected.
-// Record a learning column's plasticity work for k_learn.  Branch slot s of
-// the column is voted (mm:446-452) when some spiking neuron has its slot-s
-// NMDA bit set; only voted tiles are rewritten.  Without side statistics each
-// voted tile becomes one list entry, carrying its presynaptic group and the
-// mask of neurons whose slot-s branch learns, so that k_learn spends one wave
-// per tile with no further lookups.  With statistics the column itself is
-// listed (the per-neuron LTP/LTD totals need all of a column's tiles in one
-// workgroup).
 // Whether a voted tile needs all 8 words rewritten: the update touches words
 // of silent presynaptic columns (the legacy rule with depression, hetero-
 // synaptic LTD), or the band clip may still move counters the tile's spikes
@@ -729,51 +709,39 @@ __global__ __launch_bounds__(1024) void k_topk_test(const uint32_t* __restrict__
 __device__ __forceinline__ bool full_tile_update(const Params& P, uint32_t band) {
     return P.phase == 0u || (P.phase == 1u && P.hetero != 0u) || (P.bandHi != 0u && band == 0u);
 }
+// Whether a column learns at a learning timestep of parameters P (mm:440-455):
+// plasticity is on in this phase, the column spiked (spike word b) and, with
+// margin gating, its k-of-32 margin flag is set.
+__device__ __forceinline__ bool learns(const Params& P, uint32_t b, uint32_t marg) {
+    return P.phase != 3u && b != 0u && !(P.marginGate != 0u && marg == 0u);
+}
 
-template <bool STATS>
-__device__ __forceinline__ void list_plasticity(const NetDev& d, const Params& P, uint32_t col, uint32_t bbv,
-                                                uint32_t cn, uint32_t fwb, uint32_t band, uint2 lc) {
+// With side statistics a learning column is listed for k_learn, which
+// rewrites all of the column's voted tiles in one workgroup (the per-neuron
+// LTP/LTD totals need them together); its work counters are added here.
+__device__ __forceinline__ void list_learning_column(const NetDev& d, const Params& P, uint32_t col,
+                                                     uint32_t bbv, uint32_t fwb, uint32_t band, uint2 lc) {
     const uint32_t lane = lane_id();
     const uint32_t par = P.step & 1u;
-    if (STATS) {
-        const uint32_t voted = half_or(bbv);
-        const bool v = lane < 32u && ((voted >> lane) & 1u) != 0u;
-        const uint32_t lines = wave_sum(v ? (full_tile_update(P, band) ? 8u : popc(fwb)) : 0u);
-        if (lane == 0u) {
-            reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
-            const uint32_t l = list_of(col), cap = list_cols(d.ncol);
-            d.learnList[(par * NLISTS + l) * cap + atomicAdd(list_count(d, par, l), 1u)] = col;
-        }
-        return;
-    }
-    // lane s < 32: the neurons whose slot-s branch learns (the NMDA bits transposed)
-    const uint32_t tr = half_transpose32(bbv);
-    const uint32_t want = lane < 32u ? tr : 0u;
-    const uint32_t voted = ballot32(want != 0u);
-    const bool full = full_tile_update(P, band);
-    const uint32_t lines = wave_sum(want != 0u ? (full ? 8u : popc(fwb)) : 0u);
-    uint32_t base = 0u;
+    const uint32_t voted = half_or(bbv);
+    const bool v = lane < 32u && ((voted >> lane) & 1u) != 0u;
+    const uint32_t lines = wave_sum(v ? (full_tile_update(P, band) ? 8u : popc(fwb)) : 0u);
     if (lane == 0u) {
-        base = atomicAdd(list_count(d, par, list_of(col)), popc(voted));
-        // only this column's wave writes its counters: a plain read-modify-write
         reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
-    }
-    base = __shfl(base, 0);
-    if (want != 0u) {
-        const uint32_t at = base + popc(voted & ((1u << lane) - 1u));
-        const size_t cap = (size_t)list_cols(d.ncol) * NSLOT;  // a list's columns x 32 tiles
-        d.tileList[(par * NLISTS + list_of(col)) * cap + at] =
-            make_uint4(col * NSLOT + lane, cn, want, fwb | (full ? 0x100u : 0u));
+        const uint32_t l = list_of(col), cap = list_cols(d.ncol);
+        d.learnList[(par * NLISTS + l) * cap + atomicAdd(list_count(d, par, l), 1u)] = col;
     }
 }
 
 // A column's emit inputs.  Every column needs its drive, accumulators, clamp
-// word and bias (read before the column gate, which they overlap); only a
-// column that can spike (gate passed or input clamped) needs its NMDA bits
-// and, when learning, its plasticity inputs (branch groups, fired masks, band
-// flags, counters): those are requested right after the gate, so their round
-// trip overlaps the k-of-32 selection, and the other columns never read them.
-struct EmitIn { uint32_t dv, v, stimW, bbv, cn, fwb, band, bia; uint2 lc; };
+// word and bias (read before the column gate, which they overlap).  With side
+// statistics a column that can spike (gate passed or input clamped) also
+// needs its NMDA bits and, when learning, its plasticity inputs (fired
+// masks, band flags, counters): those are requested right after the gate, so
+// their round trip overlaps the k-of-32 selection, and the other columns
+// never read them.  Without side statistics plasticity reads its inputs
+// itself (vote_column), one timestep later.
+struct EmitIn { uint32_t dv, v, stimW, bbv, fwb, band, bia; uint2 lc; };
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, uint32_t col) {
     const uint32_t n = lane_id() & 31u;
@@ -784,18 +752,17 @@ __device__ __forceinline__ EmitIn emit_inputs(const NetDev& d, const Params& P, 
     in.dv = d.drive[col];
     in.v = d.acc16 ? (uint32_t)reinterpret_cast<const uint16_t*>(d.acc)[gid] : d.acc[gid];
     in.stimW = clampCol ? d.stim[col] : 0u;
-    in.bbv = 0u; in.cn = 0u; in.fwb = 0u; in.band = 0u;
+    in.bbv = 0u; in.fwb = 0u; in.band = 0u;
     in.lc = make_uint2(0u, 0u);
     in.bia = (LEARN && rec) ? d.bias[gid] : 0u;
     return in;
 }
 template <bool LEARN, bool STATS>
 __device__ __forceinline__ void emit_inputs_spiking(const NetDev& d, const Params& P, uint32_t col, EmitIn& in) {
+    if (!STATS) return;
     const uint32_t n = lane_id() & 31u;
-    const uint32_t gid = col * NPC + n;
-    if (LEARN || STATS) in.bbv = d.bbits[gid];
+    in.bbv = d.bbits[col * NPC + n];
     if (LEARN && P.phase != 3u) {
-        in.cn = d.conn[col * NSLOT + n];
         in.fwb = d.firedW[col * NSLOT + n];
         in.band = d.tileBand[col * NSLOT + n];
         in.lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
@@ -813,9 +780,7 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
     const uint32_t gid = col * NPC + n;
     const bool clampCol = P.clamp != 0u && col < P.inCols;
     const bool rec = col >= P.inCols;
-    const uint32_t v = in.v, stimW = in.stimW, bbv = in.bbv, cn = in.cn, fwb = in.fwb, band = in.band;
-    const uint32_t bia = in.bia;
-    const uint2 lc = in.lc;
+    const uint32_t v = in.v, stimW = in.stimW, bbv = in.bbv, bia = in.bia;
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
     if (clampCol || gatePass) {
         const Kwta kw = kwta32(v, P.k, n);
@@ -861,8 +826,9 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
         const int nb = (int)bia + (post ? (int)P.biasUp : -dec);
         d.bias[gid] = (uint32_t)max(0, nb);
     }
-    const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
-    if (learnCol) list_plasticity<STATS>(d, P, col, post ? bbv : 0u, cn, fwb, band, lc);
+    // without side statistics the plasticity of this step runs one timestep later (vote_column)
+    if (STATS && LEARN && learns(P, b, (vk > vk1) ? 1u : 0u))
+        list_learning_column(d, P, col, post ? bbv : 0u, in.fwb, in.band, in.lc);
 }
 
 // One voted tile of k_learn (mm:436-495).  Lane (h, n) owns words 4h..4h+3
@@ -955,33 +921,181 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
     }
 }
 
-// k_learn over the voted-tile lists: wave w of the grid serves list w %
-// NLISTS and rewrites its tiles w / NLISTS, w / NLISTS + nwaves / NLISTS, ...
-__device__ __forceinline__ void learn_tiles(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
-                                            uint32_t wg, uint32_t nwg) {
-    const uint32_t par = P.step & 1u;
-    const uint32_t l = wg % NLISTS, w0 = wg / NLISTS, nwaves = nwg / NLISTS;
-    if (w0 >= nwaves) return;  // a grid that is not a multiple of NLISTS waves leaves these idle
-    const uint32_t cap = list_cols(d.ncol) * NSLOT;
-    const uint4* __restrict__ list = d.tileList + (size_t)(par * NLISTS + l) * cap;
-    // the first entry is read together with the count, unconditionally (the
-    // index is clamped into the list; used only if w0 < count)
-    uint4 e = list[min(w0, cap - 1u)];
-    const uint32_t cnt = *list_count(d, par, l);
-    asm volatile("" ::"v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w), "v"(cnt));  // both in flight before either is used
-    for (uint32_t i = w0; i < cnt;) {
-        uint32_t nLTP = 0, nLTD = 0;
-        learn_tile(d, P, spkIn, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
-                   __builtin_amdgcn_readfirstlane(e.z), __builtin_amdgcn_readfirstlane(e.w), nLTP, nLTD);
-        i += nwaves;
-        if (i < cnt) e = list[i];
+// ---- plasticity without side statistics, one timestep late ----
+// The plasticity of timestep t (k_learn, mm:436-495) rewrites tiles (column,
+// branch) of learning columns only, and the next reader of a tile's words is
+// its own column's propagation at timestep t + 1.  So the rewrite runs at the
+// start of that propagation, in the workgroup that propagates the column
+// (spread over the workgroup's waves, then a workgroup barrier before the
+// column's lines are read), and needs no launch of its own.  Everything the
+// update reads is still in place then: the column's spike word and margin
+// flag (step t's emit), its NMDA bits and fired-column masks (step t's
+// propagation; step t + 1's overwrites them only after this), the band flags,
+// and step t's presynaptic spikes (the ping-pong buffer that step t + 1's emit
+// overwrites).  The last timestep of a presentation has no propagation after
+// it and takes k_learn_last.
+template <uint32_t NC>  // columns of the workgroup
+struct VoteList {
+    uint4 tile[NC * NSLOT];         // voted tiles: {tile, presynaptic group, learning-neuron mask, 0}
+    uint16_t line[NC * NSLOT * 8];  // word lines to rewrite: (tile entry << 3) | word
+    uint32_t nt, nl;                // entries listed
+};
+
+// A column's plasticity inputs, requested with the launch's first loads (for
+// every column: they are a few hundred bytes each, and waiting for the spike
+// word first would cost a round trip): NMDA bits, fired-column masks and band
+// flags of lane n's neuron / branch slot, the margin flag and the column's
+// plasticity work counters `cnt` (tiles, lines).
+struct VoteIn { uint32_t bb, fwb, band, marg; uint2 lc; };
+__device__ __forceinline__ VoteIn vote_inputs(const NetDev& d, uint32_t col, const uint2* __restrict__ cnt) {
+    const uint32_t n = lane_id() & 31u;
+    return VoteIn{d.bbits[col * NPC + n], d.firedW[col * NSLOT + n], d.tileBand[col * NSLOT + n], d.marg[col],
+                  cnt[col]};
+}
+
+// Lists the voted tiles of a learning column, and their word lines, into `vl`
+// and adds them to the column's plasticity work counters `cnt`.  Branch slot
+// s is voted (mm:446-452) when some spiking neuron has its slot-s NMDA bit
+// set; a voted tile rewrites all 8 word lines when the update can touch words
+// of silent presynaptic columns (full_tile_update), else only the lines of
+// the presynaptic columns that spiked.  A full pass with the band clip
+// leaves every counter of the tile inside the band, which its band flag
+// records (read at the next voting).  Run by the half-wave h that holds the
+// column: lane n of the half is neuron n and branch slot n, gl =
+// conn[col * 32 + n], b the column's spike word at the learning timestep.
+template <uint32_t NC>
+__device__ __forceinline__ void vote_column(const NetDev& d, const Params& PL, uint32_t col, uint32_t b,
+                                            uint32_t gl, const VoteIn& vi, uint32_t h, uint2* __restrict__ cnt,
+                                            VoteList<NC>& vl) {
+    const uint32_t n = lane_id() & 31u;
+    const uint32_t want = half_transpose32(((b >> n) & 1u) != 0u ? vi.bb : 0u);  // lane s: neurons whose slot-s branch learns
+    const uint32_t voted = hball(want != 0u, h);
+    const bool full = full_tile_update(PL, vi.band);
+    const uint32_t lm = want != 0u ? (full ? 0xFFu : (vi.fwb & 0xFFu)) : 0u;  // the tile's lines to rewrite
+    const uint32_t nl = popc(lm);
+    const uint32_t lsc = half_scan(nl);
+    const uint32_t ltot = __shfl(lsc, (int)(32u * h + 31u));
+    uint2 base = make_uint2(0u, 0u);
+    if (n == 0u) {
+        base = make_uint2(atomicAdd(&vl.nt, popc(voted)), atomicAdd(&vl.nl, ltot));
+        cnt[col] = make_uint2(vi.lc.x + popc(voted), vi.lc.y + ltot);  // only this half writes them
+    }
+    base.x = __shfl(base.x, (int)(32u * h));
+    base.y = __shfl(base.y, (int)(32u * h));
+    if (want != 0u) {
+        const uint32_t i = base.x + popc(voted & ((1u << n) - 1u));
+        vl.tile[i] = make_uint4(col * NSLOT + n, gl, want, 0u);
+        uint32_t pos = base.y + lsc - nl;
+        for (uint32_t m = lm; m; m &= m - 1u) vl.line[pos++] = (uint16_t)((i << 3) | (uint32_t)__builtin_ctz(m));
+        if (full && PL.bandHi != 0u) d.tileBand[col * NSLOT + n] = 1u;
     }
 }
 
-// One wave per workgroup: the waves share nothing, and single-wave
-// workgroups dispatch and retire a little faster (measured).
-__global__ __launch_bounds__(64) void k_learn_tiles(NetDev d, Params P, const uint32_t* __restrict__ spkIn) {
-    learn_tiles(d, P, spkIn, blockIdx.x, gridDim.x);
+// The listed word lines (k_learn's per-word update, mm:471-487), dealt to the
+// workgroup's half-waves: half-wave q rewrites lines q, q + nq, q + 2nq, ...,
+// four at a time (the four lines' plane words and spike words requested
+// before any is updated); lane n of the half is neuron n.  The listed tiles
+// belong to the workgroup's NC columns, from tile0 on: each plane is
+// addressed through a buffer resource over those NC * 32 tiles (base in
+// scalar registers, one 32-bit offset per line for all four planes).
+template <uint32_t NC>
+__device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL, const uint32_t* __restrict__ spkL,
+                                              const VoteList<NC>& vl, uint32_t wave, uint32_t nwaves,
+                                              uint32_t tile0) {
+    constexpr uint32_t U = 4;
+    const uint32_t lane = lane_id(), n = lane & 31u;
+    const uint32_t nq = 2u * nwaves, q = 2u * wave + (lane >> 5);
+    const uint32_t L = vl.nl;
+    const size_t t0 = (size_t)__builtin_amdgcn_readfirstlane(tile0) * TILE_WORDS;
+    constexpr uint32_t BYTES = NC * NSLOT * TILE_WORDS * 4u;
+    const __amdgpu_buffer_rsrc_t r0 = buf_rsrc(d.W0 + t0, BYTES), r1 = buf_rsrc(d.W1 + t0, BYTES),
+                                 r2 = buf_rsrc(d.W2 + t0, BYTES), r3 = buf_rsrc(d.W3 + t0, BYTES);
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(spkL, d.ncol * 4u);
+    uint32_t nLTP = 0u, nLTD = 0u;  // not reported without side statistics
+    for (uint32_t j0 = 0; j0 < L; j0 += nq * U) {  // L is workgroup-uniform
+        uint32_t e[U], off[U], so[U], wm[U], a[U][4], sp[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) e[u] = vl.line[min(j0 + q + u * nq, L - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint4 te = vl.tile[e[u] >> 3];
+            const uint32_t w = e[u] & 7u;
+            const bool ok = j0 + q + u * nq < L;  // else no request: reads 0, writes nothing
+            off[u] = ok ? (((te.x - tile0) * 8u + w) * 32u + n) * 4u : SKIP;
+            so[u] = ok ? (8u * te.y + w) * 4u : SKIP;
+            wm[u] = te.z;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            a[u][0] = buf_ld(r0, off[u]); a[u][1] = buf_ld(r1, off[u]);
+            a[u][2] = buf_ld(r2, off[u]); a[u][3] = buf_ld(r3, off[u]);
+            sp[u] = buf_ld(rs, so[u]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            learn_word(a[u][0], a[u][1], a[u][2], a[u][3], sp[u], ((wm[u] >> n) & 1u) != 0u, PL, nLTP, nLTD);
+            buf_st(r0, off[u], a[u][0]); buf_st(r1, off[u], a[u][1]);
+            buf_st(r2, off[u], a[u][2]); buf_st(r3, off[u], a[u][3]);
+        }
+    }
+}
+
+// The plasticity of a presentation's last timestep, which no propagation
+// follows to take it: 16 columns per workgroup, one wave each for the vote,
+// then all 16 waves rewrite the listed lines.  spkL is the step's presynaptic
+// spikes, spkOut its output spikes.  Its work is counted in cntLast.
+__global__ __launch_bounds__(1024) void k_learn_last(NetDev d, Params PL, const uint32_t* __restrict__ spkL,
+                                                     const uint32_t* __restrict__ spkOut) {
+    __shared__ VoteList<16> vl;
+    __shared__ uint32_t anyLearn;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t col = blockIdx.x * 16u + wave;
+    const uint32_t ccol = min(col, d.ncol - 1u);
+    const uint32_t b = spkOut[ccol];
+    const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
+    const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
+    const bool lc = col < d.ncol && learns(PL, b, vi.marg);
+    if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; anyLearn = 0u; }
+    __syncthreads();
+    if (lc && lane == 0u) anyLearn = 1u;
+    __syncthreads();
+    if (anyLearn == 0u) return;  // workgroup-uniform
+    if (lc && lane < 32u) vote_column<16>(d, PL, col, b, gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLast), vl);
+    __syncthreads();
+    rewrite_voted<16>(d, PL, spkL, vl, wave, 16u, blockIdx.x * 16u * NSLOT);
+}
+
+// Propagation of one network, 4 columns per workgroup, one per wave.  FUSE:
+// the deferred plasticity of the previous timestep (parameters PL,
+// presynaptic spikes spkL) first rewrites the voted tiles of the
+// workgroup's columns.
+template <int MODE, bool FUSE>
+__global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn, Params PL,
+                                               const uint32_t* __restrict__ spkL) {
+    extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
+    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
+    __shared__ VoteList<4> vl;
+    // the learning-column list of step t-1 has been consumed; free it for step t+1
+    if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t col = blockIdx.x * 4u + wave;
+    const uint32_t ccol = min(col, d.ncol - 1u);
+    const DriveIn in = drive_inputs(d, spkIn, ccol);  // in.prev: the column's spike word of the previous step
+    VoteIn vi{};
+    if (FUSE) vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLearn));
+    if (FUSE && threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
+    const uint8_t* f = stage_fired(d, fired, FUSE);
+    if (FUSE) {
+        if (col < d.ncol && lane < 32u && learns(PL, in.prev, vi.marg))
+            vote_column<4>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
+        __syncthreads();
+        if (vl.nl != 0u) {  // workgroup-uniform
+            rewrite_voted<4>(d, PL, spkL, vl, wave, 4u, blockIdx.x * 4u * NSLOT);
+            __syncthreads();  // the rewritten lines are read below, by any wave of the workgroup
+        }
+    }
+    if (col >= d.ncol) return;
+    drive_col<MODE>(d, P, spkIn, col, in, f, items[wave]);
 }
 
 // The column gate (drive >= max(theta, 1), mm:211) of a 1024-thread
@@ -1067,14 +1181,11 @@ __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_
 // and drive is 0, so no column passes the gate (drive >= 1) and only clamped
 // input columns spike, with their stimulus word and no voted tile (no NMDA
 // bits).  This kernel emits that step directly -- spikes, history row,
-// margin and spiking-column flags, the bias update -- and clears the
-// plasticity list counts of both step parities, in place of the clear, the
-// propagation and the (empty) plasticity launches.  The previous step's
+// margin and spiking-column flags, the bias update -- in place of the clear,
+// the propagation and the (empty) plasticity launches.  The previous step's
 // spikes are not read.
 template <bool LEARN>
 __global__ __launch_bounds__(1024) void k_emit_first(NetDev d, Params P, uint32_t* __restrict__ spkOut) {
-    if (blockIdx.x == 0u && threadIdx.x < 2u * NLISTS)
-        *list_count(d, threadIdx.x / NLISTS, threadIdx.x % NLISTS) = 0u;
     const uint32_t col = emit_col(blockIdx.x, threadIdx.x >> 6);
     if (col >= d.ncol) return;
     const uint32_t lane = lane_id(), n = lane & 31u;
@@ -1830,7 +1941,7 @@ __device__ __forceinline__ uint32_t mb_pattern(const MultiDev& m, uint32_t net) 
 __device__ __forceinline__ NetDev mb_net(const MultiDev& m, uint32_t net) {
     const NetDev d = m.nets[net];
     asm volatile("" ::"s"(d.W0), "s"(d.W3), "s"(d.conn), "s"(d.spk[0]), "s"(d.drive), "s"(d.cntProp),
-                 "s"(d.learnCnt), "s"(d.ncol), "s"(d.active));
+                 "s"(d.cntLearn), "s"(d.ncol), "s"(d.active));
     return d;
 }
 
@@ -1843,24 +1954,42 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
     const uint32_t* tab = reinterpret_cast<const uint32_t*>(m.stimTabs[net]) + (size_t)p * d.ncol;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < d.ncol) { d.stim[i] = tab[i]; d.spk[0][i] = 0u; d.spk[1][i] = 0u; d.colNZ[i] = 0u; }
-    if (i < 2u * NLISTS) *list_count(d, i / NLISTS, i % NLISTS) = 0u;
     if (i < d.nstats) d.stats[i] = 0u;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t) {
+// Batched propagation, 8 columns per workgroup (two per wave).  FUSE: the
+// deferred plasticity of timestep t - 1 (parameters PL) first rewrites the
+// voted tiles of the workgroup's columns, as in k_drive.
+template <int MODE, bool FUSE>
+__global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t, Params PL) {
     const uint32_t net = blockIdx.y;
     const NetDev d = mb_net(m, net);
-    if (d.active == 0xFFFFFFFFu) return;
+    if (d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_COL];
-    if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
-    const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
-    const uint32_t* in = (t & 1u) ? d.spk[1] : d.spk[0];  // no dynamic index into the local copy
-    const DriveIn di = drive_inputs(d, in, min(colA + (lane_id() >> 5), d.ncol - 1u));
-    const uint8_t* f = stage_fired(d, fired);
+    __shared__ VoteList<8> vl;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
+    const uint32_t colA = blockIdx.x * 8u + wave * 2u;  // 8 columns per workgroup
+    const uint32_t col = colA + h, ccol = min(col, d.ncol - 1u);
+    uint32_t* const s0 = d.spk[0];  // no dynamic index into the local copy
+    uint32_t* const s1 = d.spk[1];
+    const uint32_t* in = (t & 1u) ? s1 : s0;
+    const DriveIn di = drive_inputs(d, in, ccol);  // di.prev: the column's spike word of step t - 1
+    VoteIn vi{};
+    if (FUSE) vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLearn));
+    if (FUSE && threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
+    const uint8_t* f = stage_fired(d, fired, FUSE);
+    if (FUSE) {
+        if (col < d.ncol && learns(PL, di.prev, vi.marg))
+            vote_column<8>(d, PL, col, di.prev, di.gl, vi, h, reinterpret_cast<uint2*>(d.cntLearn), vl);
+        __syncthreads();
+        if (vl.nl != 0u) {  // workgroup-uniform
+            rewrite_voted<8>(d, PL, (t & 1u) ? s0 : s1, vl, wave, 4u, blockIdx.x * 8u * NSLOT);
+            __syncthreads();  // the rewritten lines are read below, by any wave of the workgroup
+        }
+    }
     if (colA >= d.ncol) return;
-    drive_pair<MODE>(d, P, in, colA, di, f, items[threadIdx.x >> 6]);
+    drive_pair<MODE>(d, P, in, colA, di, f, items[wave]);
 }
 
 // 256 threads per network: this launch is a latency chain of a few barriers,
@@ -1873,10 +2002,9 @@ __global__ __launch_bounds__(256) void k_mb_topk(MultiDev m, Params P) {
     topk_block<256>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
 }
 
-// k_emit + k_bias + the plasticity listing (as emit_cols) for two columns
-// per wave, the form for many networks at once: half h is column colA + h,
-// lane n of a half is neuron n (and branch n); the plasticity inputs are read
-// only in columns that learn.  No side statistics.
+// k_emit + k_bias (as emit_cols) for two columns per wave, the form for many
+// networks at once: half h is column colA + h, lane n of a half is neuron n.
+// No side statistics.
 template <bool LEARN>
 __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint32_t* __restrict__ spkOut,
                                           uint32_t colA) {
@@ -1910,32 +2038,7 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
         const int nb = (int)bia + (post ? (int)P.biasUp : -dec);
         d.bias[gid] = (uint32_t)max(0, nb);
     }
-    const bool learnCol = LEARN && P.phase != 3u && b != 0u && !(P.marginGate != 0u && !(vk > vk1));
-    if (!learnCol) return;  // half-uniform
-    const uint32_t bbv = post ? d.bbits[gid] : 0u;
-    const uint32_t cn = d.conn[col * NSLOT + n];
-    const uint32_t fwb = d.firedW[col * NSLOT + n];
-    const uint32_t band = d.tileBand[col * NSLOT + n];
-    const uint2 lc = reinterpret_cast<const uint2*>(d.cntLearn)[col];
-    // lane n: the neurons whose branch-n dendrite learns (this half's NMDA bits transposed)
-    const uint32_t want = half_transpose32(bbv);
-    const uint32_t voted = hball(want != 0u, h);
-    const bool full = full_tile_update(P, band);
-    uint32_t lines = want != 0u ? (full ? 8u : popc(fwb)) : 0u;
-    lines = half_sum(lines);
-    const uint32_t par = P.step & 1u;
-    uint32_t base = 0u;
-    if (n == 0u) {
-        base = atomicAdd(list_count(d, par, list_of(col)), popc(voted));
-        reinterpret_cast<uint2*>(d.cntLearn)[col] = make_uint2(lc.x + popc(voted), lc.y + lines);
-    }
-    base = __shfl(base, (int)(32u * h));
-    if (want != 0u) {
-        const uint32_t at = base + popc(voted & ((1u << n) - 1u));
-        const size_t cap = (size_t)list_cols(d.ncol) * NSLOT;
-        d.tileList[(par * NLISTS + list_of(col)) * cap + at] =
-            make_uint4(col * NSLOT + n, cn, want, fwb | (full ? 0x100u : 0u));
-    }
+    // this step's plasticity runs at the next propagation (k_mb_drive) or in k_mb_learn
 }
 
 template <bool LEARN>
@@ -1950,13 +2053,31 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
     emit_pair<LEARN>(d, P, (t & 1u) ? s0 : s1, colA);
 }
 
-__global__ __launch_bounds__(64) void k_mb_learn(MultiDev m, Params P, uint32_t t) {  // single-wave workgroups
+// The plasticity of a round's last timestep t for many networks (as
+// k_learn_last: 16 columns per workgroup, two per wave for the vote, then all
+// 8 waves rewrite the listed lines).
+__global__ __launch_bounds__(512) void k_mb_learn(MultiDev m, Params PL, uint32_t t) {
     const uint32_t net = blockIdx.y;
     const NetDev d = mb_net(m, net);
-    if (d.active == 0xFFFFFFFFu) return;
+    if (d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
+    __shared__ VoteList<16> vl;
+    __shared__ uint32_t anyLearn;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
+    const uint32_t col = blockIdx.x * 16u + wave * 2u + h, ccol = min(col, d.ncol - 1u);
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    learn_tiles(d, P, (t & 1u) ? s1 : s0, blockIdx.x, gridDim.x);
+    const uint32_t b = ((t & 1u) ? s0 : s1)[ccol];  // the step's output spikes
+    const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
+    const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
+    const bool lc = col < d.ncol && learns(PL, b, vi.marg);
+    if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; anyLearn = 0u; }
+    __syncthreads();
+    if (lc && (lane & 31u) == 0u) anyLearn = 1u;
+    __syncthreads();
+    if (anyLearn == 0u) return;  // workgroup-uniform
+    if (lc) vote_column<16>(d, PL, col, b, gl, vi, h, reinterpret_cast<uint2*>(d.cntLast), vl);
+    __syncthreads();
+    rewrite_voted<16>(d, PL, (t & 1u) ? s1 : s0, vl, wave, 8u, blockIdx.x * 16u * NSLOT);
 }
 
 // The first timestep of a presentation for many networks (the batched form
@@ -1973,7 +2094,6 @@ __global__ __launch_bounds__(256) void k_mb_emit_first(MultiDev m, Params P) {
     if (blockIdx.x == 0u && threadIdx.x == 0u) const_cast<NetDev*>(m.nets)[net].active = p;
     if (p == 0xFFFFFFFFu) return;
     const NetDev d = m.nets[net];  // its `active` may be the last round's; not read here
-    if (blockIdx.x == 0u && threadIdx.x < 2u * NLISTS) *list_count(d, threadIdx.x / NLISTS, threadIdx.x % NLISTS) = 0u;
     const uint32_t lane = lane_id(), h = lane >> 5, n = lane & 31u;
     const uint32_t col = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u + h;  // 8 columns per workgroup
     if (col >= d.ncol) return;
@@ -2094,21 +2214,31 @@ static hipError_t launch(F k, dim3 g, dim3 b, uint32_t lds, hipStream_t s, hipEv
     return hipGetLastError();
 }
 
-hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        const Params* PL) {
     const uint32_t* in = d.spk[t & 1u];
+    const uint32_t* prevIn = d.spk[(t & 1u) ^ 1u];  // step t - 1's presynaptic spikes (its plasticity reads them)
     const dim3 g(blocks_for(d.ncol, 4));
-    if (P.mode == 0u) return launch(k_drive<0>, g, dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
-    if (P.mode == 1u) return launch(k_drive<1>, g, dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
-    return launch(k_drive<2>, g, dim3(256), fired_lds_bytes(d.ncol), s, e0, e1, d, P, in);
+    const uint32_t lds = fired_lds_bytes(d.ncol);
+    if (PL) {
+        if (P.mode == 0u) return launch(k_drive<0, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
+        if (P.mode == 1u) return launch(k_drive<1, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
+        return launch(k_drive<2, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
+    }
+    if (P.mode == 0u) return launch(k_drive<0, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
+    if (P.mode == 1u) return launch(k_drive<1, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
+    return launch(k_drive<2, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
 }
 
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    const dim3 g(std::min<uint32_t>(d.ncol, 512u));  // grid-stride over the listed columns / tiles
-    if (stats) return launch(k_learn<true>, g, dim3(256), 0u, s, e0, e1, d, P, in, out);
-    return launch(k_learn_tiles, dim3(g.x * 4u), dim3(64), 0u, s, e0, e1, d, P, in);
+    if (stats) {  // grid-stride over the listed columns
+        const dim3 g(std::min<uint32_t>(d.ncol, 512u));
+        return launch(k_learn<true>, g, dim3(256), 0u, s, e0, e1, d, P, in, out);
+    }
+    return launch(k_learn_last, dim3(blocks_for(d.ncol, 16)), dim3(1024), 0u, s, e0, e1, d, P, in, out);
 }
 
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
@@ -2258,11 +2388,19 @@ hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats,
     k_mb_begin<<<g, 256, 0, s>>>(m);
     return hipGetLastError();
 }
-hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
-    dim3 g((ncol + 7u) / 8u, m.nnet);
-    if (P.mode == 0u) k_mb_drive<0><<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
-    else if (P.mode == 1u) k_mb_drive<1><<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
-    else k_mb_drive<2><<<g, 256, fired_lds_bytes(ncol), s>>>(m, P, t);
+hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s,
+                              const Params* PL) {
+    const dim3 g((ncol + 7u) / 8u, m.nnet);
+    const uint32_t lds = fired_lds_bytes(ncol);
+    if (PL) {
+        if (P.mode == 0u) k_mb_drive<0, true><<<g, 256, lds, s>>>(m, P, t, *PL);
+        else if (P.mode == 1u) k_mb_drive<1, true><<<g, 256, lds, s>>>(m, P, t, *PL);
+        else k_mb_drive<2, true><<<g, 256, lds, s>>>(m, P, t, *PL);
+    } else {
+        if (P.mode == 0u) k_mb_drive<0, false><<<g, 256, lds, s>>>(m, P, t, P);
+        else if (P.mode == 1u) k_mb_drive<1, false><<<g, 256, lds, s>>>(m, P, t, P);
+        else k_mb_drive<2, false><<<g, 256, lds, s>>>(m, P, t, P);
+    }
     return hipGetLastError();
 }
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s) {
@@ -2276,7 +2414,7 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
     return hipGetLastError();
 }
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
-    k_mb_learn<<<dim3(std::min<uint32_t>(ncol, 256u) * 4u, m.nnet), 64, 0, s>>>(m, P, t);
+    k_mb_learn<<<dim3((ncol + 15u) / 16u, m.nnet), 512, 0, s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_emit_first(const MultiDev& m, const Params& P, bool learn, uint32_t ncol, hipStream_t s) {

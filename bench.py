@@ -58,7 +58,7 @@ def acc_bytes(cfg):
     return 2 if 32 * per_branch + cfg.facil <= 0xFFFF else 4
 
 
-def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0):
+def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0, last_lines=None):
     """Algorithmic HBM bytes per kernel over `steps` learning timesteps
     (bias=False: readout steps without the intrinsic bias, which neither read
     nor update it; aw: bytes per accumulator, acc_bytes; firsts: how many of
@@ -71,27 +71,32 @@ def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0
     128-byte lines of presynaptic columns that spiked (device line counters).
     propagation: 128 B per W3 line read + conn (128 B/column) + bias read
     (4 B/neuron, learning uses the bias) + acc (aw B/neuron) and bbits
-    (4 B/neuron) writes + drive (4 B/column) + fired-column masks (32 B/column).
+    (4 B/neuron) writes + drive (4 B/column) + fired-column masks (32 B/column);
+    plus the previous timestep's plasticity, which runs inside it.
     topk_emit: every workgroup's pass over the column drives for the top-K
     (L2-resident: counted once, 4 B/column) + acc read (aw B/neuron) + bias
     read-modify-write (8 B/neuron) + drive read, spike, history and margin
     writes (16 B/column).
-    plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read + written)."""
+    plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read +
+    written); the plasticity launches themselves take the last timestep's of
+    each presentation (last_lines, axon_last_step_counters; None: all of it)."""
     n = ncol * 32
     bb = 1 if bias else 0
     full = steps - firsts
+    last = learn_lines if last_lines is None else last_lines
     prop = 128 * prop_lines + full * (128 * ncol + 4 * bb * n + (aw + 4) * n + 4 * ncol + 32 * ncol)
+    prop += 1024 * (learn_lines - last)
     emit = full * (aw * n + 8 * bb * n + 20 * ncol)
     first = firsts * (8 * bb * n + 20 * ncol)
-    plast = 1024 * learn_lines
+    plast = 1024 * last
     return {"propagation": prop, "first_emit": first, "topk_emit": emit, "plasticity": plast}
 
 
-def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0):
+def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0, last_lines=None):
     """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
     timesteps and device line counters) over the run's wall time: the average
     rate of the whole job, idle launches and gaps included."""
-    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts)
+    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts, last_lines)
     tot = sum(by.values())
     gbps = tot / seconds / 1e9
     return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
@@ -198,6 +203,7 @@ def gpu_bench(args):
     net.sync()
     dt = time.perf_counter() - t0
     lines = net.line_counters()
+    last_lines = net.last_step_counters()[1]
     prop, learn = net.work_counters(reset=True)
     net.close()
 
@@ -215,7 +221,7 @@ def gpu_bench(args):
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
     # training presentations are asynchronous: each one's first step is one launch (first_emit)
-    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg), firsts=args.steps)
+    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg), firsts=args.steps, last_lines=last_lines)
     names = ["propagation", "first_emit", "topk_emit", "plasticity"]  # the column top-K runs inside emit
     kern = {}
     for i, nm in enumerate(names):
@@ -236,7 +242,8 @@ def gpu_bench(args):
         "ms_per_step": 1e3 * dt / args.steps,
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
-        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg), firsts=args.steps),
+        "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg), firsts=args.steps,
+                                       last_lines=last_lines),
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dn),

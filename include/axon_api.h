@@ -163,14 +163,21 @@ axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset);
  * tile), out[1]: word lines plasticity rewrote (each 4 planes x 128 bytes read
  * and written).  Reset clears these and the work counters together. */
 axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t reset);
+/* No reference counterpart: the part of the plasticity counts above done by
+ * the last-timestep plasticity launches of presentations without side
+ * statistics (out[0] tiles, out[1] word lines); the other timesteps'
+ * plasticity runs inside the next timestep's propagation launch.  Cleared by
+ * the resets above. */
+axon_status axon_last_step_counters(axon_net* n, uint64_t out[2]);
 /* enable = 1: presentations run eagerly with HIP events around every kernel
  * and wait for the device before returning (an AXON_PRESENT_ASYNC
  * presentation still skips its side statistics and outputs);
  * axon_kernel_times returns summed device ms and launch counts for
- * [0] propagation, [1] the first timestep of a presentation without side
- * statistics (one launch in place of the clear, the propagation and the
+ * [0] propagation (without side statistics including the previous
+ * timestep's plasticity), [1] the first timestep of a presentation without
+ * side statistics (one launch in place of the clear, the propagation and the
  * plasticity of that step), [2] column top-K + emit (k-WTA, bias),
- * [3] plasticity. */
+ * [3] plasticity launches (without side statistics: the last timestep's). */
 axon_status axon_net_set_profiling(axon_net* n, uint32_t enable);
 axon_status axon_kernel_times(axon_net* n, double ms[4], uint64_t launches[4], uint32_t reset);
 
