@@ -1118,9 +1118,14 @@ __device__ __forceinline__ uint32_t emit_col(uint32_t b, uint32_t w) {
     return 4u * j + (w & 3u);
 }
 
-__device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive, uint32_t ncol, uint32_t inCols,
-                                              uint32_t target, uint32_t dv) {
-    __shared__ uint32_t part[16][16];  // [wave][column of the workgroup]
+// The count behind the gate for the NCW columns of a 1024-thread workgroup:
+// lane j < NCW holds in dl the drive of the workgroup's column j; returns,
+// for column `mine`, the number of recurrent drives that exceed its drive.
+// Every wave of the workgroup must call it (it ends with a barrier).
+template <uint32_t NCW>
+__device__ __forceinline__ uint32_t drives_above(const uint32_t* __restrict__ drive, uint32_t ncol, uint32_t inCols,
+                                                 uint32_t dl, uint32_t mine) {
+    __shared__ uint32_t part[16][NCW];  // [wave][column of the workgroup]
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     uint32_t rv[4];
 #pragma unroll
@@ -1128,22 +1133,27 @@ __device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive
         const uint32_t c = inCols + tid + i * 1024u;
         rv[i] = c < ncol ? drive[c] : 0u;  // past the end: 0, which exceeds no drive
     }
-    const uint32_t dl = drive[min(emit_col(blockIdx.x, lane & 15u), ncol - 1u)];  // lane j < 16: wave j's column
-    uint32_t mine = 0u;
-#pragma unroll
-    for (uint32_t j = 0; j < 16u; ++j) {
+    uint32_t own = 0u;
+#pragma unroll 4
+    for (uint32_t j = 0; j < NCW; ++j) {
         const uint32_t dj = __builtin_amdgcn_readlane(dl, j);
         uint32_t cnt = 0u;
 #pragma unroll
         for (uint32_t i = 0; i < 4u; ++i) cnt += (uint32_t)__popcll(__ballot(rv[i] > dj));
-        if (lane == j) mine = cnt;
+        if (lane == j) own = cnt;
     }
-    if (lane < 16u) part[wave][lane] = mine;
+    if (lane < NCW) part[wave][lane] = own;
     __syncthreads();
     uint32_t tot = 0u;
 #pragma unroll
-    for (uint32_t w = 0; w < 16u; ++w) tot += part[w][wave];
-    return dv >= 1u && tot < target;
+    for (uint32_t w = 0; w < 16u; ++w) tot += part[w][mine];
+    return tot;
+}
+__device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive, uint32_t ncol, uint32_t inCols,
+                                              uint32_t target, uint32_t dv) {
+    const uint32_t dl = drive[min(emit_col(blockIdx.x, lane_id() & 15u), ncol - 1u)];  // lane j < 16: wave j's column
+    const uint32_t above = drives_above<16>(drive, ncol, inCols, dl, threadIdx.x >> 6);  // every wave: a barrier inside
+    return dv >= 1u && above < target;
 }
 
 // Column gate and emit in one kernel for a single network, one wave per
@@ -2013,12 +2023,11 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
     const uint32_t gid = col * NPC + n;
     const bool clampCol = P.clamp != 0u && col < P.inCols;
     const bool rec = col >= P.inCols;
-    const uint32_t th = max(d.theta[0], 1u);
     const uint32_t dv = d.drive[col];
     const uint32_t v = d.acc16 ? (uint32_t)reinterpret_cast<const uint16_t*>(d.acc)[gid] : d.acc[gid];
     const uint32_t stimW = clampCol ? d.stim[col] : 0u;
     const uint32_t bia = (LEARN && rec) ? d.bias[gid] : 0u;
-    const bool gatePass = dv >= th;
+    const bool gatePass = dv >= max(d.theta[0], 1u);
     uint32_t b = 0u, vk = 0u, vk1 = 0u;
     if (clampCol || gatePass) {
         const Kwta kw = kwta32(v, P.k, n, h);
