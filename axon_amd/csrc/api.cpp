@@ -1179,11 +1179,20 @@ extern "C" axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint3
 
 // ================================================================== many networks
 // Networks are trained in up to MULTI_GROUPS lock-step groups, one stream
-// each: networks with the same queue length share a group (merged down to the
-// limit, shortest first), so a group's launches never carry idle networks for
-// long, and the groups' kernels overlap on the device (one group's
-// latency-bound top-K runs beside another's propagation).
+// each, and no more groups than the process has hardware queues
+// (GPU_MAX_HW_QUEUES, the HIP runtime's setting, 4 by default): networks with
+// the same queue length share a group (merged down to the limit, shortest
+// first), so a group's launches never carry idle networks for long, and the
+// groups' kernels overlap on the device (one group's latency-bound top-K runs
+// beside another's propagation).  Measured on 25 networks: 2 groups 1.05 s,
+// 3 or 4 groups 1.00 s, 5 groups 1.16 s; more hardware queues with more
+// groups were much slower (8: 2.1 s).
 constexpr uint32_t MULTI_GROUPS = 4;
+static uint32_t multi_groups() {
+    const char* e = getenv("GPU_MAX_HW_QUEUES");
+    const int q = e ? atoi(e) : (int)MULTI_GROUPS;
+    return (uint32_t)std::min<int>(std::max(q, 1), (int)MULTI_GROUPS);
+}
 
 struct MultiGroup {
     axon::MultiDev m{};
@@ -1323,7 +1332,8 @@ static axon_status multi_train(axon_multi* b, const uint32_t* epochs, uint32_t e
             if (members.empty() || len[members.back()[0]] != len[i]) members.emplace_back();
             members.back().push_back(i);
         }
-        while (members.size() > MULTI_GROUPS) {
+        const uint32_t ng = multi_groups();
+        while (members.size() > ng) {
             auto last = members.back();
             members.pop_back();
             members.back().insert(members.back().end(), last.begin(), last.end());
