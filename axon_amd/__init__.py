@@ -604,6 +604,16 @@ def mem_probe(kind: int, nbytes: int, reps: int = 5) -> float:
     return g.value
 
 
+def sync_probe(kind: int, groups: int, workgroups: int, reps: int = 200) -> float:
+    """Microseconds per launch of groups x workgroups storing workgroups with
+    (kind 1) an arrival atomic, (2) an agent-scope release fence before it,
+    (3) a full agent-scope fence, or (0) neither (axon_sync_probe)."""
+    us = ctypes.c_double()
+    _check(lib().axon_sync_probe(ctypes.c_uint32(kind), ctypes.c_uint32(groups), ctypes.c_uint32(workgroups),
+                                 ctypes.c_uint32(reps), ctypes.byref(us)))
+    return us.value
+
+
 def prop_probe(net: Net, spikes: np.ndarray, reps: int = 20):
     """(microseconds per propagation launch, W3 lines one launch reads) with
     `spikes` (ncol words) as the presynaptic activity (axon_prop_probe)."""

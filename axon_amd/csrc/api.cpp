@@ -2236,6 +2236,46 @@ extern "C" axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t re
     return AXON_OK;
 }
 
+extern "C" axon_status axon_sync_probe(uint32_t kind, uint32_t groups, uint32_t workgroups, uint32_t reps,
+                                       double* us) {
+    if (!us || kind > 3u || reps == 0u || groups == 0u || groups > 64u || workgroups == 0u || workgroups > 65536u)
+        return fail(AXON_ERR_INVALID_ARG, "kind 0..3, groups 1..64, workgroups 1..65536, reps >= 1, us not null");
+    int dev = 0;
+    axon_status st = select_device(&dev);
+    if (st != AXON_OK) return st;
+    uint32_t *out = nullptr, *cnt = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (s) (void)hipStreamSynchronize(s);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        if (cnt) (void)hipFree(cnt);
+        if (out) (void)hipFree(out);
+    };
+    const size_t outWords = (size_t)groups * workgroups * 264u;
+    if (hipMalloc((void**)&out, outWords * 4) != hipSuccess || hipMalloc((void**)&cnt, (size_t)groups * 128) != hipSuccess) {
+        cleanup();
+        return fail(AXON_ERR_OUT_OF_MEMORY, "synchronisation probe: hipMalloc");
+    }
+    hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, (size_t)groups * 128, s);
+    if (e == hipSuccess) e = axon::launch_sync_probe(kind, out, cnt, groups, workgroups, s);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = axon::launch_sync_probe(kind, out, cnt, groups, workgroups, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    cleanup();
+    if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "synchronisation probe: %s", hipGetErrorString(e));
+    *us = 1e3 * (double)ms / reps;
+    return AXON_OK;
+}
+
 // ================================================================== guarded entry points
 // The entry points whose host allocations depend on their arguments or on a
 // file: the bodies above, behind guarded() (no exception crosses the C ABI).

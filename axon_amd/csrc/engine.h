@@ -179,6 +179,9 @@ hipError_t launch_fill(uint32_t* p, uint32_t value, size_t nwords, hipStream_t s
 hipError_t launch_mem_probe(uint32_t kind, const uint32_t* p, size_t bytes, uint32_t blocks, uint32_t iters,
                             uint32_t* out, hipStream_t s);
 size_t mem_probe_bytes(uint32_t kind, size_t bytes, uint32_t blocks, uint32_t iters);
+// synchronisation probe: out holds groups * nwg * 264 words, cnt groups * 32 words
+hipError_t launch_sync_probe(uint32_t kind, uint32_t* out, uint32_t* cnt, uint32_t groups, uint32_t nwg,
+                             hipStream_t s);
 hipError_t launch_checksum(const uint32_t* p, size_t n, unsigned long long* out, hipStream_t s);
 
 hipError_t launch_bstest(uint32_t op, uint32_t lo, uint32_t hi, uint32_t* a0, uint32_t* a1, uint32_t* a2,

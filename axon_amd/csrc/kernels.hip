@@ -2482,6 +2482,38 @@ size_t mem_probe_bytes(uint32_t kind, size_t bytes, uint32_t blocks, uint32_t it
     if (kind == 0) return bytes;
     return (size_t)blocks * 4u * iters * (kind == 1 ? 2u * 128u : 1024u);
 }
+// Cost of cross-workgroup synchronisation at a propagation launch's shape
+// (the measurement behind running a timestep as separate launches rather
+// than one persistent kernel with grid-wide waits): every workgroup of group
+// y stores what a batched propagation workgroup does (8 column words and 256
+// neuron words), then KIND 1 adds one arrival atomic per workgroup on the
+// group's counter (what a grid barrier or a last-workgroup reduction needs),
+// KIND 2 an agent-scope release fence before it (so that the stores are
+// visible to the other XCDs' workgroups), KIND 3 a full agent-scope fence.
+// Nothing waits for anything: every workgroup finishes on its own.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_sync_probe(uint32_t* __restrict__ out, uint32_t* __restrict__ cnt,
+                                                    uint32_t nwg) {
+    const uint32_t g = blockIdx.y, b = blockIdx.x;
+    uint32_t* o = out + ((size_t)g * nwg + b) * 264u;
+    o[threadIdx.x] = threadIdx.x ^ b;
+    if (threadIdx.x < 8u) o[256u + threadIdx.x] = b + threadIdx.x;
+    if (KIND == 0) return;
+    if (KIND == 2) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (KIND == 3) __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0u) atomicAdd(&cnt[g * 32u], 1u);
+}
+
+hipError_t launch_sync_probe(uint32_t kind, uint32_t* out, uint32_t* cnt, uint32_t groups, uint32_t nwg,
+                             hipStream_t s) {
+    const dim3 g(nwg, groups);
+    if (kind == 0) k_sync_probe<0><<<g, 256, 0, s>>>(out, cnt, nwg);
+    else if (kind == 1) k_sync_probe<1><<<g, 256, 0, s>>>(out, cnt, nwg);
+    else if (kind == 2) k_sync_probe<2><<<g, 256, 0, s>>>(out, cnt, nwg);
+    else k_sync_probe<3><<<g, 256, 0, s>>>(out, cnt, nwg);
+    return hipGetLastError();
+}
 hipError_t launch_mem_probe(uint32_t kind, const uint32_t* p, size_t bytes, uint32_t blocks, uint32_t iters,
                             uint32_t* out, hipStream_t s) {
     if (kind == 0) k_mem_stream<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint4*>(p), bytes / 16u, out);

@@ -184,6 +184,25 @@ def hbm_calibration():
             g = axon_amd.mem_probe(kind, nbytes, 5)
             out[f"{name}_{label}"] = {"GBps": round(g, 1), "frac_of_peak": round(g / HBM_PEAK_GBS, 4)}
     out["propagation_probes"] = propagation_probes()
+    out["sync_probe"] = sync_probes()
+    return out
+
+
+def sync_probes():
+    """Microseconds per launch of storing workgroups with no cross-workgroup
+    synchronisation, with an arrival atomic per workgroup, with an agent-scope
+    release fence before it, and with a full agent-scope fence (what a grid
+    barrier or a last-workgroup reduction would add to every workgroup), at
+    the shapes of one network's propagation (1 x 1024 workgroups) and of a
+    5-network batched propagation (5 x 512): the measurement behind running
+    each timestep as separate launches rather than one persistent kernel."""
+    import axon_amd
+
+    out = {}
+    for label, groups, wgs in (("1x1024", 1, 1024), ("5x512", 5, 512)):
+        out[label] = {name: round(axon_amd.sync_probe(kind, groups, wgs, 200), 2)
+                      for kind, name in ((0, "plain"), (1, "arrival_atomic"), (2, "release_fence_atomic"),
+                                         (3, "full_fence_atomic"))}
     return out
 
 

@@ -315,6 +315,17 @@ axon_status axon_mem_probe(uint32_t kind, uint64_t bytes, uint32_t reps, double*
  * counters); the planes, conn and bias are unchanged. */
 axon_status axon_prop_probe(axon_net* n, const uint32_t* spikes, uint32_t reps, double* us, uint64_t* lines);
 
+/* No reference counterpart: the cost of cross-workgroup synchronisation at a
+ * propagation launch's shape (groups x workgroups of 256 threads, each
+ * storing what a batched propagation workgroup stores), the measurement
+ * behind running a timestep as separate launches instead of one persistent
+ * kernel with grid-wide waits.  kind 0: the stores alone; 1: plus one
+ * arrival atomic per workgroup on its group's counter; 2: plus an
+ * agent-scope release fence before it; 3: plus a full agent-scope fence.
+ * No workgroup waits for another.  *us = mean microseconds per launch over
+ * `reps` back-to-back launches after one warm-up. */
+axon_status axon_sync_probe(uint32_t kind, uint32_t groups, uint32_t workgroups, uint32_t reps, double* us);
+
 /* ---- serving: continuous batching of recall queries ----
  * Extension with no reference counterpart.  Trained networks stay resident;
  * each query is one readout presentation (a probe) of T steps on one network.

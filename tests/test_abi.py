@@ -91,3 +91,13 @@ def test_mem_probe_rejects_bad_arguments():
         with pytest.raises(axon_amd.AxonError) as e:
             axon_amd.mem_probe(kind, nbytes, reps)
         assert e.value.status == 5
+
+
+def test_sync_probe_rejects_bad_arguments():
+    """Argument checks of the synchronisation probe come before any device work."""
+    import axon_amd
+    for kind, groups, wgs, reps in ((4, 1, 1, 1), (0, 0, 1, 1), (0, 65, 1, 1), (0, 1, 0, 1), (0, 1, 65537, 1),
+                                    (0, 1, 1, 0)):
+        with pytest.raises(axon_amd.AxonError) as e:
+            axon_amd.sync_probe(kind, groups, wgs, reps)
+        assert e.value.status == 5

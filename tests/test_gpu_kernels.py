@@ -84,6 +84,21 @@ def test_mem_probe_runs_and_reports_plausible_bandwidth():
             assert 100.0 < g < 8000.0
 
 
+def test_sync_probe_orders_the_synchronisation_costs():
+    """The synchronisation probe runs every kind at a propagation launch's
+    shape; a per-workgroup agent-scope release fence costs more than the
+    stores alone (the basis for separate launches per timestep phase), and
+    bad arguments are refused."""
+    us = {kind: axon_amd.sync_probe(kind, 2, 256, 20) for kind in range(4)}
+    print(f"sync probe 2x256: {us}", flush=True)
+    assert all(0.0 < v < 10000.0 for v in us.values())
+    assert us[2] > us[0]
+    with pytest.raises(axon_amd.AxonError):
+        axon_amd.sync_probe(4, 1, 1, 1)
+    with pytest.raises(axon_amd.AxonError):
+        axon_amd.sync_probe(0, 0, 1, 1)
+
+
 def test_prop_probe_reads_exactly_the_spiking_columns_lines():
     """The propagation probe runs the network's propagation on the given
     activity: a launch reads one W3 line per (tile, spiking presynaptic
