@@ -1305,8 +1305,8 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
         pending = P.phase != 3u;
         PL = P;
     }
-    if (e == hipSuccess && pending) e = axon::launch_multi_learn(m, PL, T - 1u, b->ncol, s);
-    if (e == hipSuccess) e = axon::launch_multi_end(m, s);
+    // the last plasticity launch ends the round itself; without one, k_mb_end does
+    if (e == hipSuccess) e = pending ? axon::launch_multi_learn(m, PL, T - 1u, b->ncol, s) : axon::launch_multi_end(m, s);
     return e;
 }
 

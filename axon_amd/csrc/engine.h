@@ -128,6 +128,7 @@ hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, ui
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s);
 hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
                              hipStream_t s);
+// the last step's plasticity of a round, which also advances the round counter (as launch_multi_end)
 hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s);
 // timestep 0 of a round together with its begin work (k_mb_emit_first); then

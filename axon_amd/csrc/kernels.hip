@@ -2064,8 +2064,10 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
 
 // The plasticity of a round's last timestep t for many networks (as
 // k_learn_last: 16 columns per workgroup, two per wave for the vote, then all
-// 8 waves rewrite the listed lines).
+// 8 waves rewrite the listed lines), which also ends the round (k_mb_end).
 __global__ __launch_bounds__(512) void k_mb_learn(MultiDev m, Params PL, uint32_t t) {
+    // the round's last launch also advances the round counter (read by the next round's first launch only)
+    if (blockIdx.x == 0u && blockIdx.y == 0u && threadIdx.x == 0u) *m.round += 1u;
     const uint32_t net = blockIdx.y;
     const NetDev d = mb_net(m, net);
     if (d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
