@@ -33,8 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of this workload's kernels, measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE (means over all dispatches of the
 # Configuration C training run), each scaled by what the counter reads for a
-# 256 MiB device copy (x2.000 and x1.000); profiles/k_emit_c1_traffic.md.
-MEASURED_TRAFFIC = {"propagation": 6751900, "first_emit": 1098286, "topk_emit": 1728896, "plasticity": 3461143}
+# 256 MiB device copy (x2.000 and x1.000); profiles/k_drive_c1_traffic.md.
+# propagation: the average launch, 2 with the previous timestep's plasticity
+# (9473536 B) to 1 without (6787482 B); plasticity: the last timestep's launch.
+MEASURED_TRAFFIC = {"propagation": 8578185, "first_emit": 1091584, "topk_emit": 1525965, "plasticity": 5272474}
 SYN_PER_TILE = 8192
 CONFIG_C = ("nand=4 target=64 k=24 dthr=1 mode=1 inbr=4 stimcols=64 contrast=1 margingate=0 "
             "ltdrep=1 ltdevery=2 biasup=73 biasdown=1 T=10 clampsteps=5").split()
