@@ -1184,10 +1184,10 @@ extern "C" axon_status axon_probe_cue(const axon_net* n, uint32_t pattern, uint3
 // the same queue length share a group (merged down to the limit, shortest
 // first), so a group's launches never carry idle networks for long, and the
 // groups' kernels overlap on the device (one group's latency-bound top-K runs
-// beside another's propagation).  Measured on 25 networks: 2 groups 1.05 s,
-// 3 or 4 groups 1.00 s, 5 groups 1.16 s; more hardware queues with more
-// groups were much slower (8: 2.1 s).
-constexpr uint32_t MULTI_GROUPS = 4;
+// beside another's propagation).  Measured on 25 networks (4 hardware
+// queues): 3 groups 0.925 s, 4 groups 0.945 s; with as many queues as
+// groups, 2: 0.97 s, 3: 0.93 s, 4: 0.94 s, 5: 1.16 s, 8: 2.1 s.
+constexpr uint32_t MULTI_GROUPS = 3;
 static uint32_t multi_groups() {
     const char* e = getenv("GPU_MAX_HW_QUEUES");
     const int q = e ? atoi(e) : (int)MULTI_GROUPS;

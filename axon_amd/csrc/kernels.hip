@@ -993,16 +993,15 @@ __device__ __forceinline__ void vote_column(const NetDev& d, const Params& PL, u
 
 // The listed word lines (k_learn's per-word update, mm:471-487), dealt to the
 // workgroup's half-waves: half-wave q rewrites lines q, q + nq, q + 2nq, ...,
-// four at a time (the four lines' plane words and spike words requested
-// before any is updated); lane n of the half is neuron n.  The listed tiles
+// U at a time (the U lines' plane words and spike words requested before any
+// is updated); lane n of the half is neuron n.  The listed tiles
 // belong to the workgroup's NC columns, from tile0 on: each plane is
 // addressed through a buffer resource over those NC * 32 tiles (base in
 // scalar registers, one 32-bit offset per line for all four planes).
-template <uint32_t NC>
+template <uint32_t NC, uint32_t U = 4>
 __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL, const uint32_t* __restrict__ spkL,
                                               const VoteList<NC>& vl, uint32_t wave, uint32_t nwaves,
                                               uint32_t tile0) {
-    constexpr uint32_t U = 4;
     const uint32_t lane = lane_id(), n = lane & 31u;
     const uint32_t nq = 2u * nwaves, q = 2u * wave + (lane >> 5);
     const uint32_t L = vl.nl;
@@ -1967,6 +1966,13 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
     if (i < d.nstats) d.stats[i] = 0u;
 }
 
+union MbDriveLds {
+    VoteList<8> vl;
+    uint32_t items[4][2 * ITEMS_PER_COL];
+};
+static_assert(offsetof(VoteList<8>, nt) >= sizeof(uint32_t) * 4u * 2u * ITEMS_PER_COL,
+              "the vote counts must not share LDS with the item lists");
+
 // Batched propagation, 8 columns per workgroup (two per wave).  FUSE: the
 // deferred plasticity of timestep t - 1 (parameters PL) first rewrites the
 // voted tiles of the workgroup's columns, as in k_drive.
@@ -1976,8 +1982,12 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     const NetDev d = mb_net(m, net);
     if (d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
-    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_COL];
-    __shared__ VoteList<8> vl;
+    // The vote list is used before the propagation's item lists: they share
+    // LDS (8 KiB less per workgroup, so LDS does not bound the occupancy below
+    // the registers; 25 networks 960 -> 942 ms), all but the counts, which
+    // every wave reads after the vote whether or not it rewrites.
+    __shared__ MbDriveLds sh;
+    VoteList<8>& vl = sh.vl;
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
     const uint32_t colA = blockIdx.x * 8u + wave * 2u;  // 8 columns per workgroup
     const uint32_t col = colA + h, ccol = min(col, d.ncol - 1u);
@@ -1994,12 +2004,15 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
             vote_column<8>(d, PL, col, di.prev, di.gl, vi, h, reinterpret_cast<uint2*>(d.cntLearn), vl);
         __syncthreads();
         if (vl.nl != 0u) {  // workgroup-uniform
-            rewrite_voted<8>(d, PL, (t & 1u) ? s0 : s1, vl, wave, 4u, blockIdx.x * 8u * NSLOT);
+            // two lines per half-wave and round: 55 registers instead of 85 with four, and a many-network
+            // launch is bound by how many of its workgroups are resident (measured: 25 networks 968 ms
+            // against 1006 ms; eight lines, 145 registers: 1261 ms)
+            rewrite_voted<8, 2>(d, PL, (t & 1u) ? s0 : s1, vl, wave, 4u, blockIdx.x * 8u * NSLOT);
             __syncthreads();  // the rewritten lines are read below, by any wave of the workgroup
         }
     }
     if (colA >= d.ncol) return;
-    drive_pair<MODE>(d, P, in, colA, di, f, items[wave]);
+    drive_pair<MODE>(d, P, in, colA, di, f, sh.items[wave]);
 }
 
 // 256 threads per network: this launch is a latency chain of a few barriers,
@@ -2088,7 +2101,7 @@ __global__ __launch_bounds__(512) void k_mb_learn(MultiDev m, Params PL, uint32_
     if (anyLearn == 0u) return;  // workgroup-uniform
     if (lc) vote_column<16>(d, PL, col, b, gl, vi, h, reinterpret_cast<uint2*>(d.cntLast), vl);
     __syncthreads();
-    rewrite_voted<16>(d, PL, (t & 1u) ? s1 : s0, vl, wave, 8u, blockIdx.x * 16u * NSLOT);
+    rewrite_voted<16, 2>(d, PL, (t & 1u) ? s1 : s0, vl, wave, 8u, blockIdx.x * 16u * NSLOT);  // 50 registers, as k_mb_drive
 }
 
 // The first timestep of a presentation for many networks (the batched form
