@@ -254,9 +254,9 @@ class Net:
         return int(out[0]), int(out[1])
 
     def last_step_counters(self):
-        """(tiles, word lines) of the plasticity counts done by last-timestep plasticity
-        launches (presentations without side statistics; the other timesteps' plasticity
-        runs inside the next propagation launch)."""
+        """(tiles, word lines) of the plasticity counts done at the last timestep of
+        presentations without side statistics, inside that timestep's emit launch (the
+        other timesteps' plasticity runs inside the next propagation launch)."""
         out = (ctypes.c_uint64 * 2)()
         _check(lib().axon_last_step_counters(self._h, out))
         return int(out[0]), int(out[1])
@@ -265,7 +265,9 @@ class Net:
         _check(lib().axon_net_set_profiling(self._h, ctypes.c_uint32(int(enable))))
 
     def kernel_times(self, reset: bool = False):
-        """Summed device ms and launch counts of (propagation, first-step launch, top-K + emit, plasticity)."""
+        """Summed device ms and launch counts of (propagation, first-step launch, top-K + emit,
+        plasticity); without side statistics the last slot is the last timestep's emit launch,
+        which runs that timestep's plasticity too (its emit is not in the third)."""
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_uint64 * 4)()
         _check(lib().axon_kernel_times(self._h, ms, n, ctypes.c_uint32(int(reset))))

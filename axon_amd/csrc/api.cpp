@@ -407,21 +407,23 @@ static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t cla
         }
         e = axon::launch_drive(n->d, P, t, n->stream, E(t, 0), E(t, 1), pending ? &PL : nullptr);
         pending = false;
+        if (e == hipSuccess && !stats && learn && P.phase != 3u && t + 1u == T) {
+            // the last step's plasticity in the same launch as its emit
+            e = axon::launch_emit_learn(n->d, P, t, n->stream, E(t, 6), E(t, 7));
+            if (plast) plast[t] = 2u;
+            continue;
+        }
         if (e == hipSuccess) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, E(t, 4), E(t, 5));
         if (e != hipSuccess || !learn || P.phase == 3u) continue;
         if (stats) {
-            e = axon::launch_learn(n->d, P, t, true, n->stream, E(t, 6), E(t, 7));
+            e = axon::launch_learn(n->d, P, t, n->stream, E(t, 6), E(t, 7));
             if (plast) plast[t] = 1u;
         } else {
             PL = P;
             pending = true;
         }
     }
-    if (e == hipSuccess && pending) {
-        e = axon::launch_learn(n->d, PL, T - 1u, false, n->stream, E(T - 1u, 6), E(T - 1u, 7));
-        if (plast) plast[T - 1u] = 1u;
-    }
-    return e;
+    return e;  // nothing pending: the last step's plasticity ran with its emit, or the step did not learn
 }
 
 // The launches of enqueue_steps, each carrying start / end timing events, so
@@ -435,7 +437,7 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         if (r != hipSuccess) return r;
         n->prof_ev.push_back(e);
     }
-    std::vector<uint8_t> plast(T, 0);  // steps that launched the plasticity kernel
+    std::vector<uint8_t> plast(T, 0);  // steps with a plasticity launch: 1 its own, 2 with the emit
     hipError_t e = enqueue_steps(n, P, T, clampSteps, learn, stats, n->prof_ev.data(), plast.data());
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
@@ -443,7 +445,7 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
     for (uint32_t t = 0; t < T; ++t) {
         const bool one = first && t == 0u;  // the first-step launch alone
         for (int k = 0; k < 4; ++k) {
-            if (k == 3 ? !plast[t] : (one != (k == 1))) continue;
+            if (k == 3 ? !plast[t] : (one != (k == 1)) || (k == 2 && plast[t] == 2u)) continue;
             float ms = 0.f;
             if ((e = hipEventElapsedTime(&ms, n->prof_ev[(size_t)8 * t + 2 * k], n->prof_ev[(size_t)8 * t + 2 * k + 1])) != hipSuccess)
                 return e;
@@ -690,7 +692,8 @@ static axon_status counter_pairs(axon_net* n, uint32_t* dev, uint64_t out[2], ui
 }
 
 // The per-column work counter pairs: propagation (cntProp) and plasticity,
-// whose last-timestep launches count apart (cntLast) from the rest (cntLearn).
+// whose work at a presentation's last timestep (in the emit launch) counts
+// apart (cntLast) from the rest (cntLearn).
 static axon_status work_pairs(axon_net* n, uint64_t prop[2], uint64_t learn[2], uint64_t last[2], uint32_t reset) {
     HIPCHECK(hipSetDevice(n->device));
     HIPCHECK(hipStreamSynchronize(n->stream));
@@ -1301,12 +1304,17 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
         }
         e = axon::launch_multi_drive(m, P, t, b->ncol, s, pending ? &PL : nullptr);
         if (e == hipSuccess) e = axon::launch_multi_topk(m, P, s);
-        if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, s);
         pending = P.phase != 3u;
         PL = P;
+        if (e == hipSuccess && pending && t + 1u == T) {
+            // the last step's plasticity with its emit, a launch that also ends the round
+            e = axon::launch_multi_emit_learn(m, P, t, b->ncol, s);
+            return e;
+        }
+        if (e == hipSuccess) e = axon::launch_multi_emit(m, P, t, true, b->ncol, s);
     }
-    // the last plasticity launch ends the round itself; without one, k_mb_end does
-    if (e == hipSuccess) e = pending ? axon::launch_multi_learn(m, PL, T - 1u, b->ncol, s) : axon::launch_multi_end(m, s);
+    // without plasticity at the last step, k_mb_end ends the round
+    if (e == hipSuccess) e = axon::launch_multi_end(m, s);
     return e;
 }
 

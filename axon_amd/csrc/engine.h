@@ -71,8 +71,9 @@ struct NetDev {
     // own columns' voted tiles.)
     uint32_t* learnList; // [2][8][list_cols(ncol)]
     uint32_t* learnCnt;  // [2][8] list lengths, 32 words apart (reset by the step after next's propagation)
-    // The part of cntLearn's work done by the last-timestep plasticity launch
-    // (k_learn_last): per column (tiles, lines); the work counters add it in.
+    // The part of the plasticity work done at a presentation's last timestep
+    // (in its emit launch): per column (tiles, lines); the work counters add
+    // it in.
     uint32_t* cntLast;   // ncol * 2
     uint32_t ncol, incols, inbr, topkpar;
     // active: batched training only -- this round's pattern of the network
@@ -122,14 +123,17 @@ struct MultiDev {
 // ---- launchers (kernels.hip) ; all asynchronous on `s` ----
 hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats, hipStream_t s);
 // PL (optional): the parameters of step t - 1, whose plasticity the launch
-// applies first (deferred plasticity); the last step's takes launch_multi_learn
+// applies first (deferred plasticity); the last step's runs with its emit
+// (launch_multi_emit_learn)
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s,
                               const Params* PL = nullptr);
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s);
 hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
                              hipStream_t s);
 // the last step's plasticity of a round, which also advances the round counter (as launch_multi_end)
-hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
+// the last step of a round when it learns: emit and plasticity in one launch, which also advances the
+// round counter
+hipError_t launch_multi_emit_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s);
 hipError_t launch_multi_end(const MultiDev& m, hipStream_t s);
 // timestep 0 of a round together with its begin work (k_mb_emit_first); then
 // launch_multi_begin and step 0's other launches are not needed
@@ -161,11 +165,16 @@ hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_
 // clear, propagation or plasticity launch (k_emit_first)
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool first = false);
-// Plasticity of step t: with side statistics over the learning-column lists
-// (k_learn); without, the last step of a presentation (k_learn_last), the
-// others being deferred into the next propagation (launch_drive's PL).
-hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s,
-                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Plasticity of step t with side statistics, over the learning-column lists
+// (k_learn); without them it is deferred into the next propagation
+// (launch_drive's PL) or, at the last step, runs with the emit
+// (launch_emit_learn).
+hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
+// The last timestep of a presentation without side statistics, when it
+// learns: emit and that step's plasticity in one launch (k_emit_learn)
+hipError_t launch_emit_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0 = nullptr,
+                             hipEvent_t e1 = nullptr);
 
 hipError_t launch_wstat(const NetDev& d, uint32_t* out, hipStream_t s);
 hipError_t launch_within(const NetDev& d, const uint32_t* ball, const uint32_t* postMask, uint32_t useMask,

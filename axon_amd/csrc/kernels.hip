@@ -769,14 +769,15 @@ __device__ __forceinline__ void emit_inputs_spiking(const NetDev& d, const Param
     }
 }
 
+// Returns the column's spike word and margin flag (0, 0 past ncol).
 template <bool LEARN, bool STATS>
-__device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
-                                          uint32_t* __restrict__ spkOut, uint32_t col, const EmitIn& in,
-                                          bool gatePass) {
+__device__ __forceinline__ uint2 emit_cols(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                           uint32_t* __restrict__ spkOut, uint32_t col, const EmitIn& in,
+                                           bool gatePass) {
     const uint32_t lane = lane_id();
     const uint32_t n = lane & 31u;
     const bool hi = lane >= 32u;
-    if (col >= d.ncol) return;
+    if (col >= d.ncol) return make_uint2(0u, 0u);
     const uint32_t gid = col * NPC + n;
     const bool clampCol = P.clamp != 0u && col < P.inCols;
     const bool rec = col >= P.inCols;
@@ -826,9 +827,11 @@ __device__ __forceinline__ void emit_cols(const NetDev& d, const Params& P, cons
         const int nb = (int)bia + (post ? (int)P.biasUp : -dec);
         d.bias[gid] = (uint32_t)max(0, nb);
     }
-    // without side statistics the plasticity of this step runs one timestep later (vote_column)
+    // without side statistics the plasticity of this step runs one timestep later (vote_column), or,
+    // at a presentation's last timestep, right after the emit (k_emit_learn)
     if (STATS && LEARN && learns(P, b, (vk > vk1) ? 1u : 0u))
         list_learning_column(d, P, col, post ? bbv : 0u, in.fwb, in.band, in.lc);
+    return make_uint2(b, (vk > vk1) ? 1u : 0u);
 }
 
 // One voted tile of k_learn (mm:436-495).  Lane (h, n) owns words 4h..4h+3
@@ -933,7 +936,7 @@ __device__ __forceinline__ void learn_cols(const NetDev& d, const Params& P, con
 // propagation; step t + 1's overwrites them only after this), the band flags,
 // and step t's presynaptic spikes (the ping-pong buffer that step t + 1's emit
 // overwrites).  The last timestep of a presentation has no propagation after
-// it and takes k_learn_last.
+// it: its plasticity runs in its emit launch instead (k_emit_learn).
 template <uint32_t NC>  // columns of the workgroup
 struct VoteList {
     uint4 tile[NC * NSLOT];         // voted tiles: {tile, presynaptic group, learning-neuron mask, 0}
@@ -995,10 +998,11 @@ __device__ __forceinline__ void vote_column(const NetDev& d, const Params& PL, u
 // workgroup's half-waves: half-wave q rewrites lines q, q + nq, q + 2nq, ...,
 // U at a time (the U lines' plane words and spike words requested before any
 // is updated); lane n of the half is neuron n.  The listed tiles
-// belong to the workgroup's NC columns, from tile0 on: each plane is
-// addressed through a buffer resource over those NC * 32 tiles (base in
-// scalar registers, one 32-bit offset per line for all four planes).
-template <uint32_t NC, uint32_t U = 4>
+// belong to the workgroup's NC columns, which lie among the SPAN columns
+// from tile0 on: each plane is addressed through a buffer resource over those
+// SPAN * 32 tiles (base in scalar registers, one 32-bit offset per line for
+// all four planes).
+template <uint32_t NC, uint32_t U = 4, uint32_t SPAN = NC>
 __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL, const uint32_t* __restrict__ spkL,
                                               const VoteList<NC>& vl, uint32_t wave, uint32_t nwaves,
                                               uint32_t tile0) {
@@ -1006,7 +1010,7 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
     const uint32_t nq = 2u * nwaves, q = 2u * wave + (lane >> 5);
     const uint32_t L = vl.nl;
     const size_t t0 = (size_t)__builtin_amdgcn_readfirstlane(tile0) * TILE_WORDS;
-    constexpr uint32_t BYTES = NC * NSLOT * TILE_WORDS * 4u;
+    constexpr uint32_t BYTES = SPAN * NSLOT * TILE_WORDS * 4u;
     const __amdgpu_buffer_rsrc_t r0 = buf_rsrc(d.W0 + t0, BYTES), r1 = buf_rsrc(d.W1 + t0, BYTES),
                                  r2 = buf_rsrc(d.W2 + t0, BYTES), r3 = buf_rsrc(d.W3 + t0, BYTES);
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(spkL, d.ncol * 4u);
@@ -1037,31 +1041,6 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
             buf_st(r2, off[u], a[u][2]); buf_st(r3, off[u], a[u][3]);
         }
     }
-}
-
-// The plasticity of a presentation's last timestep, which no propagation
-// follows to take it: 16 columns per workgroup, one wave each for the vote,
-// then all 16 waves rewrite the listed lines.  spkL is the step's presynaptic
-// spikes, spkOut its output spikes.  Its work is counted in cntLast.
-__global__ __launch_bounds__(1024) void k_learn_last(NetDev d, Params PL, const uint32_t* __restrict__ spkL,
-                                                     const uint32_t* __restrict__ spkOut) {
-    __shared__ VoteList<16> vl;
-    __shared__ uint32_t anyLearn;
-    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t col = blockIdx.x * 16u + wave;
-    const uint32_t ccol = min(col, d.ncol - 1u);
-    const uint32_t b = spkOut[ccol];
-    const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
-    const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
-    const bool lc = col < d.ncol && learns(PL, b, vi.marg);
-    if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; anyLearn = 0u; }
-    __syncthreads();
-    if (lc && lane == 0u) anyLearn = 1u;
-    __syncthreads();
-    if (anyLearn == 0u) return;  // workgroup-uniform
-    if (lc && lane < 32u) vote_column<16>(d, PL, col, b, gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLast), vl);
-    __syncthreads();
-    rewrite_voted<16>(d, PL, spkL, vl, wave, 16u, blockIdx.x * 16u * NSLOT);
 }
 
 // Propagation of one network, 4 columns per workgroup, one per wave.  FUSE:
@@ -1164,10 +1143,11 @@ __device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive
 // over all column drives itself (k_topk_par / k_topk_serial, mm:259-352) --
 // LDS work on 16 KiB the drive kernel left in cache -- instead of waiting for
 // a separate one-workgroup launch.
+// Every wave of the workgroup calls it: the gate ends with a workgroup
+// barrier.  Returns the column's spike word and margin flag.
 template <bool LEARN, bool STATS>
-__global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
-                                               uint32_t* __restrict__ spkOut) {
-    const uint32_t col = emit_col(blockIdx.x, threadIdx.x >> 6);
+__device__ __forceinline__ uint2 emit_step(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
+                                           uint32_t* __restrict__ spkOut, uint32_t col) {
     EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the gate
     const uint32_t nrec = d.ncol - P.inCols;
     bool gate;
@@ -1182,7 +1162,40 @@ __global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_
     }
     if (col < d.ncol && (gate || (P.clamp != 0u && col < P.inCols)))  // wave-uniform: this column can spike
         emit_inputs_spiking<LEARN, STATS>(d, P, col, in);
-    emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, gate);
+    return emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, gate);
+}
+template <bool LEARN, bool STATS>
+__global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                               uint32_t* __restrict__ spkOut) {
+    (void)emit_step<LEARN, STATS>(d, P, spkIn, spkOut, emit_col(blockIdx.x, threadIdx.x >> 6));
+}
+
+// Columns from an emit workgroup's first to its last (emit_col: four runs of
+// four columns, 32 apart).
+constexpr uint32_t EMIT_SPAN = 4u * 8u * 3u + 4u;
+
+// The last timestep of a presentation without side statistics, when it
+// learns: the emit of k_emit<true, false>, then that step's plasticity for
+// the workgroup's own columns in the same workgroup.  A column's plasticity
+// needs its own spike word and margin (just emitted here), its own NMDA bits,
+// fired masks and band flags (read with the emit's inputs) and the step's
+// presynaptic spikes, so no launch of its own ends the presentation.  Its
+// work is counted in cntLast.
+__global__ __launch_bounds__(1024) void k_emit_learn(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                                     uint32_t* __restrict__ spkOut) {
+    __shared__ VoteList<16> vl;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t col = emit_col(blockIdx.x, wave), ccol = min(col, d.ncol - 1u);
+    const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
+    const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
+    if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
+    __syncthreads();
+    const uint2 bm = emit_step<true, false>(d, P, spkIn, spkOut, col);
+    if (col < d.ncol && lane < 32u && learns(P, bm.x, bm.y))
+        vote_column<16>(d, P, col, bm.x, gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLast), vl);
+    __syncthreads();
+    if (vl.nl != 0u)  // workgroup-uniform
+        rewrite_voted<16, 4, EMIT_SPAN>(d, P, spkIn, vl, wave, 16u, emit_col(blockIdx.x, 0u) * NSLOT);
 }
 
 // The first timestep of a presentation without side statistics.  Its
@@ -2027,10 +2040,10 @@ __global__ __launch_bounds__(256) void k_mb_topk(MultiDev m, Params P) {
 
 // k_emit + k_bias (as emit_cols) for two columns per wave, the form for many
 // networks at once: half h is column colA + h, lane n of a half is neuron n.
-// No side statistics.
+// No side statistics.  Returns the half's column's spike word and margin flag.
 template <bool LEARN>
-__device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint32_t* __restrict__ spkOut,
-                                          uint32_t colA) {
+__device__ __forceinline__ uint2 emit_pair(const NetDev& d, const Params& P, uint32_t* __restrict__ spkOut,
+                                           uint32_t colA) {
     const uint32_t lane = lane_id(), h = lane >> 5, n = lane & 31u;
     const uint32_t col = colA + h;
     const uint32_t gid = col * NPC + n;
@@ -2060,7 +2073,9 @@ __device__ __forceinline__ void emit_pair(const NetDev& d, const Params& P, uint
         const int nb = (int)bia + (post ? (int)P.biasUp : -dec);
         d.bias[gid] = (uint32_t)max(0, nb);
     }
-    // this step's plasticity runs at the next propagation (k_mb_drive) or in k_mb_learn
+    // this step's plasticity runs at the next propagation (k_mb_drive) or, at the round's last step,
+    // right after the emit (k_mb_emit_learn)
+    return make_uint2(b, (vk > vk1) ? 1u : 0u);
 }
 
 template <bool LEARN>
@@ -2072,36 +2087,36 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
     uint32_t* const s1 = d.spk[1];
     const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
     if (colA >= d.ncol) return;
-    emit_pair<LEARN>(d, P, (t & 1u) ? s0 : s1, colA);
+    (void)emit_pair<LEARN>(d, P, (t & 1u) ? s0 : s1, colA);
 }
 
-// The plasticity of a round's last timestep t for many networks (as
-// k_learn_last: 16 columns per workgroup, two per wave for the vote, then all
-// 8 waves rewrite the listed lines), which also ends the round (k_mb_end).
-__global__ __launch_bounds__(512) void k_mb_learn(MultiDev m, Params PL, uint32_t t) {
-    // the round's last launch also advances the round counter (read by the next round's first launch only)
+// The last step t of a round, when it learns (as k_emit_learn for one
+// network): k_mb_emit, then the step's plasticity of the workgroup's 8
+// columns in the same workgroup, two lines per half-wave and round (as
+// k_mb_drive).  The round's last launch: it advances the round counter
+// (read by the next round's first launch only).  Its work is counted in
+// cntLast.
+__global__ __launch_bounds__(256) void k_mb_emit_learn(MultiDev m, Params P, uint32_t t) {
     if (blockIdx.x == 0u && blockIdx.y == 0u && threadIdx.x == 0u) *m.round += 1u;
     const uint32_t net = blockIdx.y;
     const NetDev d = mb_net(m, net);
     if (d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
-    __shared__ VoteList<16> vl;
-    __shared__ uint32_t anyLearn;
-    const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
-    const uint32_t col = blockIdx.x * 16u + wave * 2u + h, ccol = min(col, d.ncol - 1u);
+    __shared__ VoteList<8> vl;
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    const uint32_t b = ((t & 1u) ? s0 : s1)[ccol];  // the step's output spikes
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
+    const uint32_t colA = blockIdx.x * 8u + wave * 2u;  // 8 columns per workgroup
+    const uint32_t col = colA + h, ccol = min(col, d.ncol - 1u);
     const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
     const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
-    const bool lc = col < d.ncol && learns(PL, b, vi.marg);
-    if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; anyLearn = 0u; }
+    if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
     __syncthreads();
-    if (lc && (lane & 31u) == 0u) anyLearn = 1u;
+    const uint2 bm = colA < d.ncol ? emit_pair<true>(d, P, (t & 1u) ? s0 : s1, colA) : make_uint2(0u, 0u);
+    if (col < d.ncol && learns(P, bm.x, bm.y))
+        vote_column<8>(d, P, col, bm.x, gl, vi, h, reinterpret_cast<uint2*>(d.cntLast), vl);
     __syncthreads();
-    if (anyLearn == 0u) return;  // workgroup-uniform
-    if (lc) vote_column<16>(d, PL, col, b, gl, vi, h, reinterpret_cast<uint2*>(d.cntLast), vl);
-    __syncthreads();
-    rewrite_voted<16, 2>(d, PL, (t & 1u) ? s1 : s0, vl, wave, 8u, blockIdx.x * 16u * NSLOT);  // 50 registers, as k_mb_drive
+    if (vl.nl != 0u)  // workgroup-uniform
+        rewrite_voted<8, 2>(d, P, (t & 1u) ? s1 : s0, vl, wave, 4u, blockIdx.x * 8u * NSLOT);
 }
 
 // The first timestep of a presentation for many networks (the batched form
@@ -2254,15 +2269,17 @@ hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_
     return launch(k_drive<2, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
 }
 
-hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, bool stats, hipStream_t s, hipEvent_t e0,
-                        hipEvent_t e1) {
+hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* out = d.spk[(t & 1u) ^ 1u];
-    if (stats) {  // grid-stride over the listed columns
-        const dim3 g(std::min<uint32_t>(d.ncol, 512u));
-        return launch(k_learn<true>, g, dim3(256), 0u, s, e0, e1, d, P, in, out);
-    }
-    return launch(k_learn_last, dim3(blocks_for(d.ncol, 16)), dim3(1024), 0u, s, e0, e1, d, P, in, out);
+    const dim3 g(std::min<uint32_t>(d.ncol, 512u));  // grid-stride over the listed columns
+    return launch(k_learn<true>, g, dim3(256), 0u, s, e0, e1, d, P, in, out);
+}
+
+hipError_t launch_emit_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0,
+                             hipEvent_t e1) {
+    const dim3 g(blocks_for(d.ncol, 128) * 8u), b(1024);  // as launch_emit
+    return launch(k_emit_learn, g, b, 0u, s, e0, e1, d, P, d.spk[t & 1u], d.spk[(t & 1u) ^ 1u]);
 }
 
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
@@ -2437,8 +2454,8 @@ hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, boo
     else k_mb_emit<false><<<dim3((ncol + 7u) / 8u, m.nnet), 256, 0, s>>>(m, P, t);
     return hipGetLastError();
 }
-hipError_t launch_multi_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
-    k_mb_learn<<<dim3((ncol + 15u) / 16u, m.nnet), 512, 0, s>>>(m, P, t);
+hipError_t launch_multi_emit_learn(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s) {
+    k_mb_emit_learn<<<dim3((ncol + 7u) / 8u, m.nnet), 256, 0, s>>>(m, P, t);
     return hipGetLastError();
 }
 hipError_t launch_multi_emit_first(const MultiDev& m, const Params& P, bool learn, uint32_t ncol, hipStream_t s) {

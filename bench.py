@@ -60,7 +60,7 @@ def acc_bytes(cfg):
     return 2 if 32 * per_branch + cfg.facil <= 0xFFFF else 4
 
 
-def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0, last_lines=None):
+def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0, last_lines=None, last_emits=0):
     """Algorithmic HBM bytes per kernel over `steps` learning timesteps
     (bias=False: readout steps without the intrinsic bias, which neither read
     nor update it; aw: bytes per accumulator, acc_bytes; firsts: how many of
@@ -79,26 +79,29 @@ def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0
     (L2-resident: counted once, 4 B/column) + acc read (aw B/neuron) + bias
     read-modify-write (8 B/neuron) + drive read, spike, history and margin
     writes (16 B/column).
-    plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read +
-    written); the plasticity launches themselves take the last timestep's of
-    each presentation (last_lines, axon_last_step_counters; None: all of it)."""
+    Plasticity: 1 KiB per rewritten word line (4 planes x 128 B, read +
+    written).  last_emit: the last timestep of each presentation, whose emit
+    launch also runs that timestep's plasticity (last_emits of the steps;
+    last_lines of the word lines, axon_last_step_counters; None: all of
+    them), the other timesteps' lines being the propagation's."""
     n = ncol * 32
     bb = 1 if bias else 0
     full = steps - firsts
     last = learn_lines if last_lines is None else last_lines
     prop = 128 * prop_lines + full * (128 * ncol + 4 * bb * n + (aw + 4) * n + 4 * ncol + 32 * ncol)
     prop += 1024 * (learn_lines - last)
-    emit = full * (aw * n + 8 * bb * n + 20 * ncol)
+    per_emit = aw * n + 8 * bb * n + 20 * ncol
+    emit = (full - last_emits) * per_emit
     first = firsts * (8 * bb * n + 20 * ncol)
-    plast = 1024 * last
-    return {"propagation": prop, "first_emit": first, "topk_emit": emit, "plasticity": plast}
+    last_emit = 1024 * last + last_emits * per_emit
+    return {"propagation": prop, "first_emit": first, "topk_emit": emit, "last_emit": last_emit}
 
 
-def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0, last_lines=None):
+def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0, last_lines=None, last_emits=0):
     """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
     timesteps and device line counters) over the run's wall time: the average
     rate of the whole job, idle launches and gaps included."""
-    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts, last_lines)
+    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts, last_lines, last_emits)
     tot = sum(by.values())
     gbps = tot / seconds / 1e9
     return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
@@ -242,8 +245,10 @@ def gpu_bench(args):
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
     # training presentations are asynchronous: each one's first step is one launch (first_emit)
-    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg), firsts=args.steps, last_lines=last_lines)
-    names = ["propagation", "first_emit", "topk_emit", "plasticity"]  # the column top-K runs inside emit
+    # each presentation's last timestep learns: its emit launch runs its plasticity too (last_emit)
+    kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg), firsts=args.steps, last_lines=last_lines,
+                      last_emits=launches[3])
+    names = ["propagation", "first_emit", "topk_emit", "last_emit"]  # the column top-K runs inside emit
     kern = {}
     for i, nm in enumerate(names):
         if launches[i] == 0:
@@ -264,7 +269,7 @@ def gpu_bench(args):
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
         "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg), firsts=args.steps,
-                                       last_lines=last_lines),
+                                       last_lines=last_lines, last_emits=launches[3]),
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dn),

@@ -163,11 +163,11 @@ axon_status axon_work_counters(axon_net* n, uint64_t out[2], uint32_t reset);
  * tile), out[1]: word lines plasticity rewrote (each 4 planes x 128 bytes read
  * and written).  Reset clears these and the work counters together. */
 axon_status axon_line_counters(axon_net* n, uint64_t out[2], uint32_t reset);
-/* No reference counterpart: the part of the plasticity counts above done by
- * the last-timestep plasticity launches of presentations without side
- * statistics (out[0] tiles, out[1] word lines); the other timesteps'
- * plasticity runs inside the next timestep's propagation launch.  Cleared by
- * the resets above. */
+/* No reference counterpart: the part of the plasticity counts above done at
+ * the last timestep of presentations without side statistics, inside that
+ * timestep's emit launch (out[0] tiles, out[1] word lines); the other
+ * timesteps' plasticity runs inside the next timestep's propagation launch.
+ * Cleared by the resets above. */
 axon_status axon_last_step_counters(axon_net* n, uint64_t out[2]);
 /* enable = 1: presentations run eagerly with HIP events around every kernel
  * and wait for the device before returning (an AXON_PRESENT_ASYNC
@@ -177,7 +177,9 @@ axon_status axon_last_step_counters(axon_net* n, uint64_t out[2]);
  * timestep's plasticity), [1] the first timestep of a presentation without
  * side statistics (one launch in place of the clear, the propagation and the
  * plasticity of that step), [2] column top-K + emit (k-WTA, bias),
- * [3] plasticity launches (without side statistics: the last timestep's). */
+ * [3] plasticity launches: with side statistics the plasticity of each
+ * learning timestep; without, the last timestep's emit launch, which also
+ * runs that timestep's plasticity (its emit is not in [2]). */
 axon_status axon_net_set_profiling(axon_net* n, uint32_t enable);
 axon_status axon_kernel_times(axon_net* n, double ms[4], uint64_t launches[4], uint32_t reset);
 
