@@ -1028,14 +1028,19 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
             so[u] = ok ? (8u * te.y + w) * 4u : SKIP;
             wm[u] = te.z;
         }
+        // slot u of this wave holds a line for its first half (and so for any of its lanes) while
+        // j0 + 2 * wave + u * nq < L: wave-uniform, so the slots past the list cost no instructions
+        const uint32_t left = L - j0 - 2u * wave;  // wraps when this wave has no slot left
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
+            if (u * nq >= left || left > L) break;
             a[u][0] = buf_ld(r0, off[u]); a[u][1] = buf_ld(r1, off[u]);
             a[u][2] = buf_ld(r2, off[u]); a[u][3] = buf_ld(r3, off[u]);
             sp[u] = buf_ld(rs, so[u]);
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
+            if (u * nq >= left || left > L) break;
             learn_word(a[u][0], a[u][1], a[u][2], a[u][3], sp[u], ((wm[u] >> n) & 1u) != 0u, PL, nLTP, nLTD);
             buf_st(r0, off[u], a[u][0]); buf_st(r1, off[u], a[u][1]);
             buf_st(r2, off[u], a[u][2]); buf_st(r3, off[u], a[u][3]);
