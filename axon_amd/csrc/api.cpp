@@ -122,10 +122,10 @@ uint32_t argu(int argc, const char* const* argv, const char* key, uint32_t dflt)
 }
 
 struct GraphKey {
-    uint32_t T, clamp, learn, k, facil, useBias, stats;
+    uint32_t T, clamp, learn, k, facil, useBias, stats, step0;
     bool operator<(const GraphKey& o) const {
-        return std::tie(T, clamp, learn, k, facil, useBias, stats) <
-               std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias, o.stats);
+        return std::tie(T, clamp, learn, k, facil, useBias, stats, step0) <
+               std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias, o.stats, o.step0);
     }
 };
 
@@ -392,9 +392,12 @@ static Params step_params(const axon_net* n, Params P, uint32_t t, uint32_t clam
 // clear/propagation/emit/plasticity.  ev (optional): 8 timing events per
 // step -- propagation, first-step emit, emit, plasticity -- and plast[t]
 // records which steps launched the plasticity kernel.
+// step0: step 0 runs inside step 1's propagation (a first-launch
+// presentation, clamped at step 0 to a stimulus inside the input columns).
 static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t clampSteps, bool learn, bool stats,
-                                hipEvent_t* ev = nullptr, uint8_t* plast = nullptr) {
+                                bool step0, hipEvent_t* ev = nullptr, uint8_t* plast = nullptr) {
     const bool first = !stats && T >= 2u;
+    step0 = step0 && first && clampSteps >= 1u;
     hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
     Params PL{};
     bool pending = false;  // the previous step's plasticity waits for this step's propagation
@@ -402,10 +405,12 @@ static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t cla
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
         const Params P = step_params(n, P0, t, clampSteps, learn);
         if (first && t == 0u) {
-            e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, E(t, 2), E(t, 3), true);
+            if (!step0) e = axon::launch_emit(n->d, P, t, learn, stats, n->stream, E(t, 2), E(t, 3), true);
             continue;
         }
-        e = axon::launch_drive(n->d, P, t, n->stream, E(t, 0), E(t, 1), pending ? &PL : nullptr);
+        const Params S0 = step_params(n, P0, 0u, clampSteps, learn);
+        e = axon::launch_drive(n->d, P, t, n->stream, E(t, 0), E(t, 1), pending ? &PL : nullptr,
+                               step0 && t == 1u ? &S0 : nullptr);
         pending = false;
         if (e == hipSuccess && !stats && learn && P.phase != 3u && t + 1u == T) {
             // the last step's plasticity in the same launch as its emit
@@ -429,7 +434,7 @@ static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t cla
 // The launches of enqueue_steps, each carrying start / end timing events, so
 // the kernels' device durations can be summed after the presentation.
 static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn,
-                                         bool stats) {
+                                         bool stats, bool step0) {
     const size_t need = (size_t)8 * std::max<uint32_t>(T, 1);
     while (n->prof_ev.size() < need) {
         hipEvent_t e;
@@ -438,11 +443,12 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         n->prof_ev.push_back(e);
     }
     std::vector<uint8_t> plast(T, 0);  // steps with a plasticity launch: 1 its own, 2 with the emit
-    hipError_t e = enqueue_steps(n, P, T, clampSteps, learn, stats, n->prof_ev.data(), plast.data());
+    hipError_t e = enqueue_steps(n, P, T, clampSteps, learn, stats, step0, n->prof_ev.data(), plast.data());
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
     const bool first = !stats && T >= 2u;  // as enqueue_steps
-    for (uint32_t t = 0; t < T; ++t) {
+    step0 = step0 && first && clampSteps >= 1u;
+    for (uint32_t t = step0 ? 1u : 0u; t < T; ++t) {
         const bool one = first && t == 0u;  // the first-step launch alone
         for (int k = 0; k < 4; ++k) {
             if (k == 3 ? !plast[t] : (one != (k == 1)) || (k == 2 && plast[t] == 2u)) continue;
@@ -486,6 +492,11 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         for (uint32_t c : n->stimCols[a->pattern])
             if (axon::canonical01(r) <= a->keep) hs[c] = 0xFFFFFFFFu;
     }
+    // Step 0 of a first-launch presentation clamped at step 0 runs inside step
+    // 1's propagation when the stimulus lies inside the input columns (then
+    // it is step 0's spike vector).
+    bool step0 = async && a->T >= 2u && a->clamp_steps >= 1u;
+    for (uint32_t c = n->cfg.incols; step0 && c < n->NCOL; ++c) step0 = hs[c] == 0u;
     HIPCHECK(hipMemcpyAsync(n->d.stim, hs, (size_t)n->NCOL * 4, hipMemcpyHostToDevice, n->stream));
     HIPCHECK(hipEventRecord(n->stim_ev[slot], n->stream));
     n->stim_ev_live[slot] = true;
@@ -493,15 +504,16 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
     const auto t0 = std::chrono::steady_clock::now();
     if (n->profiling) {
         // waits for the device even when async (the event times are read back)
-        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn, !async);
+        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn, !async, step0);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "profiled launch: %s", hipGetErrorString(e));
     } else if (n->use_graphs && a->T > 0) {
-        const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias, async ? 0u : 1u};
+        const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias, async ? 0u : 1u,
+                           step0 ? 1u : 0u};
         auto it = n->graphs.find(key);
         if (it == n->graphs.end()) {
             hipGraph_t g = nullptr;
             HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async);
+            const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, step0);
             hipError_t e2 = hipStreamEndCapture(n->stream, &g);
             if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "capture launch: %s", hipGetErrorString(e));
             if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -513,7 +525,7 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         }
         HIPCHECK(hipGraphLaunch(it->second, n->stream));
     } else {
-        const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async);
+        const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, step0);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
     }
     if (async) {

@@ -568,13 +568,33 @@ inline uint32_t fired_lds_bytes(uint32_t ncol) { return ncol <= FIRED_LDS_COLS ?
 
 // Stage the fired bytes of all columns in LDS while this wave's conn entries
 // load.  sync: end with a workgroup barrier even when nothing is staged.
-__device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds, bool sync = false) {
+// inCols0 (nonzero): the previous step is a step 0 clamped to d.stim, which
+// lies inside the first inCols0 columns: its spiking-column bytes are built
+// from those columns' stimulus words instead.
+__device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds, bool sync = false,
+                                                     uint32_t inCols0 = 0u) {
     if (d.ncol > FIRED_LDS_COLS) {  // uniform over the grid
         if (sync) __syncthreads();
         return nullptr;
     }
-    const uint4* src = reinterpret_cast<const uint4*>(d.colNZ);
-    for (uint32_t i = threadIdx.x; i < (d.ncol + 15u) / 16u; i += blockDim.x) lds[i] = src[i];
+    const uint32_t rows = (d.ncol + 15u) / 16u;
+    if (inCols0 == 0u) {
+        const uint4* src = reinterpret_cast<const uint4*>(d.colNZ);
+        for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x) lds[i] = src[i];
+    } else {
+        const uint4* sw = reinterpret_cast<const uint4*>(d.stim);  // 4 columns per load
+        for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x) {
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            if (16u * i < inCols0) {  // inCols0 is a multiple of 8; a row past it holds no stimulus
+#pragma unroll
+                for (uint32_t k = 0; k < 4u; ++k) {
+                    const uint4 v = 16u * i + 4u * k < inCols0 ? sw[4u * i + k] : make_uint4(0u, 0u, 0u, 0u);
+                    w[k] = (v.x ? 1u : 0u) | (v.y ? 0x100u : 0u) | (v.z ? 0x10000u : 0u) | (v.w ? 0x1000000u : 0u);
+                }
+            }
+            lds[i] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
     __syncthreads();
     return reinterpret_cast<const uint8_t*>(lds);
 }
@@ -1051,8 +1071,15 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
 // Propagation of one network, 4 columns per workgroup, one per wave.  FUSE:
 // the deferred plasticity of the previous timestep (parameters PL,
 // presynaptic spikes spkL) first rewrites the voted tiles of the
-// workgroup's columns.
-template <int MODE, bool FUSE>
+// workgroup's columns.  FIRST: this is step 1 of a presentation without side
+// statistics whose step 0 (parameters PL) is clamped to a stimulus inside the
+// input columns.  Step 0's spikes are then that stimulus (spkIn = d.stim),
+// since with cleared spikes no
+// column passes the gate: this launch also does step 0's own work for its
+// columns (as k_emit_first: spike word, history row, bias decay), so step 0
+// needs no launch.  Step 0's margin and spiking flags are not written: no
+// plasticity reads them (step 0 votes no tile) and the next emit rewrites them.
+template <int MODE, bool FUSE, bool FIRST = false>
 __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn, Params PL,
                                                const uint32_t* __restrict__ spkL) {
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
@@ -1063,11 +1090,22 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     const uint32_t col = blockIdx.x * 4u + wave;
     const uint32_t ccol = min(col, d.ncol - 1u);
-    const DriveIn in = drive_inputs(d, spkIn, ccol);  // in.prev: the column's spike word of the previous step
+    DriveIn in = drive_inputs(d, spkIn, ccol);  // in.prev: the column's spike word of the previous step
     VoteIn vi{};
     if (FUSE) vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLearn));
     if (FUSE && threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
-    const uint8_t* f = stage_fired(d, fired, FUSE);
+    if (FIRST) {
+        if (PL.learn != 0u && ccol >= PL.inCols) {  // step 0's k_bias: recurrent columns never spike there
+            const int dec = ((PL.step % max(PL.biasDownEvery, 1u)) == 0u) ? (int)PL.biasDown : 0;
+            in.biasw = (uint32_t)max(0, (int)in.biasw - dec);
+            if (col < d.ncol && lane < 32u) d.bias[col * NPC + lane] = in.biasw;
+        }
+        if (col < d.ncol && lane == 0u) {
+            d.spk[1][col] = in.prev;
+            if (PL.histOn != 0u) d.hist[(size_t)PL.step * d.ncol + col] = in.prev;
+        }
+    }
+    const uint8_t* f = stage_fired(d, fired, FUSE, FIRST ? PL.inCols : 0u);
     if (FUSE) {
         if (col < d.ncol && lane < 32u && learns(PL, in.prev, vi.marg))
             vote_column<4>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
@@ -2259,11 +2297,16 @@ static hipError_t launch(F k, dim3 g, dim3 b, uint32_t lds, hipStream_t s, hipEv
 }
 
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                        const Params* PL) {
+                        const Params* PL, const Params* P0) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* prevIn = d.spk[(t & 1u) ^ 1u];  // step t - 1's presynaptic spikes (its plasticity reads them)
     const dim3 g(blocks_for(d.ncol, 4));
     const uint32_t lds = fired_lds_bytes(d.ncol);
+    if (P0) {  // step 1 taking step 0's work (k_drive<MODE, false, true>)
+        if (P.mode == 0u) return launch(k_drive<0, false, true>, g, dim3(256), lds, s, e0, e1, d, P, d.stim, *P0, in);
+        if (P.mode == 1u) return launch(k_drive<1, false, true>, g, dim3(256), lds, s, e0, e1, d, P, d.stim, *P0, in);
+        return launch(k_drive<2, false, true>, g, dim3(256), lds, s, e0, e1, d, P, d.stim, *P0, in);
+    }
     if (PL) {
         if (P.mode == 0u) return launch(k_drive<0, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
         if (P.mode == 1u) return launch(k_drive<1, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);

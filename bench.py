@@ -60,14 +60,16 @@ def acc_bytes(cfg):
     return 2 if 32 * per_branch + cfg.facil <= 0xFFFF else 4
 
 
-def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0, last_lines=None, last_emits=0):
+def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0, last_lines=None, last_emits=0,
+                 first_launches=None):
     """Algorithmic HBM bytes per kernel over `steps` learning timesteps
     (bias=False: readout steps without the intrinsic bias, which neither read
     nor update it; aw: bytes per accumulator, acc_bytes; firsts: how many of
-    the steps are the first of a presentation without side statistics, one
-    launch that reads the stimulus (4 B/column) and writes the spike,
-    history, margin and spiking flag (16 B/column) and the bias
-    read-modify-write (8 B/neuron), with no propagation).
+    the steps are the first of a presentation without side statistics, which
+    read the stimulus (4 B/column) and write the spike, history, margin and
+    spiking flag (16 B/column) and the bias read-modify-write (8 B/neuron),
+    with no propagation: a launch of their own for first_launches of them
+    (None: all), the others' work being done by the next step's propagation).
 
     The synapse planes are word-major tiles, so a kernel touches only the
     128-byte lines of presynaptic columns that spiked (device line counters).
@@ -92,16 +94,19 @@ def kernel_bytes(ncol, steps, prop_lines, learn_lines, bias=True, aw=2, firsts=0
     prop += 1024 * (learn_lines - last)
     per_emit = aw * n + 8 * bb * n + 20 * ncol
     emit = (full - last_emits) * per_emit
-    first = firsts * (8 * bb * n + 20 * ncol)
+    nf = firsts if first_launches is None else first_launches
+    first = nf * (8 * bb * n + 20 * ncol)
+    prop += (firsts - nf) * (8 * bb * n + 20 * ncol)
     last_emit = 1024 * last + last_emits * per_emit
     return {"propagation": prop, "first_emit": first, "topk_emit": emit, "last_emit": last_emit}
 
 
-def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0, last_lines=None, last_emits=0):
+def whole_run_hbm(ncol, steps, lines, seconds, bias=True, aw=2, firsts=0, last_lines=None, last_emits=0,
+                  first_launches=None):
     """Algorithmic HBM bytes of every kernel of a run (kernel_bytes over all its
     timesteps and device line counters) over the run's wall time: the average
     rate of the whole job, idle launches and gaps included."""
-    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts, last_lines, last_emits)
+    by = kernel_bytes(ncol, steps, lines[0], lines[1], bias, aw, firsts, last_lines, last_emits, first_launches)
     tot = sum(by.values())
     gbps = tot / seconds / 1e9
     return {"bytes": int(tot), "by_kernel": {k: int(v) for k, v in by.items()}, "GBps": round(gbps, 1),
@@ -245,9 +250,10 @@ def gpu_bench(args):
     assert (prop2, learn2) == (prop, learn), "twin network diverged"
     steps = args.steps * T
     # training presentations are asynchronous: each one's first step is one launch (first_emit)
-    # each presentation's last timestep learns: its emit launch runs its plasticity too (last_emit)
+    # each presentation's last timestep learns: its emit launch runs its plasticity too (last_emit); its
+    # first timestep runs inside the second's propagation (no first_emit launches) when clamped
     kb = kernel_bytes(ncol, steps, lines[0], lines[1], aw=acc_bytes(cfg), firsts=args.steps, last_lines=last_lines,
-                      last_emits=launches[3])
+                      last_emits=launches[3], first_launches=launches[1])
     names = ["propagation", "first_emit", "topk_emit", "last_emit"]  # the column top-K runs inside emit
     kern = {}
     for i, nm in enumerate(names):
@@ -269,7 +275,7 @@ def gpu_bench(args):
         "prop_tiles": prop, "learn_tiles": learn,
         "kernels": kern,
         "whole_run_hbm": whole_run_hbm(ncol, steps, lines, dt, aw=acc_bytes(cfg), firsts=args.steps,
-                                       last_lines=last_lines, last_emits=launches[3]),
+                                       last_lines=last_lines, last_emits=launches[3], first_launches=launches[1]),
         "roofline": {"bound": "hbm", "kernel": dn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dn),

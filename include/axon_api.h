@@ -174,9 +174,12 @@ axon_status axon_last_step_counters(axon_net* n, uint64_t out[2]);
  * presentation still skips its side statistics and outputs);
  * axon_kernel_times returns summed device ms and launch counts for
  * [0] propagation (without side statistics including the previous
- * timestep's plasticity), [1] the first timestep of a presentation without
- * side statistics (one launch in place of the clear, the propagation and the
- * plasticity of that step), [2] column top-K + emit (k-WTA, bias),
+ * timestep's plasticity, and at the second timestep the first's work when
+ * that runs inside it), [1] the first timestep of a presentation without
+ * side statistics as a launch of its own (in place of the clear, the
+ * propagation and the plasticity of that step; none when the first timestep
+ * is clamped to a stimulus inside the input columns: then the second
+ * timestep's propagation does its work), [2] column top-K + emit (k-WTA, bias),
  * [3] plasticity launches: with side statistics the plasticity of each
  * learning timestep; without, the last timestep's emit launch, which also
  * runs that timestep's plasticity (its emit is not in [2]). */
