@@ -1298,10 +1298,11 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
     P.facil = 0u;
     P.useBias = 1u;
     const uint32_t T = b->T, clampSteps = b->trainClamp;
-    // with two or more steps, step 0 (spikes start cleared) is one launch that also does the begin work
-    const bool first = T >= 2u;
+    // with two or more steps, step 0 (spikes start cleared) is one launch that also does the begin work,
+    // or, when clamped, runs inside step 1's propagation
+    const bool first = T >= 2u, step0 = first && clampSteps >= 1u;
     hipError_t e = first ? hipSuccess : axon::launch_multi_begin(m, b->ncol, b->nstats, s);
-    Params PL{};
+    Params PL{}, S0{};
     bool pending = false;
     for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
         P.step = t;
@@ -1311,10 +1312,11 @@ static hipError_t multi_enqueue_round(const axon_multi* b, const axon::MultiDev&
         P.depress = (t % 2u == 1u) ? 1u : 0u;
         P.phase = !b->contrast ? 0u : (t < clampSteps) ? 1u : (((t - clampSteps) % b->ltdEvery == 0u) ? 2u : 3u);
         if (first && t == 0u) {
-            e = axon::launch_multi_emit_first(m, P, true, b->ncol, s);
+            if (step0) S0 = P;
+            else e = axon::launch_multi_emit_first(m, P, true, b->ncol, s);
             continue;
         }
-        e = axon::launch_multi_drive(m, P, t, b->ncol, s, pending ? &PL : nullptr);
+        e = axon::launch_multi_drive(m, P, t, b->ncol, s, pending ? &PL : nullptr, step0 && t == 1u ? &S0 : nullptr);
         if (e == hipSuccess) e = axon::launch_multi_topk(m, P, s);
         pending = P.phase != 3u;
         PL = P;

@@ -124,9 +124,10 @@ struct MultiDev {
 hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats, hipStream_t s);
 // PL (optional): the parameters of step t - 1, whose plasticity the launch
 // applies first (deferred plasticity); the last step's runs with its emit
-// (launch_multi_emit_learn)
+// (launch_multi_emit_learn).  P0 (optional, t = 1): step 0's parameters, a
+// clamped step whose work the launch takes (launch_multi_emit_first's).
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s,
-                              const Params* PL = nullptr);
+                              const Params* PL = nullptr, const Params* P0 = nullptr);
 hipError_t launch_multi_topk(const MultiDev& m, const Params& P, hipStream_t s);
 hipError_t launch_multi_emit(const MultiDev& m, const Params& P, uint32_t t, bool learn, uint32_t ncol,
                              hipStream_t s);

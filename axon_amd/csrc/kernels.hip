@@ -568,21 +568,21 @@ inline uint32_t fired_lds_bytes(uint32_t ncol) { return ncol <= FIRED_LDS_COLS ?
 
 // Stage the fired bytes of all columns in LDS while this wave's conn entries
 // load.  sync: end with a workgroup barrier even when nothing is staged.
-// inCols0 (nonzero): the previous step is a step 0 clamped to d.stim, which
-// lies inside the first inCols0 columns: its spiking-column bytes are built
-// from those columns' stimulus words instead.
+// stim0 (not null): the previous step is a step 0 clamped to the stimulus
+// stim0, which lies inside the first inCols0 columns: its spiking-column
+// bytes are built from those columns' stimulus words instead.
 __device__ __forceinline__ const uint8_t* stage_fired(const NetDev& d, uint4* lds, bool sync = false,
-                                                     uint32_t inCols0 = 0u) {
+                                                     const uint32_t* stim0 = nullptr, uint32_t inCols0 = 0u) {
     if (d.ncol > FIRED_LDS_COLS) {  // uniform over the grid
         if (sync) __syncthreads();
         return nullptr;
     }
     const uint32_t rows = (d.ncol + 15u) / 16u;
-    if (inCols0 == 0u) {
+    if (!stim0) {
         const uint4* src = reinterpret_cast<const uint4*>(d.colNZ);
         for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x) lds[i] = src[i];
     } else {
-        const uint4* sw = reinterpret_cast<const uint4*>(d.stim);  // 4 columns per load
+        const uint4* sw = reinterpret_cast<const uint4*>(stim0);  // 4 columns per load
         for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x) {
             uint32_t w[4] = {0u, 0u, 0u, 0u};
             if (16u * i < inCols0) {  // inCols0 is a multiple of 8; a row past it holds no stimulus
@@ -1105,7 +1105,7 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
             if (PL.histOn != 0u) d.hist[(size_t)PL.step * d.ncol + col] = in.prev;
         }
     }
-    const uint8_t* f = stage_fired(d, fired, FUSE, FIRST ? PL.inCols : 0u);
+    const uint8_t* f = stage_fired(d, fired, FUSE, FIRST ? d.stim : nullptr, PL.inCols);
     if (FUSE) {
         if (col < d.ncol && lane < 32u && learns(PL, in.prev, vi.marg))
             vote_column<4>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
@@ -2031,12 +2031,22 @@ static_assert(offsetof(VoteList<8>, nt) >= sizeof(uint32_t) * 4u * 2u * ITEMS_PE
 
 // Batched propagation, 8 columns per workgroup (two per wave).  FUSE: the
 // deferred plasticity of timestep t - 1 (parameters PL) first rewrites the
-// voted tiles of the workgroup's columns, as in k_drive.
-template <int MODE, bool FUSE>
+// voted tiles of the workgroup's columns, as in k_drive.  FIRST: step 1 of a
+// round whose step 0 (parameters PL) is clamped: as k_drive<MODE, false,
+// true>, with the round's work of k_mb_emit_first (the pattern from the
+// schedule into the descriptor, its clamp row into the stimulus), whose
+// clamp row is step 0's spike vector.
+template <int MODE, bool FUSE, bool FIRST = false>
 __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t, Params PL) {
     const uint32_t net = blockIdx.y;
-    const NetDev d = mb_net(m, net);
-    if (d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
+    uint32_t p = 0u;
+    if (FIRST) {
+        p = mb_pattern(m, net);
+        if (blockIdx.x == 0u && threadIdx.x == 0u) const_cast<NetDev*>(m.nets)[net].active = p;
+        if (p == 0xFFFFFFFFu) return;  // workgroup-uniform
+    }
+    const NetDev d = FIRST ? m.nets[net] : mb_net(m, net);  // FIRST: its `active` may be the last round's, unread
+    if (!FIRST && d.active == 0xFFFFFFFFu) return;  // workgroup-uniform
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
     // The vote list is used before the propagation's item lists: they share
     // LDS (8 KiB less per workgroup, so LDS does not bound the occupancy below
@@ -2049,12 +2059,26 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     const uint32_t col = colA + h, ccol = min(col, d.ncol - 1u);
     uint32_t* const s0 = d.spk[0];  // no dynamic index into the local copy
     uint32_t* const s1 = d.spk[1];
-    const uint32_t* in = (t & 1u) ? s1 : s0;
-    const DriveIn di = drive_inputs(d, in, ccol);  // di.prev: the column's spike word of step t - 1
+    const uint32_t* tab = FIRST ? reinterpret_cast<const uint32_t*>(m.stimTabs[net]) + (size_t)p * d.ncol : nullptr;
+    const uint32_t* in = FIRST ? tab : (t & 1u) ? s1 : s0;
+    DriveIn di = drive_inputs(d, in, ccol);  // di.prev: the column's spike word of step t - 1
     VoteIn vi{};
     if (FUSE) vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLearn));
     if (FUSE && threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
-    const uint8_t* f = stage_fired(d, fired, FUSE);
+    if (FIRST) {  // step 0's work for this half's column (k_mb_emit_first)
+        const uint32_t n = lane & 31u;
+        if (PL.learn != 0u && ccol >= PL.inCols) {
+            const int dec = ((PL.step % max(PL.biasDownEvery, 1u)) == 0u) ? (int)PL.biasDown : 0;
+            di.biasw = (uint32_t)max(0, (int)di.biasw - dec);
+            if (col < d.ncol) d.bias[col * NPC + n] = di.biasw;
+        }
+        if (col < d.ncol && n == 0u) {
+            d.stim[col] = di.prev;  // read by the clamped steps that follow
+            s1[col] = di.prev;
+            if (PL.histOn != 0u) d.hist[(size_t)PL.step * d.ncol + col] = di.prev;
+        }
+    }
+    const uint8_t* f = stage_fired(d, fired, FUSE, tab, PL.inCols);
     if (FUSE) {
         if (col < d.ncol && learns(PL, di.prev, vi.marg))
             vote_column<8>(d, PL, col, di.prev, di.gl, vi, h, reinterpret_cast<uint2*>(d.cntLearn), vl);
@@ -2478,9 +2502,15 @@ hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats,
     return hipGetLastError();
 }
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s,
-                              const Params* PL) {
+                              const Params* PL, const Params* P0) {
     const dim3 g((ncol + 7u) / 8u, m.nnet);
     const uint32_t lds = fired_lds_bytes(ncol);
+    if (P0) {
+        if (P.mode == 0u) k_mb_drive<0, false, true><<<g, 256, lds, s>>>(m, P, t, *P0);
+        else if (P.mode == 1u) k_mb_drive<1, false, true><<<g, 256, lds, s>>>(m, P, t, *P0);
+        else k_mb_drive<2, false, true><<<g, 256, lds, s>>>(m, P, t, *P0);
+        return hipGetLastError();
+    }
     if (PL) {
         if (P.mode == 0u) k_mb_drive<0, true><<<g, 256, lds, s>>>(m, P, t, *PL);
         else if (P.mode == 1u) k_mb_drive<1, true><<<g, 256, lds, s>>>(m, P, t, *PL);
