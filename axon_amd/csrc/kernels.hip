@@ -1014,16 +1014,72 @@ __device__ __forceinline__ void vote_column(const NetDev& d, const Params& PL, u
     }
 }
 
-// The listed word lines (k_learn's per-word update, mm:471-487), dealt to the
-// workgroup's half-waves: half-wave q rewrites lines q, q + nq, q + 2nq, ...,
-// U at a time (the U lines' plane words and spike words requested before any
-// is updated); lane n of the half is neuron n.  The listed tiles
-// belong to the workgroup's NC columns, which lie among the SPAN columns
-// from tile0 on: each plane is addressed through a buffer resource over those
-// SPAN * 32 tiles (base in scalar registers, one 32-bit offset per line for
-// all four planes).
-template <uint32_t NC, uint32_t U = 4, uint32_t SPAN = NC>
-__device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL, const uint32_t* __restrict__ spkL,
+// One round of a half-wave's lines (slot u: line j0 + q + u * nq of the list).
+template <uint32_t U>
+struct LineRound { uint32_t off[U], wm[U], sp[U], a[U][4]; };
+struct PlaneRsrc { __amdgpu_buffer_rsrc_t r0, r1, r2, r3, rs; };
+
+// Slot u of the wave holds a line for its first half (and so for any of its
+// lanes) while j0 + 2 * wave + u * nq < L: wave-uniform, so the slots past
+// the list cost no instructions.  `left` = L - j0 - 2 * wave, which wraps
+// past L when the wave has no slot left.
+__device__ __forceinline__ bool slot_live(uint32_t u, uint32_t nq, uint32_t left, uint32_t L) {
+    return u * nq < left && left <= L;
+}
+
+// Requests round j0's plane words and spike words (no request for a slot
+// past the list: its offsets are SKIP, it reads 0 and writes nothing).
+template <uint32_t NC, uint32_t U>
+__device__ __forceinline__ void request_lines(LineRound<U>& R, const VoteList<NC>& vl, const PlaneRsrc& pr,
+                                              uint32_t j0, uint32_t L, uint32_t q, uint32_t nq, uint32_t wave,
+                                              uint32_t tile0) {
+    const uint32_t n = lane_id() & 31u;
+    const uint32_t left = L - j0 - 2u * wave;
+    uint32_t e[U], so[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        if (!slot_live(u, nq, left, L)) break;
+        e[u] = vl.line[min(j0 + q + u * nq, L - 1u)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        if (!slot_live(u, nq, left, L)) break;
+        const uint4 te = vl.tile[e[u] >> 3];
+        const uint32_t w = e[u] & 7u;
+        const bool ok = j0 + q + u * nq < L;
+        R.off[u] = ok ? (((te.x - tile0) * 8u + w) * 32u + n) * 4u : SKIP;
+        so[u] = ok ? (8u * te.y + w) * 4u : SKIP;
+        R.wm[u] = te.z;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        if (!slot_live(u, nq, left, L)) break;
+        R.a[u][0] = buf_ld(pr.r0, R.off[u]); R.a[u][1] = buf_ld(pr.r1, R.off[u]);
+        R.a[u][2] = buf_ld(pr.r2, R.off[u]); R.a[u][3] = buf_ld(pr.r3, R.off[u]);
+        R.sp[u] = buf_ld(pr.rs, so[u]);
+    }
+}
+
+// Applies k_learn's per-word update to round j0's lines and writes them back.
+template <uint32_t U>
+__device__ __forceinline__ void update_lines(LineRound<U>& R, const PlaneRsrc& pr, const Params& PL, uint32_t j0,
+                                             uint32_t L, uint32_t nq, uint32_t wave) {
+    const uint32_t n = lane_id() & 31u;
+    const uint32_t left = L - j0 - 2u * wave;
+    uint32_t nLTP = 0u, nLTD = 0u;  // not reported without side statistics
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        if (!slot_live(u, nq, left, L)) break;
+        learn_word(R.a[u][0], R.a[u][1], R.a[u][2], R.a[u][3], R.sp[u], ((R.wm[u] >> n) & 1u) != 0u, PL, nLTP, nLTD);
+        buf_st(pr.r0, R.off[u], R.a[u][0]); buf_st(pr.r1, R.off[u], R.a[u][1]);
+        buf_st(pr.r2, R.off[u], R.a[u][2]); buf_st(pr.r3, R.off[u], R.a[u][3]);
+    }
+}
+
+// The ROUNDS_EAGER form of rewrite_voted (below): one round at a time, every
+// slot's list entries read up front, words requested for the live slots only.
+template <uint32_t NC, uint32_t U, uint32_t SPAN>
+__device__ __forceinline__ void rewrite_rounds_eager(const NetDev& d, const Params& PL, const uint32_t* __restrict__ spkL,
                                               const VoteList<NC>& vl, uint32_t wave, uint32_t nwaves,
                                               uint32_t tile0) {
     const uint32_t lane = lane_id(), n = lane & 31u;
@@ -1065,6 +1121,57 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
             buf_st(r0, off[u], a[u][0]); buf_st(r1, off[u], a[u][1]);
             buf_st(r2, off[u], a[u][2]); buf_st(r3, off[u], a[u][3]);
         }
+    }
+}
+
+// The listed word lines (k_learn's per-word update, mm:471-487), dealt to the
+// workgroup's half-waves: half-wave q rewrites lines q, q + nq, q + 2nq, ...,
+// U at a time (the U lines' plane words and spike words requested before any
+// is updated); lane n of the half is neuron n.  The listed tiles belong to
+// the workgroup's NC columns, which lie among the SPAN columns from tile0 on:
+// each plane is addressed through a buffer resource over those SPAN * 32
+// tiles (base in scalar registers, one 32-bit offset per line for all four
+// planes).  FORM (measured per kernel): ROUNDS_EAGER reads every slot's list
+// entries up front and requests the live slots' words, one round at a time
+// (the batched kernels); ROUNDS one round at a time, live slots only;
+// PIPELINED requests the next round's words before this round's are updated
+// (two rounds of registers), so a workgroup with many lines waits for memory
+// about once instead of once per round (the listed lines are distinct, so no
+// round reads a line another writes).
+enum RewriteForm : uint32_t { ROUNDS_EAGER, ROUNDS, PIPELINED };
+template <uint32_t NC, uint32_t U = 4, uint32_t SPAN = NC, RewriteForm FORM = ROUNDS_EAGER>
+__device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL, const uint32_t* __restrict__ spkL,
+                                              const VoteList<NC>& vl, uint32_t wave, uint32_t nwaves,
+                                              uint32_t tile0) {
+    const uint32_t nq = 2u * nwaves, q = 2u * wave + (lane_id() >> 5);
+    const uint32_t L = vl.nl;  // workgroup-uniform
+    const size_t t0 = (size_t)__builtin_amdgcn_readfirstlane(tile0) * TILE_WORDS;
+    constexpr uint32_t BYTES = SPAN * NSLOT * TILE_WORDS * 4u;
+    const PlaneRsrc pr{buf_rsrc(d.W0 + t0, BYTES), buf_rsrc(d.W1 + t0, BYTES), buf_rsrc(d.W2 + t0, BYTES),
+                       buf_rsrc(d.W3 + t0, BYTES), buf_rsrc(spkL, d.ncol * 4u)};
+    const uint32_t R = nq * U;
+    if (FORM == ROUNDS) {
+        for (uint32_t j0 = 0; j0 < L; j0 += R) {
+            LineRound<U> A;
+            request_lines<NC, U>(A, vl, pr, j0, L, q, nq, wave, tile0);
+            update_lines<U>(A, pr, PL, j0, L, nq, wave);
+        }
+        return;
+    }
+    if (FORM == ROUNDS_EAGER) {
+        rewrite_rounds_eager<NC, U, SPAN>(d, PL, spkL, vl, wave, nwaves, tile0);
+        return;
+    }
+    LineRound<U> A, B;
+    if (L == 0u) return;
+    request_lines<NC, U>(A, vl, pr, 0u, L, q, nq, wave, tile0);
+    for (uint32_t j0 = 0;; j0 += 2u * R) {
+        if (j0 + R < L) request_lines<NC, U>(B, vl, pr, j0 + R, L, q, nq, wave, tile0);
+        update_lines<U>(A, pr, PL, j0, L, nq, wave);
+        if (j0 + R >= L) break;
+        if (j0 + 2u * R < L) request_lines<NC, U>(A, vl, pr, j0 + 2u * R, L, q, nq, wave, tile0);
+        update_lines<U>(B, pr, PL, j0 + R, L, nq, wave);
+        if (j0 + 2u * R >= L) break;
     }
 }
 
@@ -1111,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
             vote_column<4>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
         __syncthreads();
         if (vl.nl != 0u) {  // workgroup-uniform
-            rewrite_voted<4>(d, PL, spkL, vl, wave, 4u, blockIdx.x * 4u * NSLOT);
+            rewrite_voted<4, 4, 4, PIPELINED>(d, PL, spkL, vl, wave, 4u, blockIdx.x * 4u * NSLOT);
             __syncthreads();  // the rewritten lines are read below, by any wave of the workgroup
         }
     }
@@ -1238,7 +1345,7 @@ __global__ __launch_bounds__(1024) void k_emit_learn(NetDev d, Params P, const u
         vote_column<16>(d, P, col, bm.x, gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLast), vl);
     __syncthreads();
     if (vl.nl != 0u)  // workgroup-uniform
-        rewrite_voted<16, 4, EMIT_SPAN>(d, P, spkIn, vl, wave, 16u, emit_col(blockIdx.x, 0u) * NSLOT);
+        rewrite_voted<16, 4, EMIT_SPAN, ROUNDS>(d, P, spkIn, vl, wave, 16u, emit_col(blockIdx.x, 0u) * NSLOT);
 }
 
 // The first timestep of a presentation without side statistics.  Its
