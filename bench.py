@@ -34,9 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # --pmc FETCH_SIZE and WRITE_SIZE (means over all dispatches of the
 # Configuration C training run), each scaled by what the counter reads for a
 # 256 MiB device copy (x2.000 and x1.000); profiles/k_drive_c1_traffic.md.
-# propagation: the average launch, 2 with the previous timestep's plasticity
-# (9473536 B) to 1 without (6787482 B); plasticity: the last timestep's launch.
-MEASURED_TRAFFIC = {"propagation": 8578185, "first_emit": 1091584, "topk_emit": 1525965, "plasticity": 5272474}
+# propagation: the average launch, 6 with the previous timestep's plasticity
+# (9492173 B), 2 without (7940915 B) and 1 with the first timestep's work
+# (5001830 B) per presentation; last_emit: the last timestep's emit with its
+# plasticity.
+MEASURED_TRAFFIC = {"propagation": 8648522, "topk_emit": 1526784, "last_emit": 6796390}
 SYN_PER_TILE = 8192
 CONFIG_C = ("nand=4 target=64 k=24 dthr=1 mode=1 inbr=4 stimcols=64 contrast=1 margingate=0 "
             "ltdrep=1 ltdevery=2 biasup=73 biasdown=1 T=10 clampsteps=5").split()
