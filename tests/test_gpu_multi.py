@@ -65,13 +65,19 @@ def test_multi_rejects_mismatched_dynamics():
 
 
 @pytest.mark.parametrize("mode_args", [["mode=0", "hetero=1"], ["mode=1", "dthr=2", "boost=3"], ["mode=2", "dthr=3"],
-                                       ["target=384"], ["target=400", "topkpar=0"], ["ncol=520"]],
+                                       ["target=384"], ["target=400", "topkpar=0"], ["ncol=520"],
+                                       ["trainclamp=0"], ["T=2", "trainclamp=1"],
+                                       ["T=3", "trainclamp=2", "contrast=0"]],
                          ids=["mode0-hetero", "mode1-dthr2", "mode2", "gate-count-all-recurrent",
-                              "gate-topk-beyond-recurrent", "gate-topk-ncol520"])
+                              "gate-topk-beyond-recurrent", "gate-topk-ncol520", "unclamped-first-step",
+                              "two-steps", "three-steps-legacy-rule"])
 def test_multi_dendritic_modes(mode_args):
-    """Every branch-output mode of the two-column propagation, and the per-network top-K launch at the
+    """Every branch-output mode of the two-column propagation, the per-network top-K launch at the
     target boundaries (every recurrent column, beyond them) and at a column count that is not a multiple
-    of 32: batched training equals training alone."""
+    of 32, and the round's launch structure: a first timestep that is not clamped (its own launch), two
+    timesteps (the first inside the second's propagation, whose emit also runs the plasticity) and an odd
+    timestep count under the legacy rule (every timestep learns): batched training equals training
+    alone."""
     specs = [(4, 0, 2), (6, 1, 1), (5, 2, 3)]
     batched = [axon_amd.Net(axon_amd.Config.parse(cfg_args(s, sd) + mode_args)) for s, sd, _ in specs]
     mb = axon_amd.MultiNet(batched)

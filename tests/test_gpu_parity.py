@@ -118,6 +118,48 @@ def test_protocol_parity(variant):
     net.close()
 
 
+@pytest.mark.parametrize("case", ["clamped", "unclamped_first_step", "stim_outside_inputs", "readout", "two_steps"])
+def test_asynchronous_presentations_match_oracle(case):
+    """Presentations without side statistics take the launch paths that fold
+    timesteps together: a first timestep clamped to a stimulus inside the input
+    columns runs inside the second timestep's propagation, each learning
+    timestep's plasticity inside the next propagation and the last one's inside
+    its emit.  An unclamped first timestep, or a stimulus with a bit outside the
+    input columns, keeps the first timestep's own launch.  After every
+    presentation the history rows, final spikes, accumulators, stimulus, bias
+    and all four planes equal the oracle's."""
+    args = C0[:]
+    cfg = axon_amd.Config.parse(args)
+    net = axon_amd.Net(cfg)
+    ora = oracle.OracleNet(oracle.parse(args), threads=4)
+    ncol, nin = cfg.ncol, cfg.incols
+    T, clamp, learn = cfg.T, cfg.trainclamp, True
+    if case == "unclamped_first_step":
+        clamp = 0
+    elif case == "readout":
+        learn = False
+    elif case == "two_steps":
+        T, clamp = 2, 1
+    for i in range(6):
+        p = i % cfg.stim
+        stim = None
+        if case == "stim_outside_inputs":
+            stim = np.zeros(ncol, np.uint32)
+            stim[ora.stim_cols(p)] = 0xFFFFFFFF
+            stim[nin + 7 + i] = 0x00F0F00F  # a recurrent column: its word is no spike vector of step 0
+        print(f"async {case}: presentation {i}", flush=True)
+        net.present(p, 1.0, T, clamp, learn, False, 1000 + i, stim=stim, outputs=False)
+        ora.present(p, 1.0, T, clamp, learn, 1000 + i, stim=stim)
+        net.sync()
+        tag = f"async {case} presentation {i}"
+        assert np.array_equal(net.read("HIST")[: T * ncol], ora.buf("HIST")[: T * ncol]), f"{tag}: hist"
+        assert np.array_equal(net.read("SPK_A" if T % 2 == 0 else "SPK_B"), ora.final_spikes(T)), f"{tag}: spikes"
+        assert np.array_equal(net.read("ACC"), ora.buf("ACC")), f"{tag}: acc"
+        assert np.array_equal(net.read("STIM"), ora.buf("STIM")), f"{tag}: stim"
+        compare_state(net, ora, tag)
+    net.close()
+
+
 def test_wdiag_parity():
     args = C0[:]
     net = axon_amd.Net(axon_amd.Config.parse(args))
