@@ -123,9 +123,10 @@ uint32_t argu(int argc, const char* const* argv, const char* key, uint32_t dflt)
 
 struct GraphKey {
     uint32_t T, clamp, learn, k, facil, useBias, stats, step0;
+    uint32_t row;  // 1 + the pattern whose table row step 1 reads, or 0 (the stimulus is copied)
     bool operator<(const GraphKey& o) const {
-        return std::tie(T, clamp, learn, k, facil, useBias, stats, step0) <
-               std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias, o.stats, o.step0);
+        return std::tie(T, clamp, learn, k, facil, useBias, stats, step0, row) <
+               std::tie(o.T, o.clamp, o.learn, o.k, o.facil, o.useBias, o.stats, o.step0, o.row);
     }
 };
 
@@ -145,6 +146,7 @@ struct axon_net {
     hipEvent_t stim_ev[NSTAGE] = {};
     bool stim_ev_live[NSTAGE] = {};
     uint32_t stage = 0;
+    uint32_t* d_tab = nullptr;    // the patterns' stimuli at keep 1.0, NCOL words each (built at first use)
     uint32_t* d_small = nullptr;  // 64 words of readout counters
     uint32_t* d_ball = nullptr;   // NCOL words
     uint32_t* d_mask = nullptr;   // N words
@@ -393,9 +395,12 @@ static Params step_params(const axon_net* n, Params P, uint32_t t, uint32_t clam
 // step -- propagation, first-step emit, emit, plasticity -- and plast[t]
 // records which steps launched the plasticity kernel.
 // step0: step 0 runs inside step 1's propagation (a first-launch
-// presentation, clamped at step 0 to a stimulus inside the input columns).
+// presentation, clamped at step 0 to a stimulus inside the input columns),
+// which reads the stimulus from stim0 (d.stim when null, or a row of the
+// pattern table) and keeps it in d.stim.
 static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t clampSteps, bool learn, bool stats,
-                                bool step0, hipEvent_t* ev = nullptr, uint8_t* plast = nullptr) {
+                                bool step0, const uint32_t* stim0 = nullptr, hipEvent_t* ev = nullptr,
+                                uint8_t* plast = nullptr) {
     const bool first = !stats && T >= 2u;
     step0 = step0 && first && clampSteps >= 1u;
     hipError_t e = first ? hipSuccess : axon::launch_clear(n->d, n->stream);
@@ -410,7 +415,7 @@ static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t cla
         }
         const Params S0 = step_params(n, P0, 0u, clampSteps, learn);
         e = axon::launch_drive(n->d, P, t, n->stream, E(t, 0), E(t, 1), pending ? &PL : nullptr,
-                               step0 && t == 1u ? &S0 : nullptr);
+                               step0 && t == 1u ? &S0 : nullptr, stim0);
         pending = false;
         if (e == hipSuccess && !stats && learn && P.phase != 3u && t + 1u == T) {
             // the last step's plasticity in the same launch as its emit
@@ -434,7 +439,7 @@ static hipError_t enqueue_steps(axon_net* n, Params P0, uint32_t T, uint32_t cla
 // The launches of enqueue_steps, each carrying start / end timing events, so
 // the kernels' device durations can be summed after the presentation.
 static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint32_t clampSteps, bool learn,
-                                         bool stats, bool step0) {
+                                         bool stats, bool step0, const uint32_t* stim0) {
     const size_t need = (size_t)8 * std::max<uint32_t>(T, 1);
     while (n->prof_ev.size() < need) {
         hipEvent_t e;
@@ -443,7 +448,7 @@ static hipError_t enqueue_steps_profiled(axon_net* n, Params P, uint32_t T, uint
         n->prof_ev.push_back(e);
     }
     std::vector<uint8_t> plast(T, 0);  // steps with a plasticity launch: 1 its own, 2 with the emit
-    hipError_t e = enqueue_steps(n, P, T, clampSteps, learn, stats, step0, n->prof_ev.data(), plast.data());
+    hipError_t e = enqueue_steps(n, P, T, clampSteps, learn, stats, step0, stim0, n->prof_ev.data(), plast.data());
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return e;
     const bool first = !stats && T >= 2u;  // as enqueue_steps
@@ -479,41 +484,62 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
     P.facil = learn ? 0u : n->facil;
     P.useBias = (a->use_bias == 2u) ? (learn ? 1u : 0u) : a->use_bias;
 
-    // stimulus: loadStim (mm:800-804) into a pinned staging slot
-    const uint32_t slot = n->stage;
-    n->stage = (n->stage + 1u) % NSTAGE;
-    if (n->stim_ev_live[slot]) HIPCHECK(hipEventSynchronize(n->stim_ev[slot]));
-    uint32_t* hs = n->h_stim[slot];
-    if (a->stim) {
-        memcpy(hs, a->stim, (size_t)n->NCOL * 4);
-    } else {
-        memset(hs, 0, (size_t)n->NCOL * 4);
-        std::mt19937 r(a->seed);
-        for (uint32_t c : n->stimCols[a->pattern])
-            if (axon::canonical01(r) <= a->keep) hs[c] = 0xFFFFFFFFu;
-    }
-    // Step 0 of a first-launch presentation clamped at step 0 runs inside step
-    // 1's propagation when the stimulus lies inside the input columns (then
-    // it is step 0's spike vector).
+    // Stimulus: loadStim (mm:800-804).  Step 0 of a first-launch presentation
+    // clamped at step 0 runs inside step 1's propagation when the stimulus lies
+    // inside the input columns (then it is step 0's spike vector).  A stored
+    // pattern at keep 1.0 (every column of the pattern, whatever the seed:
+    // canonical01 < 1) is then read from the pattern table, with no copy; any
+    // other stimulus goes through a pinned staging slot into d.stim.
     bool step0 = async && a->T >= 2u && a->clamp_steps >= 1u;
-    for (uint32_t c = n->cfg.incols; step0 && c < n->NCOL; ++c) step0 = hs[c] == 0u;
-    HIPCHECK(hipMemcpyAsync(n->d.stim, hs, (size_t)n->NCOL * 4, hipMemcpyHostToDevice, n->stream));
-    HIPCHECK(hipEventRecord(n->stim_ev[slot], n->stream));
-    n->stim_ev_live[slot] = true;
+    const uint32_t* stim0 = nullptr;
+    uint32_t row = 0u;
+    if (step0 && !a->stim && a->keep >= 1.0f) {
+        if (!n->d_tab) {
+            const uint32_t np = (uint32_t)n->stimCols.size();
+            std::vector<uint32_t> tab((size_t)np * n->NCOL, 0u);
+            for (uint32_t q = 0; q < np; ++q)
+                for (uint32_t c : n->stimCols[q]) tab[(size_t)q * n->NCOL + c] = 0xFFFFFFFFu;
+            void* p = nullptr;
+            const axon_status st = dev_alloc(n, &p, tab.size() * 4);
+            if (st != AXON_OK) return st;
+            n->d_tab = static_cast<uint32_t*>(p);
+            HIPCHECK(hipMemcpyAsync(p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, n->stream));
+            HIPCHECK(hipStreamSynchronize(n->stream));  // tab goes away on return
+        }
+        row = a->pattern + 1u;
+        stim0 = n->d_tab + (size_t)a->pattern * n->NCOL;
+    } else {
+        const uint32_t slot = n->stage;  // the presentation's until the device has passed the copy (stim_ev)
+        n->stage = (n->stage + 1u) % NSTAGE;
+        if (n->stim_ev_live[slot]) HIPCHECK(hipEventSynchronize(n->stim_ev[slot]));
+        uint32_t* hs = n->h_stim[slot];
+        if (a->stim) {
+            memcpy(hs, a->stim, (size_t)n->NCOL * 4);
+        } else {
+            memset(hs, 0, (size_t)n->NCOL * 4);
+            std::mt19937 r(a->seed);
+            for (uint32_t c : n->stimCols[a->pattern])
+                if (axon::canonical01(r) <= a->keep) hs[c] = 0xFFFFFFFFu;
+        }
+        for (uint32_t c = n->cfg.incols; step0 && c < n->NCOL; ++c) step0 = hs[c] == 0u;
+        HIPCHECK(hipMemcpyAsync(n->d.stim, hs, (size_t)n->NCOL * 4, hipMemcpyHostToDevice, n->stream));
+        HIPCHECK(hipEventRecord(n->stim_ev[slot], n->stream));
+        n->stim_ev_live[slot] = true;
+    }
 
     const auto t0 = std::chrono::steady_clock::now();
     if (n->profiling) {
         // waits for the device even when async (the event times are read back)
-        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn, !async, step0);
+        hipError_t e = enqueue_steps_profiled(n, P, a->T, a->clamp_steps, learn, !async, step0, stim0);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "profiled launch: %s", hipGetErrorString(e));
     } else if (n->use_graphs && a->T > 0) {
         const GraphKey key{a->T, a->clamp_steps, learn ? 1u : 0u, P.k, P.facil, P.useBias, async ? 0u : 1u,
-                           step0 ? 1u : 0u};
+                           step0 ? 1u : 0u, row};
         auto it = n->graphs.find(key);
         if (it == n->graphs.end()) {
             hipGraph_t g = nullptr;
             HIPCHECK(hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, step0);
+            const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, step0, stim0);
             hipError_t e2 = hipStreamEndCapture(n->stream, &g);
             if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "capture launch: %s", hipGetErrorString(e));
             if (e2 != hipSuccess) return fail(AXON_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -525,7 +551,7 @@ extern "C" axon_status axon_present(axon_net* n, const axon_present_args* a, axo
         }
         HIPCHECK(hipGraphLaunch(it->second, n->stream));
     } else {
-        const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, step0);
+        const hipError_t e = enqueue_steps(n, P, a->T, a->clamp_steps, learn, !async, step0, stim0);
         if (e != hipSuccess) return fail(AXON_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
     }
     if (async) {

@@ -160,11 +160,13 @@ hipError_t launch_clear(const NetDev& d, hipStream_t s);
 // e0 / e1 (optional): timing events recorded at the start / end of the kernel.
 // PL (optional): the parameters of step t - 1, whose plasticity (without side
 // statistics) the propagation applies first to its columns' voted tiles.
-// P0 (optional, t = 1, no PL): the parameters of step 0, clamped to d.stim,
-// which lies inside the input columns; the launch also does step 0's work,
-// which then has no launch.
+// P0 (optional, t = 1, no PL): the parameters of step 0, clamped to stim0
+// (d.stim when null, else a pattern-table row, which the launch copies into
+// d.stim), which lies inside the input columns; the launch also does step
+// 0's work, which then has no launch.
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr, const Params* PL = nullptr, const Params* P0 = nullptr);
+                        hipEvent_t e1 = nullptr, const Params* PL = nullptr, const Params* P0 = nullptr,
+                        const uint32_t* stim0 = nullptr);
 // first: step 0 of a presentation without side statistics, which needs no
 // clear, propagation or plasticity launch (k_emit_first)
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,

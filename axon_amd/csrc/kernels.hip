@@ -1180,7 +1180,8 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
 // presynaptic spikes spkL) first rewrites the voted tiles of the
 // workgroup's columns.  FIRST: this is step 1 of a presentation without side
 // statistics whose step 0 (parameters PL) is clamped to a stimulus inside the
-// input columns.  Step 0's spikes are then that stimulus (spkIn = d.stim),
+// input columns.  Step 0's spikes are then that stimulus (spkIn: d.stim, or a
+// row of the pattern table, which this launch also copies into d.stim),
 // since with cleared spikes no
 // column passes the gate: this launch also does step 0's own work for its
 // columns (as k_emit_first: spike word, history row, bias decay), so step 0
@@ -1209,10 +1210,11 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
         }
         if (col < d.ncol && lane == 0u) {
             d.spk[1][col] = in.prev;
+            if (spkIn != d.stim) d.stim[col] = in.prev;  // a pattern table row: kept for the clamped steps
             if (PL.histOn != 0u) d.hist[(size_t)PL.step * d.ncol + col] = in.prev;
         }
     }
-    const uint8_t* f = stage_fired(d, fired, FUSE, FIRST ? d.stim : nullptr, PL.inCols);
+    const uint8_t* f = stage_fired(d, fired, FUSE, FIRST ? spkIn : nullptr, PL.inCols);
     if (FUSE) {
         if (col < d.ncol && lane < 32u && learns(PL, in.prev, vi.marg))
             vote_column<4>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
@@ -2428,15 +2430,16 @@ static hipError_t launch(F k, dim3 g, dim3 b, uint32_t lds, hipStream_t s, hipEv
 }
 
 hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                        const Params* PL, const Params* P0) {
+                        const Params* PL, const Params* P0, const uint32_t* stim0) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* prevIn = d.spk[(t & 1u) ^ 1u];  // step t - 1's presynaptic spikes (its plasticity reads them)
     const dim3 g(blocks_for(d.ncol, 4));
     const uint32_t lds = fired_lds_bytes(d.ncol);
     if (P0) {  // step 1 taking step 0's work (k_drive<MODE, false, true>)
-        if (P.mode == 0u) return launch(k_drive<0, false, true>, g, dim3(256), lds, s, e0, e1, d, P, d.stim, *P0, in);
-        if (P.mode == 1u) return launch(k_drive<1, false, true>, g, dim3(256), lds, s, e0, e1, d, P, d.stim, *P0, in);
-        return launch(k_drive<2, false, true>, g, dim3(256), lds, s, e0, e1, d, P, d.stim, *P0, in);
+        const uint32_t* st = stim0 ? stim0 : d.stim;
+        if (P.mode == 0u) return launch(k_drive<0, false, true>, g, dim3(256), lds, s, e0, e1, d, P, st, *P0, in);
+        if (P.mode == 1u) return launch(k_drive<1, false, true>, g, dim3(256), lds, s, e0, e1, d, P, st, *P0, in);
+        return launch(k_drive<2, false, true>, g, dim3(256), lds, s, e0, e1, d, P, st, *P0, in);
     }
     if (PL) {
         if (P.mode == 0u) return launch(k_drive<0, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
