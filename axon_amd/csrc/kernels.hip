@@ -1241,26 +1241,29 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
 // Column of wave w of emit workgroup b: the workgroups on XCD x = b % 8 take
 // the columns that propagation ran on XCD x (its 4-column workgroups j with
 // j % 8 = x), so a column's accumulators, NMDA bits and fired masks are read
-// from the L2 that received them.  The grid is ceil(ncol / 128) * 8
-// workgroups; waves past ncol emit nothing.
+// from the L2 that received them.  With WG waves (columns) per workgroup the
+// grid is ceil(ncol / (8 * WG)) * 8 workgroups; waves past ncol emit nothing.
+template <uint32_t WG = 16>
 __device__ __forceinline__ uint32_t emit_col(uint32_t b, uint32_t w) {
-    const uint32_t j = (b & 7u) + 8u * (4u * (b >> 3) + (w >> 2));
+    const uint32_t j = (b & 7u) + 8u * ((WG / 4u) * (b >> 3) + (w >> 2));
     return 4u * j + (w & 3u);
 }
 
-// The count behind the gate for the NCW columns of a 1024-thread workgroup:
-// lane j < NCW holds in dl the drive of the workgroup's column j; returns,
-// for column `mine`, the number of recurrent drives that exceed its drive.
-// Every wave of the workgroup must call it (it ends with a barrier).
-template <uint32_t NCW>
+// The count behind the gate for the NCW columns of an NT-thread workgroup
+// (up to 4096 recurrent drives held in registers): lane j < NCW holds in dl
+// the drive of the workgroup's column j; returns, for column `mine`, the
+// number of recurrent drives that exceed its drive.  Every wave of the
+// workgroup must call it (it ends with a barrier).
+template <uint32_t NCW, uint32_t NT = 1024>
 __device__ __forceinline__ uint32_t drives_above(const uint32_t* __restrict__ drive, uint32_t ncol, uint32_t inCols,
                                                  uint32_t dl, uint32_t mine) {
-    __shared__ uint32_t part[16][NCW];  // [wave][column of the workgroup]
+    constexpr uint32_t NV = 4096u / NT, NWV = NT / 64u;
+    __shared__ uint32_t part[NWV][NCW];  // [wave][column of the workgroup]
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    uint32_t rv[4];
+    uint32_t rv[NV];
 #pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-        const uint32_t c = inCols + tid + i * 1024u;
+    for (uint32_t i = 0; i < NV; ++i) {
+        const uint32_t c = inCols + tid + i * NT;
         rv[i] = c < ncol ? drive[c] : 0u;  // past the end: 0, which exceeds no drive
     }
     uint32_t own = 0u;
@@ -1269,20 +1272,22 @@ __device__ __forceinline__ uint32_t drives_above(const uint32_t* __restrict__ dr
         const uint32_t dj = __builtin_amdgcn_readlane(dl, j);
         uint32_t cnt = 0u;
 #pragma unroll
-        for (uint32_t i = 0; i < 4u; ++i) cnt += (uint32_t)__popcll(__ballot(rv[i] > dj));
+        for (uint32_t i = 0; i < NV; ++i) cnt += (uint32_t)__popcll(__ballot(rv[i] > dj));
         if (lane == j) own = cnt;
     }
     if (lane < NCW) part[wave][lane] = own;
     __syncthreads();
     uint32_t tot = 0u;
 #pragma unroll
-    for (uint32_t w = 0; w < 16u; ++w) tot += part[w][mine];
+    for (uint32_t w = 0; w < NWV; ++w) tot += part[w][mine];
     return tot;
 }
+template <uint32_t WG = 16>
 __device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive, uint32_t ncol, uint32_t inCols,
                                               uint32_t target, uint32_t dv) {
-    const uint32_t dl = drive[min(emit_col(blockIdx.x, lane_id() & 15u), ncol - 1u)];  // lane j < 16: wave j's column
-    const uint32_t above = drives_above<16>(drive, ncol, inCols, dl, threadIdx.x >> 6);  // every wave: a barrier inside
+    // lane j < WG: wave j's column
+    const uint32_t dl = drive[min(emit_col<WG>(blockIdx.x, lane_id() & (WG - 1u)), ncol - 1u)];
+    const uint32_t above = drives_above<WG, 64u * WG>(drive, ncol, inCols, dl, threadIdx.x >> 6);  // a barrier inside
     return dv >= 1u && above < target;
 }
 
@@ -1297,7 +1302,7 @@ __device__ __forceinline__ bool gate_by_count(const uint32_t* __restrict__ drive
 // a separate one-workgroup launch.
 // Every wave of the workgroup calls it: the gate ends with a workgroup
 // barrier.  Returns the column's spike word and margin flag.
-template <bool LEARN, bool STATS>
+template <bool LEARN, bool STATS, uint32_t WG = 16>
 __device__ __forceinline__ uint2 emit_step(const NetDev& d, const Params& P, const uint32_t* __restrict__ spkIn,
                                            uint32_t* __restrict__ spkOut, uint32_t col) {
     EmitIn in = emit_inputs<LEARN, STATS>(d, P, min(col, d.ncol - 1u));  // in flight during the gate
@@ -1305,26 +1310,31 @@ __device__ __forceinline__ uint2 emit_step(const NetDev& d, const Params& P, con
     bool gate;
     if (P.targetCols >= 1u && P.targetCols <= nrec && nrec <= 4096u) {  // uniform over the grid
         if (STATS && blockIdx.x == 0u)
-            (void)topk_block<1024>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
-        gate = gate_by_count(d.drive, d.ncol, P.inCols, P.targetCols, in.dv);
+            (void)topk_block<64u * WG>(d.drive, d.theta, d.ncol, P.inCols, P.targetCols, d.topkpar);
+        gate = gate_by_count<WG>(d.drive, d.ncol, P.inCols, P.targetCols, in.dv);
     } else {
-        const uint32_t th = topk_block<1024>(d.drive, blockIdx.x == 0u ? d.theta : nullptr, d.ncol, P.inCols,
-                                             P.targetCols, d.topkpar);
+        const uint32_t th = topk_block<64u * WG>(d.drive, blockIdx.x == 0u ? d.theta : nullptr, d.ncol, P.inCols,
+                                                 P.targetCols, d.topkpar);
         gate = in.dv >= th;
     }
     if (col < d.ncol && (gate || (P.clamp != 0u && col < P.inCols)))  // wave-uniform: this column can spike
         emit_inputs_spiking<LEARN, STATS>(d, P, col, in);
     return emit_cols<LEARN, STATS>(d, P, spkIn, spkOut, col, in, gate);
 }
-template <bool LEARN, bool STATS>
-__global__ __launch_bounds__(1024) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
-                                               uint32_t* __restrict__ spkOut) {
-    (void)emit_step<LEARN, STATS>(d, P, spkIn, spkOut, emit_col(blockIdx.x, threadIdx.x >> 6));
+// WG columns (waves) per workgroup: 16, or 8 for the training emit
+// (EMIT_TRAIN_WAVES).
+template <bool LEARN, bool STATS, uint32_t WG = 16>
+__global__ __launch_bounds__(64 * WG) void k_emit(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                                  uint32_t* __restrict__ spkOut) {
+    (void)emit_step<LEARN, STATS, WG>(d, P, spkIn, spkOut, emit_col<WG>(blockIdx.x, threadIdx.x >> 6));
 }
+constexpr uint32_t EMIT_TRAIN_WAVES = 8;
+constexpr uint32_t EMIT_LEARN_WAVES = 16;
 
-// Columns from an emit workgroup's first to its last (emit_col: four runs of
-// four columns, 32 apart).
-constexpr uint32_t EMIT_SPAN = 4u * 8u * 3u + 4u;
+// Columns from an emit workgroup's first to its last (emit_col: WG / 4 runs
+// of four columns, 32 apart).
+template <uint32_t WG>
+constexpr uint32_t emit_span() { return 4u * 8u * (WG / 4u - 1u) + 4u; }
 
 // The last timestep of a presentation without side statistics, when it
 // learns: the emit of k_emit<true, false>, then that step's plasticity for
@@ -1333,21 +1343,22 @@ constexpr uint32_t EMIT_SPAN = 4u * 8u * 3u + 4u;
 // fired masks and band flags (read with the emit's inputs) and the step's
 // presynaptic spikes, so no launch of its own ends the presentation.  Its
 // work is counted in cntLast.
-__global__ __launch_bounds__(1024) void k_emit_learn(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
-                                                     uint32_t* __restrict__ spkOut) {
-    __shared__ VoteList<16> vl;
+template <uint32_t WG>
+__global__ __launch_bounds__(64 * WG) void k_emit_learn(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                                        uint32_t* __restrict__ spkOut) {
+    __shared__ VoteList<WG> vl;
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t col = emit_col(blockIdx.x, wave), ccol = min(col, d.ncol - 1u);
+    const uint32_t col = emit_col<WG>(blockIdx.x, wave), ccol = min(col, d.ncol - 1u);
     const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
     const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
     if (threadIdx.x == 0u) { vl.nt = 0u; vl.nl = 0u; }
     __syncthreads();
-    const uint2 bm = emit_step<true, false>(d, P, spkIn, spkOut, col);
+    const uint2 bm = emit_step<true, false, WG>(d, P, spkIn, spkOut, col);
     if (col < d.ncol && lane < 32u && learns(P, bm.x, bm.y))
-        vote_column<16>(d, P, col, bm.x, gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLast), vl);
+        vote_column<WG>(d, P, col, bm.x, gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLast), vl);
     __syncthreads();
     if (vl.nl != 0u)  // workgroup-uniform
-        rewrite_voted<16, 4, EMIT_SPAN, ROUNDS>(d, P, spkIn, vl, wave, 16u, emit_col(blockIdx.x, 0u) * NSLOT);
+        rewrite_voted<WG, 4, emit_span<WG>(), ROUNDS>(d, P, spkIn, vl, wave, WG, emit_col<WG>(blockIdx.x, 0u) * NSLOT);
 }
 
 // The first timestep of a presentation without side statistics.  Its
@@ -2460,8 +2471,9 @@ hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_
 
 hipError_t launch_emit_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0,
                              hipEvent_t e1) {
-    const dim3 g(blocks_for(d.ncol, 128) * 8u), b(1024);  // as launch_emit
-    return launch(k_emit_learn, g, b, 0u, s, e0, e1, d, P, d.spk[t & 1u], d.spk[(t & 1u) ^ 1u]);
+    constexpr uint32_t W = EMIT_LEARN_WAVES;
+    const dim3 g(blocks_for(d.ncol, 8u * W) * 8u), b(64u * W);  // as launch_emit
+    return launch(k_emit_learn<W>, g, b, 0u, s, e0, e1, d, P, d.spk[t & 1u], d.spk[(t & 1u) ^ 1u]);
 }
 
 hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn, bool stats, hipStream_t s,
@@ -2474,7 +2486,11 @@ hipError_t launch_emit(const NetDev& d, const Params& P, uint32_t t, bool learn,
         return launch(k_emit_first<false>, g, b, 0u, s, e0, e1, d, P, out);
     }
     if (learn && stats) return launch(k_emit<true, true>, g, b, 0u, s, e0, e1, d, P, in, out);
-    if (learn) return launch(k_emit<true, false>, g, b, 0u, s, e0, e1, d, P, in, out);
+    if (learn) {
+        constexpr uint32_t W = EMIT_TRAIN_WAVES;
+        return launch(k_emit<true, false, W>, dim3(blocks_for(d.ncol, 8u * W) * 8u), dim3(64u * W), 0u, s, e0, e1,
+                      d, P, in, out);
+    }
     if (stats) return launch(k_emit<false, true>, g, b, 0u, s, e0, e1, d, P, in, out);
     return launch(k_emit<false, false>, g, b, 0u, s, e0, e1, d, P, in, out);
 }
