@@ -343,6 +343,8 @@ __device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t*
 // LDS item list capacity of a half-wave: the branches are split where the
 // column's items halve, so a half holds at most (256 + 8) / 2 items.
 constexpr uint32_t ITEMS_PER_HALF = (NSLOT * 8u + 8u) / 2u;
+// Columns per propagation workgroup (k_drive), one wave each.
+constexpr uint32_t DRIVE_COLS = 16;
 
 // Work items are (branch, spiking presynaptic column) pairs, one 128-byte W3
 // line each, listed in LDS with their load offsets ready-made (the drive
@@ -1175,7 +1177,7 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
     }
 }
 
-// Propagation of one network, 4 columns per workgroup, one per wave.  FUSE:
+// Propagation of one network, DRIVE_COLS columns per workgroup, one per wave.  FUSE:
 // the deferred plasticity of the previous timestep (parameters PL,
 // presynaptic spikes spkL) first rewrites the voted tiles of the
 // workgroup's columns.  FIRST: this is step 1 of a presentation without side
@@ -1188,15 +1190,16 @@ __device__ __forceinline__ void rewrite_voted(const NetDev& d, const Params& PL,
 // needs no launch.  Step 0's margin and spiking flags are not written: no
 // plasticity reads them (step 0 votes no tile) and the next emit rewrites them.
 template <int MODE, bool FUSE, bool FIRST = false>
-__global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn, Params PL,
-                                               const uint32_t* __restrict__ spkL) {
+__global__ __launch_bounds__(64 * DRIVE_COLS) void k_drive(NetDev d, Params P, const uint32_t* __restrict__ spkIn,
+                                                           Params PL, const uint32_t* __restrict__ spkL) {
+    constexpr uint32_t DC = DRIVE_COLS;
     extern __shared__ uint4 fired[];  // fired_lds_bytes(ncol)
-    __shared__ __align__(16) uint32_t items[4][2 * ITEMS_PER_HALF];
-    __shared__ VoteList<4> vl;
+    __shared__ __align__(16) uint32_t items[DC][2 * ITEMS_PER_HALF];
+    __shared__ VoteList<DC> vl;
     // the learning-column list of step t-1 has been consumed; free it for step t+1
     if (blockIdx.x == 0u && threadIdx.x < NLISTS) *list_count(d, (P.step & 1u) ^ 1u, threadIdx.x) = 0u;
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t col = blockIdx.x * 4u + wave;
+    const uint32_t col = blockIdx.x * DC + wave;
     const uint32_t ccol = min(col, d.ncol - 1u);
     DriveIn in = drive_inputs(d, spkIn, ccol);  // in.prev: the column's spike word of the previous step
     VoteIn vi{};
@@ -1217,10 +1220,10 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
     const uint8_t* f = stage_fired(d, fired, FUSE, FIRST ? spkIn : nullptr, PL.inCols);
     if (FUSE) {
         if (col < d.ncol && lane < 32u && learns(PL, in.prev, vi.marg))
-            vote_column<4>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
+            vote_column<DC>(d, PL, col, in.prev, in.gl, vi, 0u, reinterpret_cast<uint2*>(d.cntLearn), vl);
         __syncthreads();
         if (vl.nl != 0u) {  // workgroup-uniform
-            rewrite_voted<4, 4, 4, PIPELINED>(d, PL, spkL, vl, wave, 4u, blockIdx.x * 4u * NSLOT);
+            rewrite_voted<DC, 4, DC, PIPELINED>(d, PL, spkL, vl, wave, DC, blockIdx.x * DC * NSLOT);
             __syncthreads();  // the rewritten lines are read below, by any wave of the workgroup
         }
     }
@@ -1239,14 +1242,24 @@ __global__ __launch_bounds__(256) void k_drive(NetDev d, Params P, const uint32_
 // barrier each wave adds up its own column's 16 partial counts.  This takes
 // the place of the two or three histogram passes of the radix select.
 // Column of wave w of emit workgroup b: the workgroups on XCD x = b % 8 take
-// the columns that propagation ran on XCD x (its 4-column workgroups j with
-// j % 8 = x), so a column's accumulators, NMDA bits and fired masks are read
-// from the L2 that received them.  With WG waves (columns) per workgroup the
-// grid is ceil(ncol / (8 * WG)) * 8 workgroups; waves past ncol emit nothing.
+// the columns that propagation ran on XCD x (its DRIVE_COLS-column
+// workgroups j with j % 8 = x, whole or in parts of WG columns), so a
+// column's accumulators, NMDA bits and fired masks are read from the L2 that
+// received them.  With WG waves
+// (columns) per workgroup the grid is ceil(ncol / (8 * WG)) * 8 workgroups;
+// waves past ncol emit nothing.
 template <uint32_t WG = 16>
 __device__ __forceinline__ uint32_t emit_col(uint32_t b, uint32_t w) {
-    const uint32_t j = (b & 7u) + 8u * ((WG / 4u) * (b >> 3) + (w >> 2));
-    return 4u * j + (w & 3u);
+    constexpr uint32_t DC = DRIVE_COLS;
+    if (WG >= DC) {  // G whole propagation workgroups per emit workgroup
+        constexpr uint32_t G = WG / DC;
+        static_assert(WG < DC || G * DC == WG, "whole propagation workgroups");
+        const uint32_t j = (b & 7u) + 8u * (G * (b >> 3) + w / DC);
+        return DC * j + w % DC;
+    }
+    constexpr uint32_t H = DC / WG;  // emit workgroups per propagation workgroup
+    const uint32_t r = b >> 3, j = (b & 7u) + 8u * (r / H);
+    return DC * j + (r % H) * WG + w;
 }
 
 // The count behind the gate for the NCW columns of an NT-thread workgroup
@@ -1331,10 +1344,12 @@ __global__ __launch_bounds__(64 * WG) void k_emit(NetDev d, Params P, const uint
 constexpr uint32_t EMIT_TRAIN_WAVES = 8;
 constexpr uint32_t EMIT_LEARN_WAVES = 16;
 
-// Columns from an emit workgroup's first to its last (emit_col: WG / 4 runs
-// of four columns, 32 apart).
+// Columns from an emit workgroup's first to its last (emit_col: WG /
+// DRIVE_COLS runs of DRIVE_COLS columns, 8 * DRIVE_COLS apart, or WG
+// consecutive columns when the emit workgroup takes part of a propagation
+// workgroup).
 template <uint32_t WG>
-constexpr uint32_t emit_span() { return 4u * 8u * (WG / 4u - 1u) + 4u; }
+constexpr uint32_t emit_span() { return WG >= DRIVE_COLS ? DRIVE_COLS * 8u * (WG / DRIVE_COLS - 1u) + DRIVE_COLS : WG; }
 
 // The last timestep of a presentation without side statistics, when it
 // learns: the emit of k_emit<true, false>, then that step's plasticity for
@@ -2444,22 +2459,23 @@ hipError_t launch_drive(const NetDev& d, const Params& P, uint32_t t, hipStream_
                         const Params* PL, const Params* P0, const uint32_t* stim0) {
     const uint32_t* in = d.spk[t & 1u];
     const uint32_t* prevIn = d.spk[(t & 1u) ^ 1u];  // step t - 1's presynaptic spikes (its plasticity reads them)
-    const dim3 g(blocks_for(d.ncol, 4));
+    const dim3 g(blocks_for(d.ncol, DRIVE_COLS));
+    const dim3 bl(64u * DRIVE_COLS);
     const uint32_t lds = fired_lds_bytes(d.ncol);
     if (P0) {  // step 1 taking step 0's work (k_drive<MODE, false, true>)
         const uint32_t* st = stim0 ? stim0 : d.stim;
-        if (P.mode == 0u) return launch(k_drive<0, false, true>, g, dim3(256), lds, s, e0, e1, d, P, st, *P0, in);
-        if (P.mode == 1u) return launch(k_drive<1, false, true>, g, dim3(256), lds, s, e0, e1, d, P, st, *P0, in);
-        return launch(k_drive<2, false, true>, g, dim3(256), lds, s, e0, e1, d, P, st, *P0, in);
+        if (P.mode == 0u) return launch(k_drive<0, false, true>, g, bl, lds, s, e0, e1, d, P, st, *P0, in);
+        if (P.mode == 1u) return launch(k_drive<1, false, true>, g, bl, lds, s, e0, e1, d, P, st, *P0, in);
+        return launch(k_drive<2, false, true>, g, bl, lds, s, e0, e1, d, P, st, *P0, in);
     }
     if (PL) {
-        if (P.mode == 0u) return launch(k_drive<0, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
-        if (P.mode == 1u) return launch(k_drive<1, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
-        return launch(k_drive<2, true>, g, dim3(256), lds, s, e0, e1, d, P, in, *PL, prevIn);
+        if (P.mode == 0u) return launch(k_drive<0, true>, g, bl, lds, s, e0, e1, d, P, in, *PL, prevIn);
+        if (P.mode == 1u) return launch(k_drive<1, true>, g, bl, lds, s, e0, e1, d, P, in, *PL, prevIn);
+        return launch(k_drive<2, true>, g, bl, lds, s, e0, e1, d, P, in, *PL, prevIn);
     }
-    if (P.mode == 0u) return launch(k_drive<0, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
-    if (P.mode == 1u) return launch(k_drive<1, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
-    return launch(k_drive<2, false>, g, dim3(256), lds, s, e0, e1, d, P, in, P, prevIn);
+    if (P.mode == 0u) return launch(k_drive<0, false>, g, bl, lds, s, e0, e1, d, P, in, P, prevIn);
+    if (P.mode == 1u) return launch(k_drive<1, false>, g, bl, lds, s, e0, e1, d, P, in, P, prevIn);
+    return launch(k_drive<2, false>, g, bl, lds, s, e0, e1, d, P, in, P, prevIn);
 }
 
 hipError_t launch_learn(const NetDev& d, const Params& P, uint32_t t, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
