@@ -343,7 +343,10 @@ __device__ __forceinline__ DriveIn drive_inputs(const NetDev& d, const uint32_t*
 // LDS item list capacity of a half-wave: the branches are split where the
 // column's items halve, so a half holds at most (256 + 8) / 2 items.
 constexpr uint32_t ITEMS_PER_HALF = (NSLOT * 8u + 8u) / 2u;
-// Columns per propagation workgroup (k_drive), one wave each.
+// Columns per propagation workgroup (k_drive), one wave each.  Measured on
+// one 131072-neuron network: 4 columns 16.05k GSyn/s, 8 columns 16.3k, 16
+// columns 16.4k (a heavy learning column's voted lines spread over more
+// half-waves).
 constexpr uint32_t DRIVE_COLS = 16;
 
 // Work items are (branch, spiking presynaptic column) pairs, one 128-byte W3
@@ -2157,14 +2160,17 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
     if (i < d.nstats) d.stats[i] = 0u;
 }
 
+// Columns per batched propagation workgroup (two per wave).  Measured on 25
+// networks: 8 columns 906 ms, 16 columns 966 ms, 32 columns 1083 ms.
+constexpr uint32_t MB_DRIVE_COLS = 8;
 union MbDriveLds {
-    VoteList<8> vl;
-    uint32_t items[4][2 * ITEMS_PER_COL];
+    VoteList<MB_DRIVE_COLS> vl;
+    uint32_t items[MB_DRIVE_COLS / 2][2 * ITEMS_PER_COL];
 };
-static_assert(offsetof(VoteList<8>, nt) >= sizeof(uint32_t) * 4u * 2u * ITEMS_PER_COL,
+static_assert(offsetof(VoteList<MB_DRIVE_COLS>, nt) >= sizeof(uint32_t) * (MB_DRIVE_COLS / 2) * 2u * ITEMS_PER_COL,
               "the vote counts must not share LDS with the item lists");
 
-// Batched propagation, 8 columns per workgroup (two per wave).  FUSE: the
+// Batched propagation, MB_DRIVE_COLS columns per workgroup (two per wave).  FUSE: the
 // deferred plasticity of timestep t - 1 (parameters PL) first rewrites the
 // voted tiles of the workgroup's columns, as in k_drive.  FIRST: step 1 of a
 // round whose step 0 (parameters PL) is clamped: as k_drive<MODE, false,
@@ -2172,7 +2178,7 @@ static_assert(offsetof(VoteList<8>, nt) >= sizeof(uint32_t) * 4u * 2u * ITEMS_PE
 // schedule into the descriptor, its clamp row into the stimulus), whose
 // clamp row is step 0's spike vector.
 template <int MODE, bool FUSE, bool FIRST = false>
-__global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t t, Params PL) {
+__global__ __launch_bounds__(32 * MB_DRIVE_COLS) void k_mb_drive(MultiDev m, Params P, uint32_t t, Params PL) {
     const uint32_t net = blockIdx.y;
     uint32_t p = 0u;
     if (FIRST) {
@@ -2188,9 +2194,9 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     // the registers; 25 networks 960 -> 942 ms), all but the counts, which
     // every wave reads after the vote whether or not it rewrites.
     __shared__ MbDriveLds sh;
-    VoteList<8>& vl = sh.vl;
+    VoteList<MB_DRIVE_COLS>& vl = sh.vl;
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
-    const uint32_t colA = blockIdx.x * 8u + wave * 2u;  // 8 columns per workgroup
+    const uint32_t colA = blockIdx.x * MB_DRIVE_COLS + wave * 2u;
     const uint32_t col = colA + h, ccol = min(col, d.ncol - 1u);
     uint32_t* const s0 = d.spk[0];  // no dynamic index into the local copy
     uint32_t* const s1 = d.spk[1];
@@ -2216,13 +2222,14 @@ __global__ __launch_bounds__(256) void k_mb_drive(MultiDev m, Params P, uint32_t
     const uint8_t* f = stage_fired(d, fired, FUSE, tab, PL.inCols);
     if (FUSE) {
         if (col < d.ncol && learns(PL, di.prev, vi.marg))
-            vote_column<8>(d, PL, col, di.prev, di.gl, vi, h, reinterpret_cast<uint2*>(d.cntLearn), vl);
+            vote_column<MB_DRIVE_COLS>(d, PL, col, di.prev, di.gl, vi, h, reinterpret_cast<uint2*>(d.cntLearn), vl);
         __syncthreads();
         if (vl.nl != 0u) {  // workgroup-uniform
             // two lines per half-wave and round: 55 registers instead of 85 with four, and a many-network
             // launch is bound by how many of its workgroups are resident (measured: 25 networks 968 ms
             // against 1006 ms; eight lines, 145 registers: 1261 ms)
-            rewrite_voted<8, 2>(d, PL, (t & 1u) ? s0 : s1, vl, wave, 4u, blockIdx.x * 8u * NSLOT);
+            rewrite_voted<MB_DRIVE_COLS, 2>(d, PL, (t & 1u) ? s0 : s1, vl, wave, MB_DRIVE_COLS / 2u,
+                                            blockIdx.x * MB_DRIVE_COLS * NSLOT);
             __syncthreads();  // the rewritten lines are read below, by any wave of the workgroup
         }
     }
@@ -2645,22 +2652,23 @@ hipError_t launch_multi_begin(const MultiDev& m, uint32_t ncol, uint32_t nstats,
 }
 hipError_t launch_multi_drive(const MultiDev& m, const Params& P, uint32_t t, uint32_t ncol, hipStream_t s,
                               const Params* PL, const Params* P0) {
-    const dim3 g((ncol + 7u) / 8u, m.nnet);
+    const dim3 g((ncol + MB_DRIVE_COLS - 1u) / MB_DRIVE_COLS, m.nnet);
+    constexpr uint32_t NT = 32u * MB_DRIVE_COLS;
     const uint32_t lds = fired_lds_bytes(ncol);
     if (P0) {
-        if (P.mode == 0u) k_mb_drive<0, false, true><<<g, 256, lds, s>>>(m, P, t, *P0);
-        else if (P.mode == 1u) k_mb_drive<1, false, true><<<g, 256, lds, s>>>(m, P, t, *P0);
-        else k_mb_drive<2, false, true><<<g, 256, lds, s>>>(m, P, t, *P0);
+        if (P.mode == 0u) k_mb_drive<0, false, true><<<g, NT, lds, s>>>(m, P, t, *P0);
+        else if (P.mode == 1u) k_mb_drive<1, false, true><<<g, NT, lds, s>>>(m, P, t, *P0);
+        else k_mb_drive<2, false, true><<<g, NT, lds, s>>>(m, P, t, *P0);
         return hipGetLastError();
     }
     if (PL) {
-        if (P.mode == 0u) k_mb_drive<0, true><<<g, 256, lds, s>>>(m, P, t, *PL);
-        else if (P.mode == 1u) k_mb_drive<1, true><<<g, 256, lds, s>>>(m, P, t, *PL);
-        else k_mb_drive<2, true><<<g, 256, lds, s>>>(m, P, t, *PL);
+        if (P.mode == 0u) k_mb_drive<0, true><<<g, NT, lds, s>>>(m, P, t, *PL);
+        else if (P.mode == 1u) k_mb_drive<1, true><<<g, NT, lds, s>>>(m, P, t, *PL);
+        else k_mb_drive<2, true><<<g, NT, lds, s>>>(m, P, t, *PL);
     } else {
-        if (P.mode == 0u) k_mb_drive<0, false><<<g, 256, lds, s>>>(m, P, t, P);
-        else if (P.mode == 1u) k_mb_drive<1, false><<<g, 256, lds, s>>>(m, P, t, P);
-        else k_mb_drive<2, false><<<g, 256, lds, s>>>(m, P, t, P);
+        if (P.mode == 0u) k_mb_drive<0, false><<<g, NT, lds, s>>>(m, P, t, P);
+        else if (P.mode == 1u) k_mb_drive<1, false><<<g, NT, lds, s>>>(m, P, t, P);
+        else k_mb_drive<2, false><<<g, NT, lds, s>>>(m, P, t, P);
     }
     return hipGetLastError();
 }
