@@ -33,7 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of this workload's kernels, measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE (means over all dispatches of the
 # Configuration C training run), each scaled by what the counter reads for a
-# 256 MiB device copy (x2.000 and x1.000); profiles/k_drive_c1_traffic.md.
+# 256 MiB device copy (x2.000 and x1.000); profiles/k_drive_c1_traffic.md
+# (taken with 4-column propagation and 16-column emit workgroups, the same
+# accesses per column).
 # propagation: the average launch, 6 with the previous timestep's plasticity
 # (9492173 B), 2 without (7940915 B) and 1 with the first timestep's work
 # (5001830 B) per presentation; last_emit: the last timestep's emit with its
