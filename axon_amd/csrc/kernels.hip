@@ -2161,8 +2161,19 @@ __global__ __launch_bounds__(256) void k_mb_begin(MultiDev m) {
 }
 
 // Columns per batched propagation workgroup (two per wave).  Measured on 25
-// networks: 8 columns 906 ms, 16 columns 966 ms, 32 columns 1083 ms.
+// networks: 8 columns 906 ms, 16 columns 955-966 ms, 32 columns 1073-1083 ms
+// (with the emits' columns on the propagation's XCD or not).
 constexpr uint32_t MB_DRIVE_COLS = 8;
+
+// First column of wave w of a batched emit workgroup b (8 columns, two per
+// wave): the emit workgroups on XCD b % 8 take the columns that the
+// propagation workgroups on that XCD ran (MB_DRIVE_COLS columns each, split
+// into parts of 8).
+__device__ __forceinline__ uint32_t mb_emit_col(uint32_t b, uint32_t w) {
+    constexpr uint32_t H = MB_DRIVE_COLS / 8u;  // emit workgroups per propagation workgroup
+    const uint32_t r = b >> 3, j = (b & 7u) + 8u * (r / H);
+    return MB_DRIVE_COLS * j + (r % H) * 8u + 2u * w;
+}
 union MbDriveLds {
     VoteList<MB_DRIVE_COLS> vl;
     uint32_t items[MB_DRIVE_COLS / 2][2 * ITEMS_PER_COL];
@@ -2294,7 +2305,7 @@ __global__ __launch_bounds__(256) void k_mb_emit(MultiDev m, Params P, uint32_t 
     if (d.active == 0xFFFFFFFFu) return;
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
-    const uint32_t colA = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u;  // 8 columns per workgroup
+    const uint32_t colA = mb_emit_col(blockIdx.x, threadIdx.x >> 6);  // 8 columns per workgroup
     if (colA >= d.ncol) return;
     (void)emit_pair<LEARN>(d, P, (t & 1u) ? s0 : s1, colA);
 }
@@ -2314,7 +2325,7 @@ __global__ __launch_bounds__(256) void k_mb_emit_learn(MultiDev m, Params P, uin
     uint32_t* const s0 = d.spk[0];
     uint32_t* const s1 = d.spk[1];
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id(), h = lane >> 5;
-    const uint32_t colA = blockIdx.x * 8u + wave * 2u;  // 8 columns per workgroup
+    const uint32_t colA = mb_emit_col(blockIdx.x, wave);  // 8 columns per workgroup
     const uint32_t col = colA + h, ccol = min(col, d.ncol - 1u);
     const uint32_t gl = d.conn[ccol * NSLOT + (lane & 31u)];
     const VoteIn vi = vote_inputs(d, ccol, reinterpret_cast<const uint2*>(d.cntLast));
@@ -2325,7 +2336,7 @@ __global__ __launch_bounds__(256) void k_mb_emit_learn(MultiDev m, Params P, uin
         vote_column<8>(d, P, col, bm.x, gl, vi, h, reinterpret_cast<uint2*>(d.cntLast), vl);
     __syncthreads();
     if (vl.nl != 0u)  // workgroup-uniform
-        rewrite_voted<8, 2>(d, P, (t & 1u) ? s1 : s0, vl, wave, 4u, blockIdx.x * 8u * NSLOT);
+        rewrite_voted<8, 2>(d, P, (t & 1u) ? s1 : s0, vl, wave, 4u, mb_emit_col(blockIdx.x, 0u) * NSLOT);
 }
 
 // The first timestep of a presentation for many networks (the batched form
@@ -2343,7 +2354,7 @@ __global__ __launch_bounds__(256) void k_mb_emit_first(MultiDev m, Params P) {
     if (p == 0xFFFFFFFFu) return;
     const NetDev d = m.nets[net];  // its `active` may be the last round's; not read here
     const uint32_t lane = lane_id(), h = lane >> 5, n = lane & 31u;
-    const uint32_t col = blockIdx.x * 8u + (threadIdx.x >> 6) * 2u + h;  // 8 columns per workgroup
+    const uint32_t col = mb_emit_col(blockIdx.x, threadIdx.x >> 6) + h;  // 8 columns per workgroup
     if (col >= d.ncol) return;
     const uint32_t sw = reinterpret_cast<const uint32_t*>(m.stimTabs[net])[(size_t)p * d.ncol + col];
     const uint32_t b = (P.clamp != 0u && col < P.inCols) ? sw : 0u;
