@@ -2249,8 +2249,12 @@ __global__ __launch_bounds__(32 * MB_DRIVE_COLS) void k_mb_drive(MultiDev m, Par
 }
 
 // 256 threads per network: this launch is a latency chain of a few barriers,
-// cheaper with 4 waves than with 16 (measured), and it leaves the device to
-// the other stream groups' kernels
+// cheaper with 4 waves than with 16 or 2, and it leaves the device to the
+// other stream groups' kernels.  Measured on 25 networks: 256 threads 906 ms,
+// 1024 threads 947 ms, 128 threads 918 ms.  Without this launch, the emit
+// gating each column by counting the drives above it (gate_by_count, as for
+// one network) took 1150 ms: each 8-column workgroup then reads its
+// network's whole drive vector.
 __global__ __launch_bounds__(256) void k_mb_topk(MultiDev m, Params P) {
     const uint32_t net = blockIdx.x;
     const NetDev d = mb_net(m, net);
